@@ -1,0 +1,172 @@
+// Internal declarations of libhypre_mi355x.so (MI355X / gfx950, single rank).
+//
+// Everything the solver touches lives in HBM: CSR operators with int32 row
+// pointers / columns and FP64 values (optional FP32 value copies for the
+// mixed-precision V-cycle), FP64 level vectors.  One library stream orders all
+// work; host code only sees scalars (dot products, level sizes, counters).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+extern "C" {
+#include "HYPRE.h"
+#include "hypre_mi355x.h"
+}
+
+namespace hx {
+
+// ------------------------------------------------------------------ errors
+// hypre's sticky global error bitmask (src/utilities/error.c).
+extern int g_error_flag;
+extern int g_error_arg;
+
+struct Error : public std::runtime_error
+{
+   int code;
+   Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_error(int code, const char *msg);
+[[noreturn]] void fail(int code, const std::string &msg);
+
+#define HX_HIP(call)                                                                   \
+   do {                                                                                \
+      hipError_t _e = (call);                                                          \
+      if (_e != hipSuccess) {                                                          \
+         ::hx::fail(HYPRE_ERROR_GENERIC, std::string("HIP error ") +                   \
+                    hipGetErrorString(_e) + " at " __FILE__ ":" + std::to_string(__LINE__)); \
+      }                                                                                \
+   } while (0)
+
+#define HX_LAUNCH_CHECK() HX_HIP(hipGetLastError())
+
+// API wrapper: runs body, converts exceptions into the error flag.
+template <class F>
+inline int guard(F &&f)
+{
+   try {
+      f();
+   } catch (const Error &e) {
+      set_error(e.code, e.what());
+   } catch (const std::bad_alloc &) {
+      set_error(HYPRE_ERROR_MEMORY, "host allocation failed");
+   } catch (const std::exception &e) {
+      set_error(HYPRE_ERROR_GENERIC, e.what());
+   }
+   return g_error_flag;
+}
+
+// ------------------------------------------------------------------ runtime
+void        ensure_init();
+hipStream_t stream();
+void        sync();                       // hipStreamSynchronize(stream())
+void       *dalloc(size_t bytes);         // caching device allocator
+void        dfree(void *p);
+void        release_cache();
+void       *pinned_scratch(size_t bytes); // host-pinned staging for scalars
+int        *device_flags();               // 256 ints of device status words
+
+template <class T>
+inline T *dalloc_n(size_t n) { return static_cast<T *>(dalloc(n * sizeof(T) + 16)); }
+
+void h2d(void *d, const void *h, size_t bytes);
+void d2h(void *h, const void *d, size_t bytes);
+void d2d(void *dst, const void *src, size_t bytes);
+void dzero(void *d, size_t bytes);
+
+// exclusive scan of n ints -> out[0..n], out[n] = total (in may alias out)
+long long exclusive_scan(const int *in, int *out, int n);
+// reductions returning to host
+long long reduce_sum_int(const int *in, int n);
+int       reduce_max_int(const int *in, int n);
+
+// ------------------------------------------------------------------ timing
+bool timing_on();
+struct TimedLaunch
+{
+   int   slot = -1;
+   TimedLaunch(const char *cls, double bytes);
+   ~TimedLaunch();
+};
+
+// ------------------------------------------------------------------ CSR
+struct Csr
+{
+   int       nrows = 0, ncols = 0;
+   long long nnz = 0;
+   int      *ia = nullptr, *ja = nullptr;
+   double   *a = nullptr;
+   float    *a32 = nullptr;
+   void      free_all();
+   void      alloc(int n, int nc, long long nz, bool values = true);
+};
+
+void csr_copy(const Csr &src, Csr &dst);
+void csr_make_fp32(Csr &A);
+void csr_to_host(const Csr &A, std::vector<int> &ia, std::vector<int> &ja, std::vector<double> &a);
+void csr_from_host(Csr &A, int n, int ncols, const int *ia, const int *ja, const double *a);
+
+// ------------------------------------------------------------------ solve kernels
+// y = alpha*A*x + beta*b  (b may alias y; b ignored when beta == 0)
+void spmv(const Csr &A, double alpha, const double *x, double beta, const double *b, double *y,
+          const char *cls = "spmv");
+void spmv32(const Csr &A, float alpha, const float *x, float beta, const float *b, float *y,
+            const char *cls = "spmv");
+// s = A p and part[] = per-block partial sums of s.*p (deterministic)
+void spmv_dot(const Csr &A, const double *p, double *s, double *dot_out_dev, const char *cls);
+// l1-Jacobi: u_out = u + (w*f - w*A*u)./l1 ; zero_guess: u_out = (w*f)./l1
+void relax_l1(const Csr &A, const double *f, const double *u, const double *l1, double w,
+              double *u_out, bool zero_guess, const char *cls);
+void relax_l1_32(const Csr &A, const float *f, const float *u, const float *l1, float w,
+                 float *u_out, bool zero_guess, const char *cls);
+// weighted Jacobi (relax type 0): u_out = (1-w) u + w (f - sum_{j!=i} a_ij u_j)/a_ii
+void relax_jacobi(const Csr &A, const double *f, const double *u, double w, double *u_out);
+void relax_jacobi_32(const Csr &A, const float *f, const float *u, float w, float *u_out);
+// BLAS-1
+double dot(int n, const double *x, const double *y);
+void   dot_dev(int n, const double *x, const double *y, double *out_dev);
+void   dot2_dev(int n, const double *r, const double *s, double *out2_dev); // <r,s>, <r,r>
+void   axpy(int n, double alpha, const double *x, double *y);               // y += alpha x
+void   scale(int n, double alpha, double *x);
+void   fill(int n, double v, double *x);
+void   fill32(int n, float v, float *x);
+void   copy(int n, const double *x, double *y);
+void   pcg_update_xr(int n, const double *alpha_dev, const double *p, const double *s, double *x,
+                     double *r);                                            // x += a p; r -= a s
+void   pcg_update_p(int n, const double *beta_dev, const double *s, double *p); // p = beta p + s
+void   d2f(int n, const double *x, float *y);
+void   f2d(int n, const float *x, double *y);
+void   scalar_div(const double *num, const double *den, double *out);       // device scalars
+// coarsest dense solve, column-major matrix (relax type 9)
+void   gselim_solve(int n, const double *Amat_dev, const double *f, double *u);
+void   gselim_solve32(int n, const double *Amat_dev, const float *f, float *u);
+
+// ------------------------------------------------------------------ setup kernels
+// strength pattern (S.a unused), sequential row order kept
+void create_s(const Csr &A, double theta, double max_row_sum, Csr &S);
+// l1 norms option 1 of hypre_ParCSRComputeL1Norms
+void l1_norms(const Csr &A, double *l1);
+// PMIS: measure (optional input: given measure on device, else Park-Miller
+// stream seed 2747), CF out; returns number of rounds
+int  pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_keep_dev);
+// fine->coarse map; returns n_coarse
+int  coarse_map(int n, const int *CF, int *f2c);
+// ext+i interpolation + truncation (CF: -3 -> -1 on exit)
+void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
+                  double trunc_factor, int max_elmts, Csr &P);
+void transpose(const Csr &P, Csr &R);
+void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac);
+// dense column-major copy of a small CSR matrix (n <= 64)
+void csr_to_dense_colmajor(const Csr &A, double *M_dev);
+
+// Park-Miller helpers shared with the generators
+void pm_uniform(int n, int seed, double *out_dev);   // hypre_Rand() stream
+
+} // namespace hx

@@ -1,0 +1,560 @@
+// Solve-phase kernels: CSR SpMV (FP64 / FP32 values), l1-Jacobi and Jacobi
+// relaxation, deterministic dot products, fused PCG vector updates and the
+// coarsest-grid Gaussian elimination.
+//
+// SpMV layout: K lanes of a wave64 share one row (K = 4..64 picked from the
+// average row length), loads of ja/a are contiguous per row, partial sums are
+// combined with lane shuffles.  Each block owns a contiguous slice of rows and
+// block ids are remapped so that every XCD (blocks are dealt round-robin to the
+// 8 XCDs) streams one contiguous eighth of the matrix and its x-gathers stay
+// in that XCD's L2.
+#include "hx_internal.h"
+
+#include <algorithm>
+
+namespace hx {
+
+constexpr int BS = 256;
+
+__device__ __forceinline__ int xcd_block(int bid, int nb)
+{
+   // bijection on [0, nb) when nb % 8 == 0: XCD (bid % 8) gets a contiguous range
+   return (nb & 7) ? bid : (bid & 7) * (nb >> 3) + (bid >> 3);
+}
+
+static int pick_k(long long nnz, int n)
+{
+   double avg = n > 0 ? (double) nnz / n : 1.0;
+   if (avg <= 4.5) { return 4; }
+   if (avg <= 9.0) { return 8; }
+   if (avg <= 18.0) { return 16; }
+   if (avg <= 40.0) { return 32; }
+   return 64;
+}
+
+static int grid_for_rows(int n, int K)
+{
+   long long groups = ((long long) n + 0) ;
+   long long blocks = (groups * K + BS - 1) / BS;
+   blocks = std::min<long long>(blocks, 8192);
+   blocks = std::max<long long>(blocks, 1);
+   if (blocks >= 64) { blocks = (blocks + 7) / 8 * 8; }
+   return (int) blocks;
+}
+
+template <int K, typename V, typename X>
+__device__ __forceinline__ X row_dot(int s, int e, int lane, const int *__restrict__ ja,
+                                     const V *__restrict__ a, const X *__restrict__ x)
+{
+   X t = 0;
+   for (int j = s + lane; j < e; j += K) { t += (X) a[j] * x[ja[j]]; }
+#pragma unroll
+   for (int off = K / 2; off > 0; off >>= 1) { t += __shfl_xor(t, off, K); }
+   return t;
+}
+
+// ---------------------------------------------------------------- SpMV
+template <int K, typename V, typename X, bool DOT>
+__global__ __launch_bounds__(BS) void k_spmv(int n, const int *__restrict__ ia,
+                                             const int *__restrict__ ja, const V *__restrict__ a,
+                                             const X *__restrict__ x, X alpha, X beta,
+                                             const X *b, X *y, double *__restrict__ part)
+{
+   constexpr int G = BS / K;
+   const int nb = gridDim.x;
+   const int blk = xcd_block(blockIdx.x, nb);
+   const int rows_per_block = (n + nb - 1) / nb;
+   const int r0 = blk * rows_per_block;
+   const int r1 = min(n, r0 + rows_per_block);
+   const int lane = threadIdx.x & (K - 1);
+   const int g = threadIdx.x / K;
+   double acc = 0.0;
+   for (int row = r0 + g; row < r1; row += G) {
+      X t = row_dot<K, V, X>(ia[row], ia[row + 1], lane, ja, a, x);
+      if (lane == 0) {
+         X v = (beta == (X) 0) ? alpha * t : alpha * t + beta * b[row];
+         y[row] = v;
+         if (DOT) { acc += (double) v * (double) x[row]; }
+      }
+   }
+   if (DOT) {
+      __shared__ double sh[BS];
+      sh[threadIdx.x] = acc;
+      __syncthreads();
+      for (int off = BS / 2; off > 0; off >>= 1) {
+         if (threadIdx.x < off) { sh[threadIdx.x] += sh[threadIdx.x + off]; }
+         __syncthreads();
+      }
+      if (threadIdx.x == 0) { part[blk] = sh[0]; }
+   }
+}
+
+__global__ __launch_bounds__(BS) void k_reduce_parts(const double *__restrict__ part, int np,
+                                                     double *__restrict__ out)
+{
+   __shared__ double sh[BS];
+   double s = 0.0;
+   for (int i = threadIdx.x; i < np; i += BS) { s += part[i]; }
+   sh[threadIdx.x] = s;
+   __syncthreads();
+   for (int off = BS / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) { sh[threadIdx.x] += sh[threadIdx.x + off]; }
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) { *out = sh[0]; }
+}
+
+static double *g_part = nullptr;
+static double *parts()
+{
+   if (!g_part) { g_part = dalloc_n<double>(8192 * 2); }
+   return g_part;
+}
+
+template <typename V, typename X, bool DOT>
+static void launch_spmv(const Csr &A, const V *vals, X alpha, const X *x, X beta, const X *b,
+                        X *y, double *part, const char *cls, double vbytes)
+{
+   if (A.nrows <= 0) { return; }
+   int K = pick_k(A.nnz, A.nrows);
+   int nb = grid_for_rows(A.nrows, K);
+   double bytes = (double) A.nnz * (4.0 + vbytes) + 4.0 * (A.nrows + 1) +
+                  (double) sizeof(X) * (A.ncols + A.nrows) + (beta != (X) 0 ? sizeof(X) * (double) A.nrows : 0.0);
+   TimedLaunch tl(cls, bytes);
+   hipStream_t s = stream();
+   switch (K) {
+      case 4: k_spmv<4, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, x, alpha, beta, b, y, part); break;
+      case 8: k_spmv<8, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, x, alpha, beta, b, y, part); break;
+      case 16: k_spmv<16, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, x, alpha, beta, b, y, part); break;
+      case 32: k_spmv<32, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, x, alpha, beta, b, y, part); break;
+      default: k_spmv<64, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, x, alpha, beta, b, y, part); break;
+   }
+   HX_LAUNCH_CHECK();
+}
+
+void spmv(const Csr &A, double alpha, const double *x, double beta, const double *b, double *y,
+          const char *cls)
+{
+   launch_spmv<double, double, false>(A, A.a, alpha, x, beta, b, y, nullptr, cls, 8.0);
+}
+
+void spmv32(const Csr &A, float alpha, const float *x, float beta, const float *b, float *y,
+            const char *cls)
+{
+   if (!A.a32) { fail(HYPRE_ERROR_GENERIC, "FP32 values missing"); }
+   launch_spmv<float, float, false>(A, A.a32, alpha, x, beta, b, y, nullptr, cls, 4.0);
+}
+
+void spmv_dot(const Csr &A, const double *p, double *s, double *dot_out_dev, const char *cls)
+{
+   int K = pick_k(A.nnz, A.nrows);
+   int nb = grid_for_rows(A.nrows, K);
+   launch_spmv<double, double, true>(A, A.a, 1.0, p, 0.0, nullptr, s, parts(), cls, 8.0);
+   k_reduce_parts<<<1, BS, 0, stream()>>>(parts(), nb, dot_out_dev);
+   HX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- relaxation
+// l1-Jacobi (src/parcsr_ls/par_relax.c:1178-1240): v = w*f - w*(A u); u_out = u + v/l1
+template <int K, typename V, typename X, bool ZERO>
+__global__ __launch_bounds__(BS) void k_relax_l1(int n, const int *__restrict__ ia,
+                                                 const int *__restrict__ ja,
+                                                 const V *__restrict__ a, const X *__restrict__ f,
+                                                 const X *__restrict__ u, const X *__restrict__ l1,
+                                                 X w, X *__restrict__ uo)
+{
+   constexpr int G = BS / K;
+   const int nb = gridDim.x;
+   const int blk = xcd_block(blockIdx.x, nb);
+   const int rows_per_block = (n + nb - 1) / nb;
+   const int r0 = blk * rows_per_block;
+   const int r1 = min(n, r0 + rows_per_block);
+   const int lane = threadIdx.x & (K - 1);
+   const int g = threadIdx.x / K;
+   for (int row = r0 + g; row < r1; row += G) {
+      if (ZERO) {
+         if (lane == 0) { uo[row] = (X) 0 + (w * f[row]) / l1[row]; }
+      } else {
+         X t = row_dot<K, V, X>(ia[row], ia[row + 1], lane, ja, a, u);
+         if (lane == 0) {
+            X v = w * f[row] - w * t;
+            uo[row] = u[row] + v / l1[row];
+         }
+      }
+   }
+}
+
+// weighted Jacobi, relax type 0 (par_relax.c:180-320, Skip_diag = 1)
+template <int K, typename V, typename X>
+__global__ __launch_bounds__(BS) void k_relax_jac(int n, const int *__restrict__ ia,
+                                                  const int *__restrict__ ja,
+                                                  const V *__restrict__ a, const X *__restrict__ f,
+                                                  const X *__restrict__ u, X w,
+                                                  X *__restrict__ uo)
+{
+   constexpr int G = BS / K;
+   const int nb = gridDim.x;
+   const int blk = xcd_block(blockIdx.x, nb);
+   const int rows_per_block = (n + nb - 1) / nb;
+   const int r0 = blk * rows_per_block;
+   const int r1 = min(n, r0 + rows_per_block);
+   const int lane = threadIdx.x & (K - 1);
+   const int g = threadIdx.x / K;
+   for (int row = r0 + g; row < r1; row += G) {
+      int s = ia[row], e = ia[row + 1];
+      X t = row_dot<K, V, X>(s + 1, e, lane, ja, a, u);
+      if (lane == 0) {
+         X di = (X) a[s];
+         if (di != (X) 0) {
+            X res = f[row] - t;
+            uo[row] = ((X) 1 - w) * u[row] + w * res / di;
+         } else {
+            uo[row] = u[row];
+         }
+      }
+   }
+}
+
+template <typename V, typename X>
+static void launch_relax_l1(const Csr &A, const V *vals, const X *f, const X *u, const X *l1, X w,
+                            X *uo, bool zero, const char *cls, double vbytes)
+{
+   if (A.nrows <= 0) { return; }
+   int K = zero ? 64 : pick_k(A.nnz, A.nrows);
+   int nb = grid_for_rows(A.nrows, K);
+   double xs = sizeof(X);
+   double bytes = zero ? xs * 3.0 * A.nrows
+                       : (double) A.nnz * (4.0 + vbytes) + 4.0 * (A.nrows + 1) + xs * 4.0 * A.nrows;
+   TimedLaunch tl(cls, bytes);
+   hipStream_t s = stream();
+#define HX_RL(KK)                                                                                   \
+   if (zero) k_relax_l1<KK, V, X, true><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, f, u, l1, w, uo); \
+   else k_relax_l1<KK, V, X, false><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, f, u, l1, w, uo);
+   switch (K) {
+      case 4: HX_RL(4); break;
+      case 8: HX_RL(8); break;
+      case 16: HX_RL(16); break;
+      case 32: HX_RL(32); break;
+      default: HX_RL(64); break;
+   }
+#undef HX_RL
+   HX_LAUNCH_CHECK();
+}
+
+void relax_l1(const Csr &A, const double *f, const double *u, const double *l1, double w,
+              double *u_out, bool zero_guess, const char *cls)
+{
+   launch_relax_l1<double, double>(A, A.a, f, u, l1, w, u_out, zero_guess, cls, 8.0);
+}
+
+void relax_l1_32(const Csr &A, const float *f, const float *u, const float *l1, float w,
+                 float *u_out, bool zero_guess, const char *cls)
+{
+   launch_relax_l1<float, float>(A, A.a32, f, u, l1, w, u_out, zero_guess, cls, 4.0);
+}
+
+template <typename V, typename X>
+static void launch_relax_jac(const Csr &A, const V *vals, const X *f, const X *u, X w, X *uo)
+{
+   if (A.nrows <= 0) { return; }
+   int K = pick_k(A.nnz, A.nrows);
+   int nb = grid_for_rows(A.nrows, K);
+   hipStream_t s = stream();
+   switch (K) {
+      case 4: k_relax_jac<4, V, X><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, f, u, w, uo); break;
+      case 8: k_relax_jac<8, V, X><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, f, u, w, uo); break;
+      case 16: k_relax_jac<16, V, X><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, f, u, w, uo); break;
+      case 32: k_relax_jac<32, V, X><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, f, u, w, uo); break;
+      default: k_relax_jac<64, V, X><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, f, u, w, uo); break;
+   }
+   HX_LAUNCH_CHECK();
+}
+
+void relax_jacobi(const Csr &A, const double *f, const double *u, double w, double *u_out)
+{
+   launch_relax_jac<double, double>(A, A.a, f, u, w, u_out);
+}
+
+void relax_jacobi_32(const Csr &A, const float *f, const float *u, float w, float *u_out)
+{
+   launch_relax_jac<float, float>(A, A.a32, f, u, w, u_out);
+}
+
+// ---------------------------------------------------------------- BLAS-1
+static int blas_grid(int n)
+{
+   int nb = (int) std::min<long long>(((long long) n + BS * 4 - 1) / (BS * 4), 2048);
+   return std::max(nb, 1);
+}
+
+__global__ __launch_bounds__(BS) void k_dot_part(int n, const double *__restrict__ x,
+                                                 const double *__restrict__ y,
+                                                 double *__restrict__ part)
+{
+   __shared__ double sh[BS];
+   double s = 0.0;
+   for (long long i = (long long) blockIdx.x * BS + threadIdx.x; i < n; i += (long long) gridDim.x * BS) {
+      s += x[i] * y[i];
+   }
+   sh[threadIdx.x] = s;
+   __syncthreads();
+   for (int off = BS / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) { sh[threadIdx.x] += sh[threadIdx.x + off]; }
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) { part[blockIdx.x] = sh[0]; }
+}
+
+__global__ __launch_bounds__(BS) void k_dot2_part(int n, const double *__restrict__ r,
+                                                  const double *__restrict__ s,
+                                                  double *__restrict__ part)
+{
+   __shared__ double sh[2][BS];
+   double a = 0.0, b = 0.0;
+   for (long long i = (long long) blockIdx.x * BS + threadIdx.x; i < n; i += (long long) gridDim.x * BS) {
+      double ri = r[i];
+      a += ri * s[i];
+      b += ri * ri;
+   }
+   sh[0][threadIdx.x] = a;
+   sh[1][threadIdx.x] = b;
+   __syncthreads();
+   for (int off = BS / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) {
+         sh[0][threadIdx.x] += sh[0][threadIdx.x + off];
+         sh[1][threadIdx.x] += sh[1][threadIdx.x + off];
+      }
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) {
+      part[blockIdx.x] = sh[0][0];
+      part[8192 + blockIdx.x] = sh[1][0];
+   }
+}
+
+void dot_dev(int n, const double *x, const double *y, double *out_dev)
+{
+   int nb = blas_grid(n);
+   TimedLaunch tl("blas", 16.0 * n);
+   k_dot_part<<<nb, BS, 0, stream()>>>(n, x, y, parts());
+   k_reduce_parts<<<1, BS, 0, stream()>>>(parts(), nb, out_dev);
+   HX_LAUNCH_CHECK();
+}
+
+void dot2_dev(int n, const double *r, const double *s, double *out2_dev)
+{
+   int nb = blas_grid(n);
+   TimedLaunch tl("blas", 16.0 * n);
+   k_dot2_part<<<nb, BS, 0, stream()>>>(n, r, s, parts());
+   k_reduce_parts<<<1, BS, 0, stream()>>>(parts(), nb, out2_dev);
+   k_reduce_parts<<<1, BS, 0, stream()>>>(parts() + 8192, nb, out2_dev + 1);
+   HX_LAUNCH_CHECK();
+}
+
+double dot(int n, const double *x, const double *y)
+{
+   double *d = reinterpret_cast<double *>(device_flags() + 32);
+   dot_dev(n, x, y, d);
+   double *h = static_cast<double *>(pinned_scratch(64));
+   HX_HIP(hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, stream()));
+   sync();
+   return h[0];
+}
+
+__global__ void k_axpy(int n, double alpha, const double *__restrict__ x, double *__restrict__ y)
+{
+   for (long long i = (long long) blockIdx.x * BS + threadIdx.x; i < n; i += (long long) gridDim.x * BS) {
+      y[i] += alpha * x[i];
+   }
+}
+__global__ void k_scale(int n, double alpha, double *__restrict__ x)
+{
+   for (long long i = (long long) blockIdx.x * BS + threadIdx.x; i < n; i += (long long) gridDim.x * BS) {
+      x[i] *= alpha;
+   }
+}
+template <typename T>
+__global__ void k_fill(int n, T v, T *__restrict__ x)
+{
+   for (long long i = (long long) blockIdx.x * BS + threadIdx.x; i < n; i += (long long) gridDim.x * BS) {
+      x[i] = v;
+   }
+}
+__global__ void k_copy(int n, const double *__restrict__ x, double *__restrict__ y)
+{
+   for (long long i = (long long) blockIdx.x * BS + threadIdx.x; i < n; i += (long long) gridDim.x * BS) {
+      y[i] = x[i];
+   }
+}
+// x += alpha p ; r += (-alpha) s   (two hypre_ParKrylovAxpy calls, pcg.c)
+__global__ void k_pcg_xr(int n, const double *__restrict__ ad, const double *__restrict__ p,
+                         const double *__restrict__ s, double *__restrict__ x,
+                         double *__restrict__ r)
+{
+   double alpha = *ad, malpha = -alpha;
+   for (long long i = (long long) blockIdx.x * BS + threadIdx.x; i < n; i += (long long) gridDim.x * BS) {
+      x[i] += alpha * p[i];
+      r[i] += malpha * s[i];
+   }
+}
+// p = beta p ; p += 1.0 s   (ScaleVector + Axpy, pcg.c)
+__global__ void k_pcg_p(int n, const double *__restrict__ bd, const double *__restrict__ s,
+                        double *__restrict__ p)
+{
+   double beta = *bd;
+   for (long long i = (long long) blockIdx.x * BS + threadIdx.x; i < n; i += (long long) gridDim.x * BS) {
+      double t = beta * p[i];
+      p[i] = t + s[i];
+   }
+}
+__global__ void k_d2f(int n, const double *__restrict__ x, float *__restrict__ y)
+{
+   for (long long i = (long long) blockIdx.x * BS + threadIdx.x; i < n; i += (long long) gridDim.x * BS) {
+      y[i] = (float) x[i];
+   }
+}
+__global__ void k_f2d(int n, const float *__restrict__ x, double *__restrict__ y)
+{
+   for (long long i = (long long) blockIdx.x * BS + threadIdx.x; i < n; i += (long long) gridDim.x * BS) {
+      y[i] = (double) x[i];
+   }
+}
+__global__ void k_sdiv(const double *num, const double *den, double *out) { *out = *num / *den; }
+
+void axpy(int n, double alpha, const double *x, double *y)
+{
+   if (n <= 0) { return; }
+   k_axpy<<<blas_grid(n), BS, 0, stream()>>>(n, alpha, x, y);
+   HX_LAUNCH_CHECK();
+}
+void scale(int n, double alpha, double *x)
+{
+   if (n <= 0) { return; }
+   k_scale<<<blas_grid(n), BS, 0, stream()>>>(n, alpha, x);
+   HX_LAUNCH_CHECK();
+}
+void fill(int n, double v, double *x)
+{
+   if (n <= 0) { return; }
+   k_fill<double><<<blas_grid(n), BS, 0, stream()>>>(n, v, x);
+   HX_LAUNCH_CHECK();
+}
+void fill32(int n, float v, float *x)
+{
+   if (n <= 0) { return; }
+   k_fill<float><<<blas_grid(n), BS, 0, stream()>>>(n, v, x);
+   HX_LAUNCH_CHECK();
+}
+void copy(int n, const double *x, double *y)
+{
+   if (n <= 0) { return; }
+   k_copy<<<blas_grid(n), BS, 0, stream()>>>(n, x, y);
+   HX_LAUNCH_CHECK();
+}
+void pcg_update_xr(int n, const double *alpha_dev, const double *p, const double *s, double *x,
+                   double *r)
+{
+   TimedLaunch tl("blas", 40.0 * n);
+   k_pcg_xr<<<blas_grid(n), BS, 0, stream()>>>(n, alpha_dev, p, s, x, r);
+   HX_LAUNCH_CHECK();
+}
+void pcg_update_p(int n, const double *beta_dev, const double *s, double *p)
+{
+   TimedLaunch tl("blas", 24.0 * n);
+   k_pcg_p<<<blas_grid(n), BS, 0, stream()>>>(n, beta_dev, s, p);
+   HX_LAUNCH_CHECK();
+}
+void d2f(int n, const double *x, float *y)
+{
+   if (n <= 0) { return; }
+   k_d2f<<<blas_grid(n), BS, 0, stream()>>>(n, x, y);
+   HX_LAUNCH_CHECK();
+}
+void f2d(int n, const float *x, double *y)
+{
+   if (n <= 0) { return; }
+   k_f2d<<<blas_grid(n), BS, 0, stream()>>>(n, x, y);
+   HX_LAUNCH_CHECK();
+}
+void scalar_div(const double *num, const double *den, double *out)
+{
+   k_sdiv<<<1, 1, 0, stream()>>>(num, den, out);
+   HX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- coarsest solve
+// hypre_gselim (src/utilities/gselim.h) on the column-major matrix of
+// par_gauss_elim.c:312-316, one block: the updates of one elimination step
+// are independent element-wise and are spread over the threads; each element
+// sees the same operation sequence as the serial loop.
+constexpr int GE_MAX = 64;
+
+template <typename X>
+__global__ __launch_bounds__(BS) void k_gselim(int n, const double *__restrict__ Am,
+                                               const X *__restrict__ f, X *__restrict__ u)
+{
+   __shared__ double A[GE_MAX * GE_MAX];
+   __shared__ double x[GE_MAX];
+   for (int i = threadIdx.x; i < n * n; i += BS) { A[i] = Am[i]; }
+   for (int i = threadIdx.x; i < n; i += BS) { x[i] = (double) f[i]; }
+   __syncthreads();
+   if (n == 1) {
+      if (threadIdx.x == 0) {
+         if (A[0] != 0.0) { x[0] = x[0] / A[0]; }
+         u[0] = (X) x[0];
+      }
+      return;
+   }
+   for (int k = 0; k < n - 1; k++) {
+      double akk = A[k * n + k];
+      if (akk != 0.0) {
+         double divA = 1.0 / akk;
+         int cnt = (n - k - 1) * (n - k);     // rows j>k, columns m in (k, n) plus the rhs
+         for (int t = threadIdx.x; t < cnt; t += BS) {
+            int j = k + 1 + t / (n - k);
+            int m = k + 1 + t % (n - k);      // m == n encodes the right-hand side
+            double ajk = A[j * n + k];
+            if (ajk != 0.0) {
+               double factor = ajk * divA;
+               if (m < n) { A[j * n + m] -= factor * A[k * n + m]; }
+               else { x[j] -= factor * x[k]; }
+            }
+         }
+      }
+      __syncthreads();
+   }
+   for (int k = n - 1; k > 0; --k) {
+      __syncthreads();
+      double akk = A[k * n + k];
+      if (akk != 0.0) {
+         if (threadIdx.x == 0) { x[k] /= akk; }
+         __syncthreads();
+         double xk = x[k];
+         for (int j = threadIdx.x; j < k; j += BS) {
+            if (A[j * n + k] != 0.0) { x[j] -= xk * A[j * n + k]; }
+         }
+      }
+   }
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      if (A[0] != 0.0) { x[0] /= A[0]; }
+   }
+   __syncthreads();
+   for (int i = threadIdx.x; i < n; i += BS) { u[i] = (X) x[i]; }
+}
+
+void gselim_solve(int n, const double *Amat_dev, const double *f, double *u)
+{
+   if (n > GE_MAX) { fail(HYPRE_ERROR_GENERIC, "coarse grid too large for the dense solve"); }
+   k_gselim<double><<<1, BS, 0, stream()>>>(n, Amat_dev, f, u);
+   HX_LAUNCH_CHECK();
+}
+
+void gselim_solve32(int n, const double *Amat_dev, const float *f, float *u)
+{
+   if (n > GE_MAX) { fail(HYPRE_ERROR_GENERIC, "coarse grid too large for the dense solve"); }
+   k_gselim<float><<<1, BS, 0, stream()>>>(n, Amat_dev, f, u);
+   HX_LAUNCH_CHECK();
+}
+
+} // namespace hx

@@ -1,0 +1,114 @@
+/*
+ * HYPRE_parcsr_ls.h -- BoomerAMG and ParCSR-PCG, single rank, on MI355X.
+ *
+ * Entry points replace src/parcsr_ls/HYPRE_parcsr_amg.c and
+ * src/parcsr_ls/HYPRE_parcsr_pcg.c of hypre 3.1.0 (line numbers below).
+ * Supported option values are those of the hot path (PMIS coarsening 8/9,
+ * extended+i interpolation 6, l1-Jacobi 18 / Jacobi 0 smoothing, Gaussian
+ * elimination 9 on the coarsest grid, V-cycle); any other value returns
+ * HYPRE_ERROR_ARG and leaves the previous setting in place -- there is no
+ * silent fallback.
+ */
+#ifndef HYPRE_PARCSR_LS_HEADER
+#define HYPRE_PARCSR_LS_HEADER
+
+#include "HYPRE_utilities.h"
+#include "HYPRE_parcsr_mv.h"
+#include "HYPRE_krylov.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/parcsr_ls/HYPRE_parcsr_ls.h:50 */
+typedef HYPRE_Int (*HYPRE_PtrToParSolverFcn)(HYPRE_Solver, HYPRE_ParCSRMatrix,
+                                             HYPRE_ParVector, HYPRE_ParVector);
+
+/* ---------------- BoomerAMG (src/parcsr_ls/HYPRE_parcsr_amg.c) ---------- */
+HYPRE_Int HYPRE_BoomerAMGCreate(HYPRE_Solver *solver);                       /* :15 */
+HYPRE_Int HYPRE_BoomerAMGDestroy(HYPRE_Solver solver);                       /* :32 */
+HYPRE_Int HYPRE_BoomerAMGSetup(HYPRE_Solver solver, HYPRE_ParCSRMatrix A,
+                               HYPRE_ParVector b, HYPRE_ParVector x);        /* :42 */
+HYPRE_Int HYPRE_BoomerAMGSolve(HYPRE_Solver solver, HYPRE_ParCSRMatrix A,
+                               HYPRE_ParVector b, HYPRE_ParVector x);        /* :64 */
+HYPRE_Int HYPRE_BoomerAMGSetMaxLevels(HYPRE_Solver solver, HYPRE_Int max_levels);      /* :165 */
+HYPRE_Int HYPRE_BoomerAMGGetMaxLevels(HYPRE_Solver solver, HYPRE_Int *max_levels);
+HYPRE_Int HYPRE_BoomerAMGSetMaxCoarseSize(HYPRE_Solver solver, HYPRE_Int size);        /* :183 */
+HYPRE_Int HYPRE_BoomerAMGSetMinCoarseSize(HYPRE_Solver solver, HYPRE_Int size);        /* :201 */
+HYPRE_Int HYPRE_BoomerAMGSetSeqThreshold(HYPRE_Solver solver, HYPRE_Int seq_threshold);/* :219 */
+HYPRE_Int HYPRE_BoomerAMGSetRedundant(HYPRE_Solver solver, HYPRE_Int redundant);      /* :237 */
+HYPRE_Int HYPRE_BoomerAMGSetStrongThreshold(HYPRE_Solver solver, HYPRE_Real strong_threshold); /* :273 */
+HYPRE_Int HYPRE_BoomerAMGGetStrongThreshold(HYPRE_Solver solver, HYPRE_Real *strong_threshold);
+HYPRE_Int HYPRE_BoomerAMGSetMaxRowSum(HYPRE_Solver solver, HYPRE_Real max_row_sum);   /* :334 */
+HYPRE_Int HYPRE_BoomerAMGSetTruncFactor(HYPRE_Solver solver, HYPRE_Real trunc_factor);/* :354 */
+HYPRE_Int HYPRE_BoomerAMGSetPMaxElmts(HYPRE_Solver solver, HYPRE_Int P_max_elmts);    /* :374 */
+HYPRE_Int HYPRE_BoomerAMGSetInterpType(HYPRE_Solver solver, HYPRE_Int interp_type);   /* :451 */
+HYPRE_Int HYPRE_BoomerAMGSetMaxIter(HYPRE_Solver solver, HYPRE_Int max_iter);         /* :484 */
+HYPRE_Int HYPRE_BoomerAMGSetCoarsenType(HYPRE_Solver solver, HYPRE_Int coarsen_type); /* :502 */
+HYPRE_Int HYPRE_BoomerAMGSetMeasureType(HYPRE_Solver solver, HYPRE_Int measure_type); /* :520 */
+HYPRE_Int HYPRE_BoomerAMGSetCycleType(HYPRE_Solver solver, HYPRE_Int cycle_type);     /* :562 */
+HYPRE_Int HYPRE_BoomerAMGSetTol(HYPRE_Solver solver, HYPRE_Real tol);                 /* :616 */
+HYPRE_Int HYPRE_BoomerAMGSetNumSweeps(HYPRE_Solver solver, HYPRE_Int num_sweeps);     /* :649 */
+HYPRE_Int HYPRE_BoomerAMGSetCycleNumSweeps(HYPRE_Solver solver, HYPRE_Int num_sweeps,
+                                           HYPRE_Int k);                             /* :660 */
+HYPRE_Int HYPRE_BoomerAMGSetRelaxType(HYPRE_Solver solver, HYPRE_Int relax_type);     /* :785 */
+HYPRE_Int HYPRE_BoomerAMGSetCycleRelaxType(HYPRE_Solver solver, HYPRE_Int relax_type,
+                                           HYPRE_Int k);                             /* :796 */
+HYPRE_Int HYPRE_BoomerAMGSetRelaxOrder(HYPRE_Solver solver, HYPRE_Int relax_order);   /* :814 */
+HYPRE_Int HYPRE_BoomerAMGSetRelaxWt(HYPRE_Solver solver, HYPRE_Real relax_wt);        /* :853 */
+HYPRE_Int HYPRE_BoomerAMGSetOuterWt(HYPRE_Solver solver, HYPRE_Real omega);           /* :887 */
+HYPRE_Int HYPRE_BoomerAMGSetLogging(HYPRE_Solver solver, HYPRE_Int logging);          /* :964 */
+HYPRE_Int HYPRE_BoomerAMGSetPrintLevel(HYPRE_Solver solver, HYPRE_Int print_level);   /* :1055 */
+HYPRE_Int HYPRE_BoomerAMGSetDebugFlag(HYPRE_Solver solver, HYPRE_Int debug_flag);     /* :1084 */
+HYPRE_Int HYPRE_BoomerAMGSetNumFunctions(HYPRE_Solver solver, HYPRE_Int num_functions);/* :1583 */
+HYPRE_Int HYPRE_BoomerAMGSetAggNumLevels(HYPRE_Solver solver, HYPRE_Int agg_num_levels);/* :1696 */
+HYPRE_Int HYPRE_BoomerAMGSetKeepTranspose(HYPRE_Solver solver, HYPRE_Int keepTranspose);
+HYPRE_Int HYPRE_BoomerAMGSetRAP2(HYPRE_Solver solver, HYPRE_Int rap2);
+HYPRE_Int HYPRE_BoomerAMGSetModuleRAP2(HYPRE_Solver solver, HYPRE_Int mod_rap2);
+HYPRE_Int HYPRE_BoomerAMGGetNumIterations(HYPRE_Solver solver, HYPRE_Int *num_iterations);    /* :1102 */
+HYPRE_Int HYPRE_BoomerAMGGetFinalRelativeResidualNorm(HYPRE_Solver solver, HYPRE_Real *rel_resid_norm); /* :1136 */
+HYPRE_Int HYPRE_BoomerAMGGetNumLevels(HYPRE_Solver solver, HYPRE_Int *num_levels);
+
+/* ---------------- ParCSR PCG (src/parcsr_ls/HYPRE_parcsr_pcg.c) --------- */
+HYPRE_Int HYPRE_ParCSRPCGCreate(MPI_Comm comm, HYPRE_Solver *solver);        /* :15 */
+HYPRE_Int HYPRE_ParCSRPCGDestroy(HYPRE_Solver solver);                       /* :46 */
+HYPRE_Int HYPRE_ParCSRPCGSetup(HYPRE_Solver solver, HYPRE_ParCSRMatrix A,
+                               HYPRE_ParVector b, HYPRE_ParVector x);        /* :56 */
+HYPRE_Int HYPRE_ParCSRPCGSolve(HYPRE_Solver solver, HYPRE_ParCSRMatrix A,
+                               HYPRE_ParVector b, HYPRE_ParVector x);        /* :72 */
+HYPRE_Int HYPRE_ParCSRPCGSetTol(HYPRE_Solver solver, HYPRE_Real tol);        /* :88 */
+HYPRE_Int HYPRE_ParCSRPCGSetAbsoluteTol(HYPRE_Solver solver, HYPRE_Real tol);/* :98 */
+HYPRE_Int HYPRE_ParCSRPCGSetMaxIter(HYPRE_Solver solver, HYPRE_Int max_iter);/* :109 */
+HYPRE_Int HYPRE_ParCSRPCGSetStopCrit(HYPRE_Solver solver, HYPRE_Int stop_crit); /* :120 */
+HYPRE_Int HYPRE_ParCSRPCGSetTwoNorm(HYPRE_Solver solver, HYPRE_Int two_norm);/* :131 */
+HYPRE_Int HYPRE_ParCSRPCGSetRelChange(HYPRE_Solver solver, HYPRE_Int rel_change); /* :142 */
+HYPRE_Int HYPRE_ParCSRPCGSetPrecond(HYPRE_Solver solver, HYPRE_PtrToParSolverFcn precond,
+                                    HYPRE_PtrToParSolverFcn precond_setup,
+                                    HYPRE_Solver precond_solver);            /* :153 */
+HYPRE_Int HYPRE_ParCSRPCGGetPrecond(HYPRE_Solver solver, HYPRE_Solver *precond_data);  /* :180 */
+HYPRE_Int HYPRE_ParCSRPCGSetPrintLevel(HYPRE_Solver solver, HYPRE_Int level);/* :192 */
+HYPRE_Int HYPRE_ParCSRPCGSetLogging(HYPRE_Solver solver, HYPRE_Int level);   /* :204 */
+HYPRE_Int HYPRE_ParCSRPCGGetNumIterations(HYPRE_Solver solver, HYPRE_Int *num_iterations); /* :215 */
+HYPRE_Int HYPRE_ParCSRPCGGetFinalRelativeResidualNorm(HYPRE_Solver solver, HYPRE_Real *norm); /* :226 */
+HYPRE_Int HYPRE_ParCSRDiagScaleSetup(HYPRE_Solver solver, HYPRE_ParCSRMatrix A,
+                                     HYPRE_ParVector y, HYPRE_ParVector x);
+HYPRE_Int HYPRE_ParCSRDiagScale(HYPRE_Solver solver, HYPRE_ParCSRMatrix HA,
+                                HYPRE_ParVector Hy, HYPRE_ParVector Hx);
+
+/* ---------------- test-problem generators --------------------------------
+ * src/parcsr_ls/par_laplace.c, par_laplace_27pt.c (declared at
+ * src/parcsr_ls/HYPRE_parcsr_ls.h:5447,5459).  Single rank: P=Q=R=1,
+ * p=q=r=0 are the only accepted decompositions.  The matrix is returned in
+ * HBM; destroy it with HYPRE_ParCSRMatrixDestroy. */
+HYPRE_ParCSRMatrix GenerateLaplacian(MPI_Comm comm, HYPRE_BigInt nx, HYPRE_BigInt ny,
+                                     HYPRE_BigInt nz, HYPRE_Int P, HYPRE_Int Q, HYPRE_Int R,
+                                     HYPRE_Int p, HYPRE_Int q, HYPRE_Int r, HYPRE_Real *value);
+HYPRE_ParCSRMatrix GenerateLaplacian27pt(MPI_Comm comm, HYPRE_BigInt nx, HYPRE_BigInt ny,
+                                         HYPRE_BigInt nz, HYPRE_Int P, HYPRE_Int Q, HYPRE_Int R,
+                                         HYPRE_Int p, HYPRE_Int q, HYPRE_Int r, HYPRE_Real *value);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

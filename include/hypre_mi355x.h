@@ -1,0 +1,133 @@
+/*
+ * hypre_mi355x.h -- extensions of libhypre_mi355x.so that have no
+ * counterpart in hypre's public API.  They serve three purposes:
+ *   1. the FP32 V-cycle inside FP64 PCG and the learned-parameter model,
+ *   2. inspection of the AMG hierarchy and of single setup kernels, so a
+ *      caller can compare them with a CPU implementation of the same steps,
+ *   3. kernel timing with HIP events and the problem generators used by the
+ *      benchmark workloads.
+ * All arrays passed in or out are HOST arrays unless a name says _dev.
+ */
+#ifndef HYPRE_MI355X_HEADER
+#define HYPRE_MI355X_HEADER
+
+#include "HYPRE_utilities.h"
+#include "HYPRE_parcsr_mv.h"
+#include "HYPRE_parcsr_ls.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- precision of the V-cycle: 0 = FP64 (default), 1 = FP32 operators and
+ * level vectors inside the FP64 outer iteration. */
+HYPRE_Int HYPRE_MI355X_BoomerAMGSetCyclePrecision(HYPRE_Solver solver, HYPRE_Int fp32);
+
+/* ---- per-level parameter overrides (used by the learned model): strong
+ * threshold and relaxation weight of level `level`; sweeps applies to both
+ * pre- and post-smoothing on that level. Negative level = all levels. */
+HYPRE_Int HYPRE_MI355X_BoomerAMGSetLevelStrongThreshold(HYPRE_Solver solver, HYPRE_Int level,
+                                                        HYPRE_Real theta);
+HYPRE_Int HYPRE_MI355X_BoomerAMGSetLevelRelaxWt(HYPRE_Solver solver, HYPRE_Int level,
+                                                HYPRE_Real omega);
+HYPRE_Int HYPRE_MI355X_BoomerAMGSetLevelNumSweeps(HYPRE_Solver solver, HYPRE_Int level,
+                                                  HYPRE_Int sweeps);
+HYPRE_Int HYPRE_MI355X_BoomerAMGClearLevelParams(HYPRE_Solver solver);
+/* keep the strength pattern S and the PMIS measure of every level for
+ * inspection (off by default; they are freed during setup otherwise) */
+HYPRE_Int HYPRE_MI355X_BoomerAMGSetKeepSetupData(HYPRE_Solver solver, HYPRE_Int keep);
+
+/* ---- hierarchy inspection.  which: 'A' level operator, 'P' interpolation,
+ * 'R' its stored transpose, 'S' strength pattern (a == NULL allowed).
+ * Call with ia == NULL to query sizes; then pass arrays of those sizes. */
+HYPRE_Int HYPRE_MI355X_BoomerAMGGetLevelMatrix(HYPRE_Solver solver, HYPRE_Int level, char which,
+                                               HYPRE_Int *nrows, HYPRE_Int *ncols,
+                                               HYPRE_Int *nnz, HYPRE_Int *ia, HYPRE_Int *ja,
+                                               HYPRE_Real *a);
+HYPRE_Int HYPRE_MI355X_BoomerAMGGetLevelCF(HYPRE_Solver solver, HYPRE_Int level, HYPRE_Int *cf);
+HYPRE_Int HYPRE_MI355X_BoomerAMGGetLevelL1(HYPRE_Solver solver, HYPRE_Int level, HYPRE_Real *l1);
+HYPRE_Int HYPRE_MI355X_BoomerAMGGetLevelMeasure(HYPRE_Solver solver, HYPRE_Int level,
+                                                HYPRE_Real *measure);
+HYPRE_Int HYPRE_MI355X_BoomerAMGGetLevelPMISRounds(HYPRE_Solver solver, HYPRE_Int level,
+                                                   HYPRE_Int *rounds);
+/* one V-cycle on host vectors of the fine size: u <- V(f, u) */
+HYPRE_Int HYPRE_MI355X_BoomerAMGCycleHost(HYPRE_Solver solver, const HYPRE_Real *f,
+                                          HYPRE_Real *u);
+/* PCG residual history ||r_i||/||b||, i = 1..iters of the last solve */
+HYPRE_Int HYPRE_MI355X_PCGGetResidualHistory(HYPRE_Solver solver, HYPRE_Int max_len,
+                                             HYPRE_Real *hist, HYPRE_Int *len);
+/* wall-clock seconds of the last PCG setup / solve (device-synchronised) */
+HYPRE_Int HYPRE_MI355X_PCGGetTimes(HYPRE_Solver solver, HYPRE_Real *setup_s,
+                                   HYPRE_Real *solve_s);
+
+/* ---- single setup kernels on host CSR input (ia/ja/a of n rows) -------- */
+/* strength pattern: S_ia (n+1) and S_ja (nnz(A) capacity) */
+HYPRE_Int HYPRE_MI355X_CreateS(HYPRE_Int n, const HYPRE_Int *ia, const HYPRE_Int *ja,
+                               const HYPRE_Real *a, HYPRE_Real theta, HYPRE_Real max_row_sum,
+                               HYPRE_Int *S_ia, HYPRE_Int *S_ja);
+/* PMIS on a strength pattern; measure == NULL draws the reference's
+ * Park-Miller stream (seed 2747), else the given measure is used */
+HYPRE_Int HYPRE_MI355X_PMIS(HYPRE_Int n, const HYPRE_Int *S_ia, const HYPRE_Int *S_ja,
+                            const HYPRE_Real *measure, HYPRE_Int *cf, HYPRE_Int *rounds);
+/* ext+i interpolation + truncation; P arrays sized by a first call with
+ * P_ia == NULL that returns *P_nnz; cf is updated (-3 -> -1) */
+HYPRE_Int HYPRE_MI355X_ExtPIInterp(HYPRE_Int n, const HYPRE_Int *ia, const HYPRE_Int *ja,
+                                   const HYPRE_Real *a, const HYPRE_Int *S_ia,
+                                   const HYPRE_Int *S_ja, HYPRE_Int *cf,
+                                   HYPRE_Real trunc_factor, HYPRE_Int max_elmts,
+                                   HYPRE_Int *P_ncols, HYPRE_Int *P_nnz, HYPRE_Int *P_ia,
+                                   HYPRE_Int *P_ja, HYPRE_Real *P_a);
+/* Galerkin product Ac = P^T A P (R = P^T is built internally); two-call
+ * pattern as above */
+HYPRE_Int HYPRE_MI355X_RAP(HYPRE_Int n, HYPRE_Int nc, const HYPRE_Int *ia, const HYPRE_Int *ja,
+                           const HYPRE_Real *a, const HYPRE_Int *P_ia, const HYPRE_Int *P_ja,
+                           const HYPRE_Real *P_a, HYPRE_Int *Ac_nnz, HYPRE_Int *Ac_ia,
+                           HYPRE_Int *Ac_ja, HYPRE_Real *Ac_a);
+/* y = alpha*A*x + beta*y on host vectors, through the HIP SpMV */
+HYPRE_Int HYPRE_MI355X_SpMV(HYPRE_Int n, HYPRE_Int ncols, const HYPRE_Int *ia,
+                            const HYPRE_Int *ja, const HYPRE_Real *a, HYPRE_Real alpha,
+                            const HYPRE_Real *x, HYPRE_Real beta, HYPRE_Real *y);
+
+/* ---- ParCSR access ------------------------------------------------------ */
+/* borrowed device pointers of the (single-rank) CSR block */
+HYPRE_Int HYPRE_MI355X_ParCSRMatrixGetDevice(HYPRE_ParCSRMatrix A, HYPRE_Int *n,
+                                             HYPRE_Int *nnz, HYPRE_Int **ia_dev,
+                                             HYPRE_Int **ja_dev, HYPRE_Real **a_dev);
+HYPRE_Int HYPRE_MI355X_ParCSRMatrixCopyToHost(HYPRE_ParCSRMatrix A, HYPRE_Int *ia,
+                                              HYPRE_Int *ja, HYPRE_Real *a);
+HYPRE_Int HYPRE_MI355X_ParVectorGetDevice(HYPRE_ParVector v, HYPRE_Int *n, HYPRE_Real **d);
+HYPRE_Int HYPRE_MI355X_ParVectorCopyFromHost(HYPRE_ParVector v, const HYPRE_Real *x);
+HYPRE_Int HYPRE_MI355X_ParVectorCopyToHost(HYPRE_ParVector v, HYPRE_Real *x);
+/* direct construction from host CSR (diagonal must be first in each row) */
+HYPRE_ParCSRMatrix HYPRE_MI355X_ParCSRMatrixFromHostCSR(HYPRE_Int n, HYPRE_Int ncols,
+                                                        const HYPRE_Int *ia,
+                                                        const HYPRE_Int *ja,
+                                                        const HYPRE_Real *a);
+
+/* ---- benchmark problem generators (built on the device) ---------------- */
+/* 7- or 27-point diffusion on an N^3 grid with a piecewise-constant kappa on
+ * cubic cells of `cell` nodes, kappa = 10^(lo + (hi-lo) U), U from the
+ * Park-Miller stream seeded with `seed` in lexicographic cell order, harmonic
+ * face means, Dirichlet faces using the node's own kappa. */
+HYPRE_ParCSRMatrix HYPRE_MI355X_GenerateVarDiffusion(HYPRE_Int N, HYPRE_Int stencil,
+                                                     HYPRE_Int cell, HYPRE_Real log10_lo,
+                                                     HYPRE_Real log10_hi, HYPRE_Int seed);
+/* 7-point anisotropic diffusion, diagonal of R_z(theta) diag(1, eps, 1) R_z^T */
+HYPRE_ParCSRMatrix HYPRE_MI355X_GenerateAniso7(HYPRE_Int N, HYPRE_Real eps, HYPRE_Real theta);
+/* x_i = 2*U_i - 1 with U the Park-Miller stream from `seed` (seed < 1 -> 1) */
+HYPRE_Int HYPRE_MI355X_ParVectorSetRandomPM(HYPRE_ParVector v, HYPRE_Int seed);
+
+/* ---- kernel timing with HIP events --------------------------------------
+ * While enabled, every launch of the named kernel class is bracketed by HIP
+ * events on the stream it runs on.  Classes: "spmv_A0" (fine-level SpMV of
+ * the solve phase), "relax_A0", "spmv", "relax", "setup", "solve". */
+HYPRE_Int HYPRE_MI355X_SetKernelTiming(HYPRE_Int on);
+HYPRE_Int HYPRE_MI355X_ResetKernelTiming(void);
+HYPRE_Int HYPRE_MI355X_GetKernelTiming(const char *name, HYPRE_Real *total_ms,
+                                       HYPRE_Int *launches, HYPRE_Real *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1254 @@
+/*
+ * CPU oracle for the BoomerAMG-PCG hot path -- TEST INFRASTRUCTURE ONLY.
+ * Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+ * the checker; never linked into or called by the product library.
+ *
+ * Serial restatement of hypre 3.1.0's host path (single rank: every `offd`
+ * block is empty).  Loops keep the reference's traversal and floating-point
+ * operation order so that patterns and values are reproduced exactly; the
+ * file is compiled with -ffp-contract=off.  Citations are file:line into the
+ * hypre source tree (src/...).
+ */
+#include "amg_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_MAX(a, b) (((a) < (b)) ? (b) : (a))
+#define ORC_MIN(a, b) (((a) < (b)) ? (a) : (b))
+
+#define C_PT  1
+#define F_PT -1
+#define SF_PT -3
+
+static void *xmalloc(size_t n)
+{
+   void *p = malloc(n ? n : 1);
+   if (!p) { abort(); }
+   return p;
+}
+static void *xcalloc(size_t n, size_t s)
+{
+   void *p = calloc(n ? n : 1, s ? s : 1);
+   if (!p) { abort(); }
+   return p;
+}
+
+void orc_csr_free(orc_csr *M)
+{
+   if (!M) { return; }
+   free(M->ia); free(M->ja); free(M->a);
+   M->ia = NULL; M->ja = NULL; M->a = NULL;
+   M->nrows = M->ncols = 0;
+}
+
+/* ======================================================================
+ * Park-Miller minimal standard generator (src/utilities/random.c:36-103):
+ * a = 16807, m = 2^31-1, Schrage factorisation q = 127773, r = 2836.
+ * ====================================================================== */
+static int orc_Seed = 13579;
+
+void orc_seed_rand(int seed)                   /* random.c hypre_SeedRand */
+{
+   if (seed < 1) { seed = 1; }
+   else if (seed >= 2147483647) { seed = 2147483647 - 1; }
+   orc_Seed = seed;
+}
+
+int orc_rand_i(void)                           /* random.c hypre_RandI */
+{
+   int low, high, test;
+   high = orc_Seed / 127773;
+   low  = orc_Seed % 127773;
+   test = 16807 * low - 2836 * high;
+   if (test > 0) { orc_Seed = test; }
+   else          { orc_Seed = test + 2147483647; }
+   return orc_Seed;
+}
+
+double orc_rand(void)                          /* random.c hypre_Rand */
+{
+   return ((double) orc_rand_i()) / ((double) 2147483647);
+}
+
+void orc_rand_stream(int seed, int n, double *out)
+{
+   int i;
+   orc_seed_rand(seed);
+   for (i = 0; i < n; i++) { out[i] = orc_rand(); }
+}
+
+/* ======================================================================
+ * Problem generators
+ * ====================================================================== */
+static int csr_alloc(orc_csr *A, int n, int ncols, long nnz)
+{
+   A->nrows = n; A->ncols = ncols;
+   A->ia = (int *) xmalloc(sizeof(int) * (size_t)(n + 1));
+   A->ja = (int *) xmalloc(sizeof(int) * (size_t) nnz);
+   A->a  = (double *) xmalloc(sizeof(double) * (size_t) nnz);
+   return 0;
+}
+
+/* src/parcsr_ls/par_laplace.c GenerateLaplacian (P=Q=R=1), stencil values as
+ * in src/test/ij.c:10454-10470 BuildParLaplacian. Row layout at
+ * par_laplace.c:158-250: diag, -z, -y, -x, +x, +y, +z. */
+int orc_gen_laplace7(int nx, int ny, int nz, double cx, double cy, double cz,
+                     orc_csr *A)
+{
+   double v0 = 0.0, v1 = -cx, v2 = -cy, v3 = -cz;
+   long n = (long) nx * ny * nz, nnz = 0;
+   int ix, iy, iz, row = 0;
+   if (nx > 1) { v0 += 2.0 * cx; }
+   if (ny > 1) { v0 += 2.0 * cy; }
+   if (nz > 1) { v0 += 2.0 * cz; }
+   csr_alloc(A, (int) n, (int) n, 7 * n);
+   A->ia[0] = 0;
+   for (iz = 0; iz < nz; iz++)
+      for (iy = 0; iy < ny; iy++)
+         for (ix = 0; ix < nx; ix++)
+         {
+            A->ja[nnz] = row; A->a[nnz++] = v0;
+            if (iz > 0)      { A->ja[nnz] = row - nx * ny; A->a[nnz++] = v3; }
+            if (iy > 0)      { A->ja[nnz] = row - nx;      A->a[nnz++] = v2; }
+            if (ix > 0)      { A->ja[nnz] = row - 1;       A->a[nnz++] = v1; }
+            if (ix + 1 < nx) { A->ja[nnz] = row + 1;       A->a[nnz++] = v1; }
+            if (iy + 1 < ny) { A->ja[nnz] = row + nx;      A->a[nnz++] = v2; }
+            if (iz + 1 < nz) { A->ja[nnz] = row + nx * ny; A->a[nnz++] = v3; }
+            A->ia[++row] = (int) nnz;
+         }
+   return 0;
+}
+
+/* src/parcsr_ls/par_laplace_27pt.c GenerateLaplacian27pt (one rank): diag
+ * first, then the 26 neighbours in ascending column order (the z-1 plane, the
+ * own plane, the z+1 plane, each row-major). Values: src/test/ij.c:11903-11912 */
+int orc_gen_laplace27(int nx, int ny, int nz, orc_csr *A)
+{
+   double v0 = 26.0, v1 = -1.0;
+   long n = (long) nx * ny * nz, nnz = 0;
+   int ix, iy, iz, dx, dy, dz, row = 0;
+   if (nx == 1 || ny == 1 || nz == 1) { v0 = 8.0; }
+   if (nx * ny == 1 || nx * nz == 1 || ny * nz == 1) { v0 = 2.0; }
+   csr_alloc(A, (int) n, (int) n, 27 * n);
+   A->ia[0] = 0;
+   for (iz = 0; iz < nz; iz++)
+      for (iy = 0; iy < ny; iy++)
+         for (ix = 0; ix < nx; ix++)
+         {
+            A->ja[nnz] = row; A->a[nnz++] = v0;
+            for (dz = -1; dz <= 1; dz++)
+               for (dy = -1; dy <= 1; dy++)
+                  for (dx = -1; dx <= 1; dx++)
+                  {
+                     int jx = ix + dx, jy = iy + dy, jz = iz + dz;
+                     if (!dx && !dy && !dz) { continue; }
+                     if (jx < 0 || jx >= nx || jy < 0 || jy >= ny || jz < 0 || jz >= nz) { continue; }
+                     A->ja[nnz] = row + dz * nx * ny + dy * nx + dx;
+                     A->a[nnz++] = v1;
+                  }
+            A->ia[++row] = (int) nnz;
+         }
+   return 0;
+}
+
+static double *cell_kappa(int N, int cell, double lo, double hi, int seed, int *nc_out)
+{
+   int nc = (N + cell - 1) / cell, c;
+   long ncell = (long) nc * nc * nc;
+   double *kap = (double *) xmalloc(sizeof(double) * (size_t) ncell);
+   orc_seed_rand(seed);
+   for (c = 0; c < ncell; c++)
+   {
+      double u = orc_rand();
+      kap[c] = pow(10.0, lo + (hi - lo) * u);
+   }
+   *nc_out = nc;
+   return kap;
+}
+
+static double harm(double a, double b) { return 2.0 * a * b / (a + b); }
+
+static int gen_vardiff(int N, int cell, double lo, double hi, int seed, int stencil27,
+                       orc_csr *A)
+{
+   int nc, ix, iy, iz, dx, dy, dz, row = 0;
+   long n = (long) N * N * N, nnz = 0;
+   double *kap = cell_kappa(N, cell, lo, hi, seed, &nc);
+   double *node = (double *) xmalloc(sizeof(double) * (size_t) n);
+   for (iz = 0; iz < N; iz++)
+      for (iy = 0; iy < N; iy++)
+         for (ix = 0; ix < N; ix++)
+         {
+            node[(long) iz * N * N + (long) iy * N + ix] =
+               kap[((long) (iz / cell) * nc + iy / cell) * nc + ix / cell];
+         }
+   free(kap);
+   csr_alloc(A, (int) n, (int) n, (stencil27 ? 27 : 7) * n);
+   A->ia[0] = 0;
+   for (iz = 0; iz < N; iz++)
+      for (iy = 0; iy < N; iy++)
+         for (ix = 0; ix < N; ix++)
+         {
+            double kp = node[row], diag = 0.0;
+            long dpos = nnz++;
+            A->ja[dpos] = row;
+            for (dz = -1; dz <= 1; dz++)
+               for (dy = -1; dy <= 1; dy++)
+                  for (dx = -1; dx <= 1; dx++)
+                  {
+                     int jx = ix + dx, jy = iy + dy, jz = iz + dz, col;
+                     double k;
+                     if (!dx && !dy && !dz) { continue; }
+                     if (!stencil27 && (abs(dx) + abs(dy) + abs(dz)) != 1) { continue; }
+                     if (jx < 0 || jx >= N || jy < 0 || jy >= N || jz < 0 || jz >= N)
+                     {
+                        diag += kp;             /* Dirichlet face: own kappa */
+                        continue;
+                     }
+                     col = row + dz * N * N + dy * N + dx;
+                     k = harm(kp, node[col]);
+                     diag += k;
+                     A->ja[nnz] = col; A->a[nnz++] = -k;
+                  }
+            A->a[dpos] = diag;
+            A->ia[++row] = (int) nnz;
+         }
+   free(node);
+   return 0;
+}
+
+int orc_gen_vardiff7(int N, int cell, double lo, double hi, int seed, orc_csr *A)
+{
+   return gen_vardiff(N, cell, lo, hi, seed, 0, A);
+}
+
+int orc_gen_vardiff27(int N, int cell, double lo, double hi, int seed, orc_csr *A)
+{
+   return gen_vardiff(N, cell, lo, hi, seed, 1, A);
+}
+
+int orc_gen_aniso7(int N, double eps, double theta, orc_csr *A)
+{
+   double c = cos(theta), s = sin(theta);
+   double kxx = c * c + eps * s * s, kyy = s * s + eps * c * c, kzz = 1.0;
+   return orc_gen_laplace7(N, N, N, kxx, kyy, kzz, A);
+}
+
+/* ======================================================================
+ * Strength matrix: src/parcsr_ls/par_strength.c:75-525 (CreateSHost),
+ * num_functions == 1, empty offd.  S keeps A's column order without the
+ * (first) diagonal entry.
+ * ====================================================================== */
+int orc_strength(const orc_csr *A, double theta, double max_row_sum, orc_csr *S)
+{
+   int n = A->nrows, i, jA, cnt = 0;
+   const int *ia = A->ia, *ja = A->ja;
+   const double *a = A->a;
+   S->nrows = n; S->ncols = A->ncols;
+   S->ia = (int *) xmalloc(sizeof(int) * (size_t)(n + 1));
+   S->ja = (int *) xmalloc(sizeof(int) * (size_t) ia[n]);
+   S->a = NULL;
+   S->ia[0] = 0;
+   for (i = 0; i < n; i++)
+   {
+      double diag = a[ia[i]], row_scale = 0.0, row_sum = diag;
+      if (diag < 0)
+      {
+         for (jA = ia[i] + 1; jA < ia[i + 1]; jA++)
+         {
+            row_scale = ORC_MAX(row_scale, a[jA]);
+            row_sum += a[jA];
+         }
+      }
+      else
+      {
+         for (jA = ia[i] + 1; jA < ia[i + 1]; jA++)
+         {
+            row_scale = ORC_MIN(row_scale, a[jA]);
+            row_sum += a[jA];
+         }
+      }
+      /* par_strength.c:362 all dependencies weak */
+      if ((fabs(row_sum) > fabs(diag) * max_row_sum) && (max_row_sum < 1.0))
+      {
+         /* no strong entries */
+      }
+      else if (diag < 0)
+      {
+         for (jA = ia[i] + 1; jA < ia[i + 1]; jA++)
+         {
+            if (!(a[jA] <= theta * row_scale)) { S->ja[cnt++] = ja[jA]; }
+         }
+      }
+      else
+      {
+         for (jA = ia[i] + 1; jA < ia[i + 1]; jA++)
+         {
+            if (!(a[jA] >= theta * row_scale)) { S->ja[cnt++] = ja[jA]; }
+         }
+      }
+      S->ia[i + 1] = cnt;
+   }
+   return 0;
+}
+
+/* ======================================================================
+ * PMIS: src/parcsr_ls/par_coarsen.c:2094-2800 (CoarsenPMISHost, CF_init 0)
+ * ====================================================================== */
+int orc_pmis_measure(const orc_csr *S, int seed, double *measure)
+{
+   int n = S->nrows, i;
+   /* par_coarsen.c:2248-2262 column counts of S */
+   for (i = 0; i < n; i++) { measure[i] = 0.0; }
+   for (i = 0; i < S->ia[n]; i++) { measure[S->ja[i]] += 1.0; }
+   /* par_indepset.c:50-75 hypre_BoomerAMGIndepSetInit, seed 2747 + rank */
+   orc_seed_rand(seed);
+   for (i = 0; i < n; i++) { measure[i] += orc_rand(); }
+   return 0;
+}
+
+int orc_pmis_with_measure(const orc_csr *S, const double *measure_in, int *CF)
+{
+   int n = S->nrows, i, j, jS, ig, cnt, graph_size, iter = 0;
+   const int *Si = S->ia, *Sj = S->ja;
+   double *measure = (double *) xmalloc(sizeof(double) * (size_t) n);
+   int *graph = (int *) xmalloc(sizeof(int) * (size_t) n);
+   int *graph2 = (int *) xmalloc(sizeof(int) * (size_t) n);
+   memcpy(measure, measure_in, sizeof(double) * (size_t) n);
+
+   /* par_coarsen.c:2386-2407 (CF_init == 0) */
+   cnt = 0;
+   for (i = 0; i < n; i++)
+   {
+      CF[i] = 0;
+      if (Si[i + 1] - Si[i] == 0)
+      {
+         CF[i] = SF_PT;
+         measure[i] = 0;
+      }
+      else
+      {
+         graph[cnt++] = i;
+      }
+   }
+   graph_size = cnt;
+
+   while (1)   /* par_coarsen.c:2460 THE INDEPENDENT SET COARSENING LOOP */
+   {
+      if (graph_size == 0) { break; }
+      /* par_coarsen.c:2485-2550 pick the independent set */
+      for (ig = 0; ig < graph_size; ig++)
+      {
+         i = graph[ig];
+         if (measure[i] > 1) { CF[i] = 1; }
+      }
+      for (ig = 0; ig < graph_size; ig++)
+      {
+         i = graph[ig];
+         if (measure[i] > 1)
+         {
+            for (jS = Si[i]; jS < Si[i + 1]; jS++)
+            {
+               j = Sj[jS];
+               if (measure[j] > 1)
+               {
+                  if (measure[i] > measure[j])      { CF[j] = 0; }
+                  else if (measure[j] > measure[i]) { CF[i] = 0; }
+               }
+            }
+         }
+      }
+      iter++;
+      /* par_coarsen.c:2603-2660 set C and F points */
+      for (ig = 0; ig < graph_size; ig++)
+      {
+         i = graph[ig];
+         if (measure[i] < 1) { CF[i] = F_PT; }
+         if (CF[i] > 0)
+         {
+            CF[i] = C_PT;
+         }
+         else
+         {
+            for (jS = Si[i]; jS < Si[i + 1]; jS++)
+            {
+               j = Sj[jS];
+               if (CF[j] > 0) { CF[i] = F_PT; }
+            }
+         }
+      }
+      /* par_coarsen.c:2690-2750 update the subgraph */
+      cnt = 0;
+      for (ig = 0; ig < graph_size; ig++)
+      {
+         i = graph[ig];
+         if (CF[i] != 0) { measure[i] = 0; }
+         else            { graph2[cnt++] = i; }
+      }
+      graph_size = cnt;
+      { int *t = graph; graph = graph2; graph2 = t; }
+   }
+   free(measure); free(graph); free(graph2);
+   return iter;
+}
+
+int orc_pmis(const orc_csr *S, int *CF)
+{
+   double *m = (double *) xmalloc(sizeof(double) * (size_t) S->nrows);
+   int it;
+   orc_pmis_measure(S, 2747, m);
+   it = orc_pmis_with_measure(S, m, CF);
+   free(m);
+   return it;
+}
+
+/* ======================================================================
+ * Extended+i interpolation: src/parcsr_ls/par_lr_interp.c:1024-1900
+ * (BuildExtPIInterpHost, one rank, num_functions 1).
+ * ====================================================================== */
+int orc_extpi_interp_raw(const orc_csr *A, const orc_csr *S, const int *CF, orc_csr *P)
+{
+   int n = A->nrows, i, i1, i2, jj, kk, k1, jj1, nc = 0;
+   const int *Ai = A->ia, *Aj = A->ja, *Si = S->ia, *Sj = S->ja;
+   const double *Ad = A->a;
+   int *f2c = (int *) xmalloc(sizeof(int) * (size_t) n);
+   int *P_marker = (int *) xmalloc(sizeof(int) * (size_t) n);
+   int jj_counter = 0, strong_f_marker = -2;
+   long cap;
+
+   /* first pass: fine_to_coarse and row sizes (par_lr_interp.c:1230-1400) */
+   P->nrows = n;
+   P->ia = (int *) xmalloc(sizeof(int) * (size_t)(n + 1));
+   for (i = 0; i < n; i++) { P_marker[i] = -1; f2c[i] = -1; }
+   for (i = 0; i < n; i++)
+   {
+      P->ia[i] = jj_counter;
+      if (CF[i] >= 0)
+      {
+         jj_counter++;
+         f2c[i] = nc++;
+      }
+      else if (CF[i] != SF_PT)
+      {
+         for (jj = Si[i]; jj < Si[i + 1]; jj++)
+         {
+            i1 = Sj[jj];
+            if (CF[i1] >= 0)
+            {
+               if (P_marker[i1] < P->ia[i]) { P_marker[i1] = jj_counter; jj_counter++; }
+            }
+            else if (CF[i1] != SF_PT)
+            {
+               for (kk = Si[i1]; kk < Si[i1 + 1]; kk++)
+               {
+                  k1 = Sj[kk];
+                  if (CF[k1] >= 0)
+                  {
+                     if (P_marker[k1] < P->ia[i]) { P_marker[k1] = jj_counter; jj_counter++; }
+                  }
+               }
+            }
+         }
+      }
+   }
+   P->ia[n] = jj_counter;
+   P->ncols = nc;
+   cap = jj_counter;
+   P->ja = (int *) xmalloc(sizeof(int) * (size_t) cap);
+   P->a = (double *) xmalloc(sizeof(double) * (size_t) cap);
+
+   /* second pass: fill (par_lr_interp.c:1485-1810) */
+   for (i = 0; i < n; i++) { P_marker[i] = -1; }
+   for (i = 0; i < n; i++)
+   {
+      int jj_begin_row = P->ia[i], jj_end_row;
+      int jj_counter2 = jj_begin_row;
+      if (CF[i] >= 0)
+      {
+         P->ja[jj_counter2] = f2c[i];
+         P->a[jj_counter2] = 1.0;
+         jj_counter2++;
+      }
+      else if (CF[i] != SF_PT)
+      {
+         double diagonal, sum, distribute;
+         int sgn;
+         strong_f_marker--;
+         for (jj = Si[i]; jj < Si[i + 1]; jj++)
+         {
+            i1 = Sj[jj];
+            if (CF[i1] >= 0)
+            {
+               if (P_marker[i1] < jj_begin_row)
+               {
+                  P_marker[i1] = jj_counter2;
+                  P->ja[jj_counter2] = f2c[i1];
+                  P->a[jj_counter2] = 0.0;
+                  jj_counter2++;
+               }
+            }
+            else if (CF[i1] != SF_PT)
+            {
+               P_marker[i1] = strong_f_marker;
+               for (kk = Si[i1]; kk < Si[i1 + 1]; kk++)
+               {
+                  k1 = Sj[kk];
+                  if (CF[k1] >= 0)
+                  {
+                     if (P_marker[k1] < jj_begin_row)
+                     {
+                        P_marker[k1] = jj_counter2;
+                        P->ja[jj_counter2] = f2c[k1];
+                        P->a[jj_counter2] = 0.0;
+                        jj_counter2++;
+                     }
+                  }
+               }
+            }
+         }
+         jj_end_row = jj_counter2;
+         diagonal = Ad[Ai[i]];
+         for (jj = Ai[i] + 1; jj < Ai[i + 1]; jj++)
+         {
+            i1 = Aj[jj];
+            if (P_marker[i1] >= jj_begin_row)
+            {
+               P->a[P_marker[i1]] += Ad[jj];
+            }
+            else if (P_marker[i1] == strong_f_marker)
+            {
+               sum = 0.0;
+               sgn = 1;
+               if (Ad[Ai[i1]] < 0) { sgn = -1; }
+               for (jj1 = Ai[i1] + 1; jj1 < Ai[i1 + 1]; jj1++)
+               {
+                  i2 = Aj[jj1];
+                  if ((P_marker[i2] >= jj_begin_row || i2 == i) && (sgn * Ad[jj1]) < 0)
+                  {
+                     sum += Ad[jj1];
+                  }
+               }
+               if (sum != 0)
+               {
+                  distribute = Ad[jj] / sum;
+                  for (jj1 = Ai[i1] + 1; jj1 < Ai[i1 + 1]; jj1++)
+                  {
+                     i2 = Aj[jj1];
+                     if (P_marker[i2] >= jj_begin_row && (sgn * Ad[jj1]) < 0)
+                     {
+                        P->a[P_marker[i2]] += distribute * Ad[jj1];
+                     }
+                     if (i2 == i && (sgn * Ad[jj1]) < 0)
+                     {
+                        diagonal += distribute * Ad[jj1];
+                     }
+                  }
+               }
+               else
+               {
+                  diagonal += Ad[jj];
+               }
+            }
+            else if (CF[i1] != SF_PT)
+            {
+               diagonal += Ad[jj];
+            }
+         }
+         if (diagonal != 0.0)
+         {
+            for (jj = jj_begin_row; jj < jj_end_row; jj++)
+            {
+               P->a[jj] /= -diagonal;
+            }
+         }
+      }
+      strong_f_marker--;
+   }
+   free(f2c); free(P_marker);
+   return 0;
+}
+
+/* src/utilities/qsort.c:452-475 (+ hypre_swap2 at :42) */
+static void swap2(int *v, double *w, int i, int j)
+{
+   int t = v[i]; double t2 = w[i];
+   v[i] = v[j]; v[j] = t;
+   w[i] = w[j]; w[j] = t2;
+}
+
+void orc_qsort2_abs(int *v, double *w, int left, int right)
+{
+   int i, last;
+   if (left >= right) { return; }
+   swap2(v, w, left, (left + right) / 2);
+   last = left;
+   for (i = left + 1; i <= right; i++)
+   {
+      if (fabs(w[i]) > fabs(w[left])) { swap2(v, w, ++last, i); }
+   }
+   swap2(v, w, left, last);
+   orc_qsort2_abs(v, w, left, last - 1);
+   orc_qsort2_abs(v, w, last + 1, right);
+}
+
+/* src/parcsr_mv/par_csr_matrix.c:2943-3340 hypre_ParCSRMatrixTruncate with
+ * rescale = 1, nrm_type = 0 (called from par_interp.c:2622-2652). One thread,
+ * empty offd.  Compacts in place. */
+int orc_truncate(orc_csr *P, double tol, int max_row_elmts)
+{
+   int n = P->nrows, i, j;
+   int *Pi = P->ia, *Pj = P->ja;
+   double *Pd = P->a;
+   if (tol <= 0.0 && max_row_elmts == 0) { return 0; }
+   if (tol > 0)
+   {
+      int next_open = 0, now_checking = 0, num_lost = 0;
+      for (i = 0; i < n; i++)
+      {
+         double row_nrm = 0, drop_coeff, row_sum = 0, scale = 0;
+         int start_j = Pi[i];
+         for (j = Pi[i]; j < Pi[i + 1]; j++)
+         {
+            row_nrm = (row_nrm < fabs(Pd[j])) ? fabs(Pd[j]) : row_nrm;
+         }
+         drop_coeff = tol * row_nrm;
+         if (num_lost) { Pi[i] -= num_lost; }
+         for (j = start_j; j < Pi[i + 1]; j++)
+         {
+            row_sum += Pd[now_checking];
+            if (fabs(Pd[now_checking]) < drop_coeff)
+            {
+               num_lost++; now_checking++;
+            }
+            else
+            {
+               scale += Pd[now_checking];
+               Pd[next_open] = Pd[now_checking];
+               Pj[next_open] = Pj[now_checking];
+               now_checking++; next_open++;
+            }
+         }
+         if (scale != 0.)
+         {
+            if (scale != row_sum)
+            {
+               scale = row_sum / scale;
+               for (j = Pi[i]; j < (Pi[i + 1] - num_lost); j++) { Pd[j] *= scale; }
+            }
+         }
+      }
+      Pi[n] -= num_lost;
+   }
+   if (max_row_elmts > 0)
+   {
+      int mx = 0, cnt_diag = 0;
+      int *aux_j;
+      double *aux_d;
+      for (i = 0; i < n; i++) { mx = ORC_MAX(mx, Pi[i + 1] - Pi[i]); }
+      if (mx > max_row_elmts)
+      {
+         aux_j = (int *) xmalloc(sizeof(int) * (size_t) mx);
+         aux_d = (double *) xmalloc(sizeof(double) * (size_t) mx);
+         for (i = 0; i < n; i++)
+         {
+            int last_index = Pi[i + 1];
+            int num_elmts = last_index - Pi[i];
+            double row_sum = 0, scale;
+            if (max_row_elmts < num_elmts)
+            {
+               int cnt = 0, row_begin = cnt_diag;
+               for (j = Pi[i]; j < last_index; j++)
+               {
+                  aux_j[cnt] = Pj[j];
+                  aux_d[cnt++] = Pd[j];
+                  row_sum += Pd[j];
+               }
+               orc_qsort2_abs(aux_j, aux_d, 0, cnt - 1);
+               scale = 0;
+               Pi[i] = cnt_diag;
+               for (j = 0; j < max_row_elmts; j++)
+               {
+                  scale += aux_d[j];
+                  Pj[cnt_diag] = aux_j[j];
+                  Pd[cnt_diag++] = aux_d[j];
+               }
+               if (scale != 0.)
+               {
+                  if (scale != row_sum)
+                  {
+                     scale = row_sum / scale;
+                     for (j = row_begin; j < cnt_diag; j++) { Pd[j] *= scale; }
+                  }
+               }
+            }
+            else
+            {
+               int start_j = Pi[i];
+               Pi[i] = cnt_diag;
+               for (j = start_j; j < last_index; j++)
+               {
+                  Pj[cnt_diag] = Pj[j];
+                  Pd[cnt_diag++] = Pd[j];
+               }
+            }
+         }
+         Pi[n] = cnt_diag;
+         free(aux_j); free(aux_d);
+      }
+   }
+   return 0;
+}
+
+int orc_extpi_interp(const orc_csr *A, const orc_csr *S, int *CF,
+                     double trunc_factor, int max_elmts, orc_csr *P)
+{
+   int i;
+   orc_extpi_interp_raw(A, S, CF, P);
+   /* par_lr_interp.c:1870 */
+   if (trunc_factor != 0.0 || max_elmts > 0)
+   {
+      orc_truncate(P, trunc_factor, max_elmts);
+   }
+   /* par_lr_interp.c:1890-1897 */
+   for (i = 0; i < A->nrows; i++)
+   {
+      if (CF[i] == SF_PT) { CF[i] = F_PT; }
+   }
+   return 0;
+}
+
+/* src/seq_mv/csr_matop.c:1071 hypre_CSRMatrixTransposeHost: stable counting
+ * sort, so each row of R lists the fine rows of P in ascending order. */
+int orc_transpose(const orc_csr *P, orc_csr *R)
+{
+   int n = P->nrows, nc = P->ncols, i, j, nnz = P->ia[n];
+   int *cnt = (int *) xcalloc((size_t)(nc + 1), sizeof(int));
+   R->nrows = nc; R->ncols = n;
+   R->ia = (int *) xmalloc(sizeof(int) * (size_t)(nc + 1));
+   R->ja = (int *) xmalloc(sizeof(int) * (size_t) nnz);
+   R->a = (double *) xmalloc(sizeof(double) * (size_t) nnz);
+   for (j = 0; j < nnz; j++) { cnt[P->ja[j] + 1]++; }
+   for (i = 0; i < nc; i++) { cnt[i + 1] += cnt[i]; }
+   memcpy(R->ia, cnt, sizeof(int) * (size_t)(nc + 1));
+   for (i = 0; i < n; i++)
+   {
+      for (j = P->ia[i]; j < P->ia[i + 1]; j++)
+      {
+         int c = P->ja[j], pos = cnt[c]++;
+         R->ja[pos] = i;
+         R->a[pos] = P->a[j];
+      }
+   }
+   free(cnt);
+   return 0;
+}
+
+/* src/parcsr_ls/par_rap.c:30-1930 BuildCoarseOperatorKT, one rank: row ic of
+ * RA is built by a marker sweep over R(ic,:) x A, then (RA)(ic,:) x P; the
+ * diagonal entry is created first (par_rap.c:1612-1620). */
+int orc_rap(const orc_csr *R, const orc_csr *A, const orc_csr *P, orc_csr *Ac)
+{
+   int nc = R->nrows, nf = A->nrows, ic, jj1, jj2, i1, i2;
+   int *A_marker = (int *) xmalloc(sizeof(int) * (size_t) nf);
+   int *P_marker = (int *) xmalloc(sizeof(int) * (size_t) nc);
+   int *RA_j = (int *) xmalloc(sizeof(int) * (size_t) nf);
+   double *RA_d = (double *) xmalloc(sizeof(double) * (size_t) nf);
+   long cap = 1024, cnt = 0;
+   int ra_nnz = 0;
+   Ac->nrows = nc; Ac->ncols = nc;
+   Ac->ia = (int *) xmalloc(sizeof(int) * (size_t)(nc + 1));
+   Ac->ja = (int *) xmalloc(sizeof(int) * (size_t) cap);
+   Ac->a = (double *) xmalloc(sizeof(double) * (size_t) cap);
+   for (i1 = 0; i1 < nf; i1++) { A_marker[i1] = -1; }
+   for (i1 = 0; i1 < nc; i1++) { P_marker[i1] = -1; }
+   for (ic = 0; ic < nc; ic++)
+   {
+      long jj_row_begin = cnt;
+      int ra_row_begin = ra_nnz;
+      int ra_cnt = 0;
+      Ac->ia[ic] = (int) cnt;
+      /* RA row: par_rap.c:1712-1800 (A_marker holds absolute RA positions) */
+      for (jj1 = R->ia[ic]; jj1 < R->ia[ic + 1]; jj1++)
+      {
+         double r_entry = R->a[jj1];
+         i1 = R->ja[jj1];
+         for (jj2 = A->ia[i1]; jj2 < A->ia[i1 + 1]; jj2++)
+         {
+            double a_entry = A->a[jj2];
+            int marker;
+            i2 = A->ja[jj2];
+            marker = A_marker[i2];
+            if (marker < ra_row_begin)
+            {
+               A_marker[i2] = ra_nnz;
+               RA_d[ra_nnz - ra_row_begin] = r_entry * a_entry;
+               RA_j[ra_nnz - ra_row_begin] = i2;
+               ra_nnz++;
+            }
+            else
+            {
+               RA_d[marker - ra_row_begin] += r_entry * a_entry;
+            }
+         }
+      }
+      ra_cnt = ra_nnz - ra_row_begin;
+      /* make room for the row of RAP: at most 1 + sum |P(i1,:)| entries */
+      {
+         long bound = 1;
+         for (jj1 = 0; jj1 < ra_cnt; jj1++)
+         {
+            i1 = RA_j[jj1];
+            bound += P->ia[i1 + 1] - P->ia[i1];
+         }
+         if (cnt + bound > cap)
+         {
+            while (cnt + bound > cap) { cap *= 2; }
+            Ac->ja = (int *) realloc(Ac->ja, sizeof(int) * (size_t) cap);
+            Ac->a = (double *) realloc(Ac->a, sizeof(double) * (size_t) cap);
+            if (!Ac->ja || !Ac->a) { abort(); }
+         }
+      }
+      /* diagonal first */
+      P_marker[ic] = (int) cnt;
+      Ac->a[cnt] = 0.0;
+      Ac->ja[cnt] = ic;
+      cnt++;
+      /* (RA) x P: par_rap.c:1860-1920 */
+      for (jj1 = 0; jj1 < ra_cnt; jj1++)
+      {
+         double r_a_product = RA_d[jj1];
+         i1 = RA_j[jj1];
+         for (jj2 = P->ia[i1]; jj2 < P->ia[i1 + 1]; jj2++)
+         {
+            double p_entry = P->a[jj2];
+            int marker;
+            i2 = P->ja[jj2];
+            marker = P_marker[i2];
+            if (marker < jj_row_begin)
+            {
+               P_marker[i2] = (int) cnt;
+               Ac->a[cnt] = r_a_product * p_entry;
+               Ac->ja[cnt] = i2;
+               cnt++;
+            }
+            else
+            {
+               Ac->a[marker] += r_a_product * p_entry;
+            }
+         }
+      }
+   }
+   Ac->ia[nc] = (int) cnt;
+   free(A_marker); free(P_marker); free(RA_j); free(RA_d);
+   return 0;
+}
+
+/* src/parcsr_ls/ams.c:4523-4600 (option 1) and ams.c:770-797: sum |a_ij|
+ * over the row in storage order, negated where the diagonal is negative. */
+int orc_l1_norms(const orc_csr *A, double *l1)
+{
+   int i, j, bad = 0;
+   for (i = 0; i < A->nrows; i++)
+   {
+      l1[i] = 0.0;
+      for (j = A->ia[i]; j < A->ia[i + 1]; j++) { l1[i] += fabs(A->a[j]); }
+   }
+   for (i = 0; i < A->nrows; i++)
+   {
+      if (A->a[A->ia[i]] < 0.0) { l1[i] = -l1[i]; }
+   }
+   for (i = 0; i < A->nrows; i++)
+   {
+      if (fabs(l1[i]) == 0.0) { bad = 1; break; }
+   }
+   return bad ? 4 : 0;
+}
+
+/* ======================================================================
+ * Vector kernels
+ * ====================================================================== */
+void orc_matvec(double alpha, const orc_csr *A, const double *x, double beta,
+                const double *b, double *y)
+{
+   int i, j;
+   for (i = 0; i < A->nrows; i++)
+   {
+      double t = 0.0;
+      for (j = A->ia[i]; j < A->ia[i + 1]; j++) { t += A->a[j] * x[A->ja[j]]; }
+      if (beta == 0.0) { y[i] = alpha * t; }
+      else             { y[i] = alpha * t + beta * b[i]; }
+   }
+}
+
+void orc_matvecT(double alpha, const orc_csr *A, const double *x, double beta, double *y)
+{
+   int i, j;
+   for (i = 0; i < A->ncols; i++) { y[i] = (beta == 0.0) ? 0.0 : beta * y[i]; }
+   for (i = 0; i < A->nrows; i++)
+   {
+      for (j = A->ia[i]; j < A->ia[i + 1]; j++) { y[A->ja[j]] += alpha * A->a[j] * x[i]; }
+   }
+}
+
+double orc_dot(int n, const double *x, const double *y)
+{
+   double s = 0.0;
+   int i;
+   for (i = 0; i < n; i++) { s += x[i] * y[i]; }
+   return s;
+}
+
+/* ======================================================================
+ * AMG setup: level loop of src/parcsr_ls/par_amg_setup.c:1002-3185
+ * ====================================================================== */
+#define ORC_MAXLEV 64
+struct orc_amg
+{
+   orc_amg_params prm;
+   int      num_levels;
+   orc_csr  A[ORC_MAXLEV], P[ORC_MAXLEV], R[ORC_MAXLEV], S[ORC_MAXLEV];
+   int     *CF[ORC_MAXLEV];
+   double  *l1[ORC_MAXLEV];
+   int      rounds[ORC_MAXLEV];
+   int      coarse_relax;
+   int      coarse_sweeps;
+   double  *A_mat;              /* coarsest dense matrix, column major */
+   double  *F[ORC_MAXLEV], *U[ORC_MAXLEV], *Vtemp;
+};
+
+void orc_amg_default_params(orc_amg_params *p)
+{
+   p->strong_threshold = 0.25;
+   p->max_row_sum = 0.9;
+   p->trunc_factor = 0.0;
+   p->P_max_elmts = 4;
+   p->max_levels = 25;
+   p->max_coarse_size = 9;
+   p->relax_type = 18;
+   p->coarse_relax_type = 9;
+   p->relax_weight = 1.0;
+   p->num_sweeps = 1;
+}
+
+static void csr_copy(const orc_csr *A, orc_csr *B)
+{
+   int nnz = A->ia[A->nrows];
+   B->nrows = A->nrows; B->ncols = A->ncols;
+   B->ia = (int *) xmalloc(sizeof(int) * (size_t)(A->nrows + 1));
+   B->ja = (int *) xmalloc(sizeof(int) * (size_t) nnz);
+   B->a = (double *) xmalloc(sizeof(double) * (size_t) nnz);
+   memcpy(B->ia, A->ia, sizeof(int) * (size_t)(A->nrows + 1));
+   memcpy(B->ja, A->ja, sizeof(int) * (size_t) nnz);
+   memcpy(B->a, A->a, sizeof(double) * (size_t) nnz);
+}
+
+orc_amg *orc_amg_setup(const orc_csr *A0, const orc_amg_params *prm)
+{
+   orc_amg *H = (orc_amg *) xcalloc(1, sizeof(orc_amg));
+   int level = 0, j, n;
+   H->prm = *prm;
+   csr_copy(A0, &H->A[0]);
+   H->coarse_relax = prm->coarse_relax_type;
+   H->coarse_sweeps = 1;
+   while (1)
+   {
+      int fine_size = H->A[level].nrows, coarse_size = 0, i;
+      orc_csr *A = &H->A[level];
+      if (prm->max_levels == 1) { break; }
+      orc_strength(A, prm->strong_threshold, prm->max_row_sum, &H->S[level]);
+      H->CF[level] = (int *) xmalloc(sizeof(int) * (size_t) fine_size);
+      H->rounds[level] = orc_pmis(&H->S[level], H->CF[level]);
+      for (i = 0; i < fine_size; i++) { if (H->CF[level][i] == 1) { coarse_size++; } }
+      /* par_amg_setup.c:1690-1725: no coarse grid -> stop, coarsest solve is
+       * one sweep of the down smoother */
+      if (coarse_size == 0 || coarse_size == fine_size)
+      {
+         if (H->coarse_relax == 9) { H->coarse_relax = prm->relax_type; H->coarse_sweeps = 1; }
+         orc_csr_free(&H->S[level]);
+         if (level > 0) { free(H->CF[level]); H->CF[level] = NULL; }
+         break;
+      }
+      orc_extpi_interp(A, &H->S[level], H->CF[level], prm->trunc_factor,
+                       prm->P_max_elmts, &H->P[level]);
+      orc_transpose(&H->P[level], &H->R[level]);
+      orc_rap(&H->R[level], A, &H->P[level], &H->A[level + 1]);
+      level++;
+      /* par_amg_setup.c:3136-3141 (the host-only switch to CLJP at
+       * par_amg_setup.c:3128 is not taken by the device execution policy and
+       * is not reproduced) */
+      if (level == prm->max_levels - 1 || coarse_size <= prm->max_coarse_size) { break; }
+   }
+   H->num_levels = level + 1;
+   /* coarsest solve, par_amg_setup.c:3165-3185 */
+   n = H->A[level].nrows;
+   if (H->coarse_relax == 9)
+   {
+      if (n <= prm->max_coarse_size)
+      {
+         const orc_csr *Ac = &H->A[level];
+         H->A_mat = (double *) xcalloc((size_t) n * (size_t) n, sizeof(double));
+         for (j = 0; j < n; j++)
+         {
+            int jj;
+            for (jj = Ac->ia[j]; jj < Ac->ia[j + 1]; jj++)
+            {
+               H->A_mat[j + n * Ac->ja[jj]] = Ac->a[jj];   /* par_gauss_elim.c:312-316 */
+            }
+         }
+      }
+      else
+      {
+         H->coarse_relax = prm->relax_type;
+      }
+   }
+   /* l1 norms, par_amg_setup.c:3352-3365 */
+   for (j = 0; j < H->num_levels; j++)
+   {
+      int need = (j < H->num_levels - 1) ? (prm->relax_type == 18)
+                                           : (H->coarse_relax == 18);
+      H->F[j] = (double *) xcalloc((size_t) H->A[j].nrows, sizeof(double));
+      H->U[j] = (double *) xcalloc((size_t) H->A[j].nrows, sizeof(double));
+      if (need)
+      {
+         H->l1[j] = (double *) xmalloc(sizeof(double) * (size_t) H->A[j].nrows);
+         orc_l1_norms(&H->A[j], H->l1[j]);
+      }
+   }
+   H->Vtemp = (double *) xcalloc((size_t) H->A[0].nrows, sizeof(double));
+   return H;
+}
+
+void orc_amg_free(orc_amg *H)
+{
+   int j;
+   if (!H) { return; }
+   for (j = 0; j < H->num_levels; j++)
+   {
+      orc_csr_free(&H->A[j]); orc_csr_free(&H->P[j]); orc_csr_free(&H->R[j]);
+      orc_csr_free(&H->S[j]);
+      free(H->CF[j]); free(H->l1[j]); free(H->F[j]); free(H->U[j]);
+   }
+   free(H->A_mat); free(H->Vtemp);
+   free(H);
+}
+
+int orc_amg_num_levels(const orc_amg *H) { return H->num_levels; }
+const orc_csr *orc_amg_A(const orc_amg *H, int l) { return &H->A[l]; }
+const orc_csr *orc_amg_P(const orc_amg *H, int l) { return &H->P[l]; }
+const orc_csr *orc_amg_R(const orc_amg *H, int l) { return &H->R[l]; }
+const orc_csr *orc_amg_S(const orc_amg *H, int l) { return &H->S[l]; }
+const int *orc_amg_CF(const orc_amg *H, int l) { return H->CF[l]; }
+const double *orc_amg_l1(const orc_amg *H, int l) { return H->l1[l]; }
+int orc_amg_pmis_rounds(const orc_amg *H, int l) { return H->rounds[l]; }
+int orc_amg_coarse_relax(const orc_amg *H) { return H->coarse_relax; }
+
+/* relax_type 18: src/parcsr_ls/par_relax.c:1178-1240 Relax7Jacobi
+ *   Vtemp = w*(f - A u);  u += Vtemp ./ l1
+ * relax_type 0: par_relax.c:180-320 weighted Jacobi (Skip_diag = 1). */
+static void relax(orc_amg *H, int lev, int type, const double *f, double *u)
+{
+   const orc_csr *A = &H->A[lev];
+   int n = A->nrows, i, j;
+   double w = H->prm.relax_weight;
+   double *v = H->Vtemp;
+   if (type == 18)
+   {
+      for (i = 0; i < n; i++)
+      {
+         double t = 0.0;
+         for (j = A->ia[i]; j < A->ia[i + 1]; j++) { t += A->a[j] * u[A->ja[j]]; }
+         v[i] = -w * t + w * f[i];
+      }
+      for (i = 0; i < n; i++) { u[i] += v[i] / H->l1[lev][i]; }
+   }
+   else   /* Jacobi */
+   {
+      double omw = 1.0 - w;
+      for (i = 0; i < n; i++) { v[i] = u[i]; }
+      for (i = 0; i < n; i++)
+      {
+         double di = A->a[A->ia[i]], res = f[i];
+         if (di == 0.0) { continue; }
+         for (j = A->ia[i] + 1; j < A->ia[i + 1]; j++) { res -= A->a[j] * v[A->ja[j]]; }
+         u[i] *= omw;
+         u[i] += w * res / di;
+      }
+   }
+}
+
+/* src/utilities/gselim.h hypre_gselim applied to the column-major matrix
+ * (par_gauss_elim.c:640-650, relax type 9) */
+static void gselim(double *A, double *x, int n)
+{
+   int j, k, m;
+   if (n == 1)
+   {
+      if (A[0] != 0.0) { x[0] = x[0] / A[0]; }
+      return;
+   }
+   for (k = 0; k < n - 1; k++)
+   {
+      if (A[k * n + k] != 0.0)
+      {
+         double divA = 1.0 / A[k * n + k];
+         for (j = k + 1; j < n; j++)
+         {
+            if (A[j * n + k] != 0.0)
+            {
+               double factor = A[j * n + k] * divA;
+               for (m = k + 1; m < n; m++) { A[j * n + m] -= factor * A[k * n + m]; }
+               x[j] -= factor * x[k];
+            }
+         }
+      }
+   }
+   for (k = n - 1; k > 0; --k)
+   {
+      if (A[k * n + k] != 0.0)
+      {
+         x[k] /= A[k * n + k];
+         for (j = 0; j < k; j++)
+         {
+            if (A[j * n + k] != 0.0) { x[j] -= x[k] * A[j * n + k]; }
+         }
+      }
+   }
+   if (A[0] != 0.0) { x[0] /= A[0]; }
+}
+
+/* src/parcsr_ls/par_cycle.c:23-889, V-cycle, one sweep down/up */
+int orc_amg_cycle(orc_amg *H, const double *f, double *u)
+{
+   int L = H->num_levels, lev, i, s;
+   int nsw = H->prm.num_sweeps;
+   memcpy(H->F[0], f, sizeof(double) * (size_t) H->A[0].nrows);
+   memcpy(H->U[0], u, sizeof(double) * (size_t) H->A[0].nrows);
+   for (lev = 0; lev < L - 1; lev++)
+   {
+      int n = H->A[lev].nrows;
+      for (s = 0; s < nsw; s++) { relax(H, lev, H->prm.relax_type, H->F[lev], H->U[lev]); }
+      /* residual and restriction, par_cycle.c:755-790 */
+      orc_matvec(-1.0, &H->A[lev], H->U[lev], 1.0, H->F[lev], H->Vtemp);
+      for (i = 0; i < H->A[lev + 1].nrows; i++) { H->U[lev + 1][i] = 0.0; }
+      {
+         const orc_csr *R = &H->R[lev];   /* F_c = P^T r via the stored transpose */
+         int ic, jj;
+         (void) n;
+         for (ic = 0; ic < R->nrows; ic++)
+         {
+            double t = 0.0;
+            for (jj = R->ia[ic]; jj < R->ia[ic + 1]; jj++) { t += R->a[jj] * H->Vtemp[R->ja[jj]]; }
+            H->F[lev + 1][ic] = t;
+         }
+      }
+   }
+   /* coarsest level */
+   lev = L - 1;
+   if (H->coarse_relax == 9 && H->A_mat)
+   {
+      int n = H->A[lev].nrows;
+      double *work = (double *) xmalloc(sizeof(double) * (size_t)(n * n));
+      double *x = (double *) xmalloc(sizeof(double) * (size_t) n);
+      memcpy(work, H->A_mat, sizeof(double) * (size_t)(n * n));
+      memcpy(x, H->F[lev], sizeof(double) * (size_t) n);
+      gselim(work, x, n);
+      memcpy(H->U[lev], x, sizeof(double) * (size_t) n);
+      free(work); free(x);
+   }
+   else
+   {
+      for (s = 0; s < H->coarse_sweeps; s++) { relax(H, lev, H->coarse_relax, H->F[lev], H->U[lev]); }
+   }
+   for (lev = L - 2; lev >= 0; lev--)
+   {
+      /* interpolation u += P e, par_cycle.c:820-840 */
+      orc_matvec(1.0, &H->P[lev], H->U[lev + 1], 1.0, H->U[lev], H->U[lev]);
+      for (s = 0; s < nsw; s++) { relax(H, lev, H->prm.relax_type, H->F[lev], H->U[lev]); }
+   }
+   memcpy(u, H->U[0], sizeof(double) * (size_t) H->A[0].nrows);
+   return 0;
+}
+
+/* src/parcsr_ls/par_amg_solve.c:22-424 (tol > 0, converge_type 0) */
+int orc_amg_solve(orc_amg *H, const double *b, double *x, double tol, int max_iter,
+                  int *iters, double *rel_res, double *conv)
+{
+   int n = H->A[0].nrows, cycle = 0;
+   double *r = (double *) xmalloc(sizeof(double) * (size_t) n);
+   double res0, res, rhs, rel;
+   orc_matvec(-1.0, &H->A[0], x, 1.0, b, r);
+   res0 = sqrt(orc_dot(n, r, r));
+   res = res0;
+   rhs = sqrt(orc_dot(n, b, b));
+   rel = (rhs != 0.0) ? res0 / rhs : res0;
+   while (rel >= tol && cycle < max_iter)
+   {
+      orc_amg_cycle(H, b, x);
+      orc_matvec(-1.0, &H->A[0], x, 1.0, b, r);
+      res = sqrt(orc_dot(n, r, r));
+      rel = (rhs != 0.0) ? res / rhs : res;
+      cycle++;
+   }
+   *iters = cycle;
+   *rel_res = rel;
+   *conv = (cycle > 0 && res0 != 0.0) ? pow(res / res0, 1.0 / (double) cycle) : 1.0;
+   free(r);
+   return (cycle == max_iter && tol > 0.) ? 256 : 0;
+}
+
+/* src/krylov/pcg.c:313-1013, two_norm = 1, everything else default */
+int orc_pcg_solve(orc_amg *H, const double *b, double *x, double tol, int max_iter,
+                  int *iters, double *rel_res, double *history)
+{
+   int n = H->A[0].nrows, i = 0, k, err = 0;
+   const orc_csr *A = &H->A[0];
+   double *p = (double *) xcalloc((size_t) n, sizeof(double));
+   double *s = (double *) xcalloc((size_t) n, sizeof(double));
+   double *r = (double *) xcalloc((size_t) n, sizeof(double));
+   double bi_prod, eps, gamma, gamma_old, alpha, beta, sdotp, i_prod = 0.0;
+   bi_prod = orc_dot(n, b, b);
+   if (bi_prod != bi_prod) { free(p); free(s); free(r); return 1; }
+   if (bi_prod <= 0.0)
+   {
+      memcpy(x, b, sizeof(double) * (size_t) n);
+      *iters = 0; *rel_res = 0.0;
+      free(p); free(s); free(r);
+      return 0;
+   }
+   eps = tol * tol;
+   /* r = b - A x */
+   orc_matvec(-1.0, A, x, 1.0, b, r);
+   memset(p, 0, sizeof(double) * (size_t) n);
+   orc_amg_cycle(H, r, p);
+   gamma = orc_dot(n, r, p);
+   if (gamma != gamma) { free(p); free(s); free(r); return 1; }
+   while (i + 1 <= max_iter)
+   {
+      i++;
+      orc_matvec(1.0, A, p, 0.0, NULL, s);
+      sdotp = orc_dot(n, s, p);
+      if (sdotp == 0.0) { err = 256; break; }
+      alpha = gamma / sdotp;
+      if (alpha <= 0.0) { err = 256; break; }
+      gamma_old = gamma;
+      for (k = 0; k < n; k++) { x[k] += alpha * p[k]; }
+      for (k = 0; k < n; k++) { r[k] += -alpha * s[k]; }
+      memset(s, 0, sizeof(double) * (size_t) n);
+      orc_amg_cycle(H, r, s);
+      gamma = orc_dot(n, r, s);
+      i_prod = orc_dot(n, r, r);
+      if (history) { history[i] = sqrt(i_prod / bi_prod); }
+      if (i_prod / bi_prod < eps) { break; }
+      if (gamma <= 0.0) { err = 256; break; }
+      beta = gamma / gamma_old;
+      for (k = 0; k < n; k++) { p[k] = beta * p[k]; }
+      for (k = 0; k < n; k++) { p[k] += s[k]; }
+   }
+   *iters = i;
+   *rel_res = sqrt(i_prod / bi_prod);
+   if (i >= max_iter && (i_prod / bi_prod) >= eps) { err = 256; }
+   free(p); free(s); free(r);
+   return err;
+}

@@ -1,0 +1,43 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares
+(CPU only: no compute calls)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "hypre_amd", "libhypre_mi355x.so")
+
+
+def declared_functions():
+    names = set()
+    pat = re.compile(r"^\s*(?:HYPRE_Int|HYPRE_ParCSRMatrix|void)\s+\*?\s*((?:HYPRE_|Generate)\w+)\s*\(", re.M)
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            with open(os.path.join(ROOT, "include", fn)) as f:
+                names.update(pat.findall(f.read()))
+    return names
+
+
+def test_headers_declare_the_hot_path():
+    names = declared_functions()
+    for must in ("HYPRE_IJMatrixCreate", "HYPRE_IJMatrixSetValues", "HYPRE_IJMatrixAssemble",
+                 "HYPRE_IJMatrixGetObject", "HYPRE_IJVectorCreate", "HYPRE_BoomerAMGCreate",
+                 "HYPRE_BoomerAMGSetup", "HYPRE_BoomerAMGSolve", "HYPRE_ParCSRPCGCreate",
+                 "HYPRE_ParCSRPCGSetup", "HYPRE_ParCSRPCGSolve", "HYPRE_ParCSRPCGSetPrecond",
+                 "HYPRE_PCGSetTol", "HYPRE_ParCSRMatrixMatvec", "GenerateLaplacian27pt"):
+        assert must in names
+    assert len(names) > 150
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(LIB), "build with python -m hypre_amd.build"
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in sorted(declared_functions()) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_signatures_match_exports():
+    from hypre_amd.lib import SIGNATURES, lib
+    L = lib()
+    for name in SIGNATURES:
+        assert hasattr(L, name), name

@@ -1,0 +1,331 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Bar: C/F splittings and the sparsity patterns of S, P, R, A_c are bit-exact
+(identical CSR arrays, including the within-row order that fixes hypre's
+first-seen orders); hierarchy values are compared exactly as well (tolerance
+stated per test where the GPU legitimately sums in a different order);
+iteration counts within +-1; final residuals recomputed in FP64.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-parity]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def to_oracle(O, M):
+    return O.Csr(M.nrows, M.ncols, np.asarray(M.ia, np.int32), np.asarray(M.ja, np.int32),
+                 None if M.a is None else np.asarray(M.a, np.float64))
+
+
+def to_host(hx, M):
+    return hx.HostCsr(M.nrows, M.ncols, np.ascontiguousarray(M.ia, np.int32),
+                      np.ascontiguousarray(M.ja, np.int32),
+                      None if M.a is None else np.ascontiguousarray(M.a, np.float64))
+
+
+def assert_csr_equal(G, R, what, values="exact", rtol=0.0):
+    assert G.nrows == R.nrows, what
+    assert np.array_equal(np.asarray(G.ia[: G.nrows + 1]), np.asarray(R.ia[: R.nrows + 1])), what + " row pointers"
+    nnz = int(R.ia[R.nrows])
+    assert np.array_equal(np.asarray(G.ja[:nnz]), np.asarray(R.ja[:nnz])), what + " columns"
+    if values == "exact":
+        ga, ra = np.asarray(G.a[:nnz]), np.asarray(R.a[:nnz])
+        bad = np.nonzero(ga != ra)[0]
+        assert bad.size == 0, f"{what}: {bad.size} values differ, first at {bad[:5]}: {ga[bad[:5]]} vs {ra[bad[:5]]}"
+    elif values == "close":
+        np.testing.assert_allclose(G.a[:nnz], R.a[:nnz], rtol=rtol, atol=0, err_msg=what)
+
+
+def problems(O):
+    return {
+        "lap7_14": lambda: O.gen_laplace7(14, 14, 14),
+        "lap27_12": lambda: O.gen_laplace27(12, 12, 12),
+        "aniso_12": lambda: O.gen_laplace7(12, 12, 12, 1.0, 0.05, 0.6),
+        "vardiff27_12": lambda: O.gen_vardiff27(12, 3, -3.0, 3.0, 7),
+        "vardiff7_16": lambda: O.gen_vardiff7(16, 4, -3.0, 3.0, 3),
+    }
+
+
+# ------------------------------------------------------------------ kernels
+def test_spmv_matches_oracle(hx, oracle):
+    for name, gen in problems(oracle).items():
+        say("spmv", name)
+        A = gen()
+        x = 2 * oracle.rand_stream(5, A.ncols) - 1
+        y0 = 2 * oracle.rand_stream(9, A.nrows) - 1
+        for alpha, beta in ((1.0, 0.0), (-1.0, 1.0), (0.7, -0.3)):
+            g = hx.spmv(to_host(hx, A), x, alpha, beta, y0)
+            r = oracle.matvec(alpha, A, x, beta, y0)
+            scale = abs(alpha) * (abs(A.to_scipy()) @ np.abs(x)) + abs(beta) * np.abs(y0)
+            assert np.all(np.abs(g - r) <= 4e-16 * 32 * scale + 1e-300), name
+
+
+def test_generators_match_oracle(hx, oracle):
+    say("generators")
+    for (nx, ny, nz) in ((9, 7, 5), (12, 12, 12)):
+        G = hx.ParCSRMatrix.laplacian7(nx, ny, nz).to_host()
+        assert_csr_equal(G, oracle.gen_laplace7(nx, ny, nz), "7pt")
+        G = hx.ParCSRMatrix.laplacian27(nx, ny, nz).to_host()
+        assert_csr_equal(G, oracle.gen_laplace27(nx, ny, nz), "27pt")
+    G = hx.ParCSRMatrix.aniso7(10, 0.01, 0.7).to_host()
+    assert_csr_equal(G, oracle.gen_aniso7(10, 0.01, 0.7), "aniso", values="close", rtol=1e-15)
+    for st in (7, 27):
+        G = hx.ParCSRMatrix.var_diffusion(12, st, 3, -3.0, 3.0, 7).to_host()
+        R = (oracle.gen_vardiff7 if st == 7 else oracle.gen_vardiff27)(12, 3, -3.0, 3.0, 7)
+        # pow() of the device and of the host libm may differ in the last bit
+        assert_csr_equal(G, R, f"vardiff{st}", values="close", rtol=1e-14)
+
+
+@pytest.mark.parametrize("theta,mrs", [(0.25, 0.9), (0.25, 1.0), (0.5, 0.8)])
+def test_strength_bit_exact(hx, oracle, theta, mrs):
+    for name, gen in problems(oracle).items():
+        say("strength", name, theta, mrs)
+        A = gen()
+        G = hx.create_s(to_host(hx, A), theta, mrs)
+        R = oracle.strength(A, theta, mrs)
+        assert np.array_equal(G.ia, R.ia) and np.array_equal(G.ja, R.ja), name
+
+
+def test_pmis_bit_exact(hx, oracle):
+    for name, gen in problems(oracle).items():
+        say("pmis", name)
+        A = gen()
+        S = oracle.strength(A, 0.25, 0.9)
+        # the reference's own stream (seed 2747) drawn on the device by jump-ahead
+        cf_g, rounds_g = hx.pmis(to_host(hx, S))
+        cf_r, rounds_r = oracle.pmis(S)
+        assert np.array_equal(cf_g, cf_r), name
+        assert rounds_g == rounds_r
+        # an arbitrary given measure vector
+        rng = np.random.default_rng(1)
+        m = np.bincount(S.ja, minlength=S.nrows) + rng.random(S.nrows)
+        cf_g, _ = hx.pmis(to_host(hx, S), m)
+        cf_r, _ = oracle.pmis(S, m)
+        assert np.array_equal(cf_g, cf_r), name + " (given measure)"
+
+
+@pytest.mark.parametrize("trunc,pmx", [(0.0, 4), (0.0, 0), (0.1, 0), (0.2, 3)])
+def test_extpi_interp_bit_exact(hx, oracle, trunc, pmx):
+    for name, gen in problems(oracle).items():
+        say("extpi", name, trunc, pmx)
+        A = gen()
+        S = oracle.strength(A, 0.25, 0.9)
+        cf, _ = oracle.pmis(S)
+        Pg, cfg = hx.extpi_interp(to_host(hx, A), to_host(hx, S), cf, trunc, pmx)
+        Pr, cfr = oracle.extpi_interp(A, S, cf, trunc, pmx)
+        assert np.array_equal(cfg, cfr)
+        assert Pg.ncols == Pr.ncols
+        assert_csr_equal(Pg, Pr, f"P {name}")
+
+
+def test_rap_bit_exact(hx, oracle):
+    for name, gen in problems(oracle).items():
+        say("rap", name)
+        A = gen()
+        S = oracle.strength(A, 0.25, 0.9)
+        cf, _ = oracle.pmis(S)
+        P, _ = oracle.extpi_interp(A, S, cf, 0.0, 4)
+        Acg = hx.rap(to_host(hx, A), to_host(hx, P))
+        Acr = oracle.rap(oracle.transpose(P), A, P)
+        assert_csr_equal(Acg, Acr, f"Ac {name}")
+
+
+# ------------------------------------------------------------------ hierarchy
+def compare_hierarchy(hx, oracle, A_dev, A_or, **prm):
+    amg = hx.BoomerAMG(keep_setup_data=1, **{k: v for k, v in prm.items() if k != "relax_type_oracle"})
+    amg.setup(A_dev)
+    H = oracle.Hierarchy(A_or, **{k: v for k, v in prm.items()
+                                  if k in ("max_row_sum", "P_max_elmts", "strong_threshold", "trunc_factor")})
+    L = amg.num_levels()
+    assert L == H.num_levels
+    for lev in range(L):
+        assert_csr_equal(amg.level_matrix(lev, "A"), H.A(lev), f"A{lev}")
+        if lev < L - 1:
+            assert np.array_equal(amg.level_cf(lev), H.CF(lev)), f"CF{lev}"
+            Sg = amg.level_matrix(lev, "S")
+            Sr = H.S(lev)
+            assert np.array_equal(Sg.ia, Sr.ia) and np.array_equal(Sg.ja, Sr.ja), f"S{lev}"
+            assert_csr_equal(amg.level_matrix(lev, "P"), H.P(lev), f"P{lev}")
+            assert_csr_equal(amg.level_matrix(lev, "R"), H.R(lev), f"R{lev}")
+            assert np.array_equal(amg.level_l1(lev), H.l1(lev)), f"l1{lev}"
+    return amg, H
+
+
+def test_hierarchy_bit_exact_30_7pt(hx, oracle, golden):
+    say("hierarchy 30^3 7pt (ij settings)")
+    A_dev = hx.ParCSRMatrix.laplacian7(30, 30, 30)
+    amg, H = compare_hierarchy(hx, oracle, A_dev, oracle.gen_laplace7(30, 30, 30), max_row_sum=1.0)
+    m = amg.level_measure(0)
+    S = H.S(0)
+    cnt = np.bincount(S.ja, minlength=S.nrows)
+    assert np.array_equal(m, cnt + oracle.rand_stream(2747, S.nrows))
+
+
+@pytest.mark.parametrize("name", ["lap27_12", "aniso_12", "vardiff27_12", "vardiff7_16"])
+def test_hierarchy_bit_exact(hx, oracle, name):
+    say("hierarchy", name)
+    A_or = problems(oracle)[name]()
+    A_dev = hx.ParCSRMatrix.from_csr(A_or.ia, A_or.ja, A_or.a)
+    compare_hierarchy(hx, oracle, A_dev, A_or)
+
+
+def test_hierarchy_bit_exact_device_generated_jump27(hx, oracle):
+    say("hierarchy, device-generated 27pt jump 20^3")
+    A_dev = hx.ParCSRMatrix.var_diffusion(20, 27, 4, -4.0, 4.0, 11)
+    A_or = to_oracle(oracle, A_dev.to_host())
+    compare_hierarchy(hx, oracle, A_dev, A_or)
+
+
+# ------------------------------------------------------------------ solve
+def test_vcycle_matches_oracle(hx, oracle):
+    say("v-cycle")
+    A_or = oracle.gen_vardiff27(14, 4, -2.0, 2.0, 5)
+    A_dev = hx.ParCSRMatrix.from_csr(A_or.ia, A_or.ja, A_or.a)
+    amg = hx.BoomerAMG(max_iter=1, tol=0.0)
+    amg.setup(A_dev)
+    H = oracle.Hierarchy(A_or)
+    f = 2 * oracle.rand_stream(3, A_or.nrows) - 1
+    ug = amg.cycle_host(f)
+    ur = H.cycle(f)
+    np.testing.assert_allclose(ug, ur, rtol=1e-11, atol=1e-13 * np.abs(ur).max())
+
+
+def test_pcg_known_answer_30_7pt(hx, oracle, golden):
+    g = golden["ij_30_7pt"]
+    say("pcg 30^3 7pt vs the reference's recorded run")
+    A = hx.ParCSRMatrix.laplacian7(30, 30, 30)
+    n = 27000
+    b = hx.ParVector(n, oracle.ij_rhsrand(n))
+    x = hx.ParVector(n)
+    amg = hx.BoomerAMG(max_row_sum=1.0)
+    pcg = hx.PCG(amg, tol=1e-8, max_iter=1000, two_norm=1)
+    pcg.setup(A, b, x)
+    pcg.solve(A, b, x)
+    assert abs(pcg.num_iterations() - g["iterations"]) <= 1
+    assert pcg.num_iterations() == g["iterations"]
+    np.testing.assert_allclose(pcg.history(), g["history"], rtol=2e-6)
+    # true residual recomputed in FP64 on the host
+    A_or = oracle.gen_laplace7(30, 30, 30)
+    r = b.get() - A_or.to_scipy() @ x.get()
+    assert np.linalg.norm(r) / np.linalg.norm(b.get()) < 1e-8
+
+
+def test_amg_solver_default_saved(hx, oracle, golden):
+    g = golden["default_saved"]
+    say("AMG solver, TEST_ij/default.jobs:11 settings")
+    A = hx.ParCSRMatrix.laplacian7(10, 10, 10)
+    A_or = oracle.gen_laplace7(10, 10, 10)
+    b = hx.ParVector(1000, oracle.matvec(1.0, A_or, np.ones(1000)))
+    x = hx.ParVector(1000)
+    amg = hx.BoomerAMG(max_row_sum=1.0, P_max_elmts=0, relax_type=0, tol=1e-8, max_iter=100)
+    amg.setup(A)
+    amg.solve(A, b, x)
+    conv = amg.final_rel_res() ** (1.0 / amg.num_iterations())
+    assert round(conv, 6) == g["avg_convergence_factor"]
+    np.testing.assert_allclose(x.get(), 1.0, atol=1e-6)
+
+
+def test_full_size_200_27pt_known_answer(hx, oracle, golden):
+    """BASELINE's 200^3 27-point problem: the level sizes of the reference's
+    recorded run and its iteration count, at full size on the device."""
+    g = golden["ij_200_27pt"]
+    say("200^3 27pt full-size hierarchy + PCG")
+    A = hx.ParCSRMatrix.laplacian27(200, 200, 200)
+    n = 8_000_000
+    b = hx.ParVector(n, oracle.ij_rhsrand(n))
+    x = hx.ParVector(n)
+    amg = hx.BoomerAMG(max_row_sum=1.0)
+    pcg = hx.PCG(amg, tol=1e-8, max_iter=1000, two_norm=1)
+    pcg.setup(A, b, x)
+    for lev, ref in enumerate(g["levels"]):
+        M = amg.level_matrix(lev, "A")
+        assert (M.nrows, M.nnz) == (ref["rows"], ref["nnz"]), lev
+    pcg.solve(A, b, x)
+    assert abs(pcg.num_iterations() - g["iterations"]) <= 1
+    np.testing.assert_allclose(pcg.history(), g["history"], rtol=1e-5)
+
+
+def test_fp32_vcycle_inside_fp64_pcg(hx, oracle):
+    say("mixed precision")
+    A_or = oracle.gen_vardiff7(24, 8, -3.0, 3.0, 2)
+    A = hx.ParCSRMatrix.from_csr(A_or.ia, A_or.ja, A_or.a)
+    n = A_or.nrows
+    bh = 2 * oracle.rand_stream(4, n) - 1
+    its = {}
+    for fp32 in (0, 1):
+        b = hx.ParVector(n, bh)
+        x = hx.ParVector(n)
+        amg = hx.BoomerAMG(fp32_cycle=fp32)
+        pcg = hx.PCG(amg, tol=1e-8)
+        pcg.setup(A, b, x)
+        pcg.solve(A, b, x)
+        r = bh - A_or.to_scipy() @ x.get()
+        rel = np.linalg.norm(r) / np.linalg.norm(bh)
+        assert rel < 1.5e-8, (fp32, rel)
+        its[fp32] = pcg.num_iterations()
+    assert its[1] <= its[0] + 2
+
+
+# ------------------------------------------------------------------ IJ + errors
+def test_ij_assembly_semantics(hx):
+    say("IJ assembly")
+    M = hx.IJMatrix(0, 3)
+    M.initialize()
+    # row 1 given as [2, 1(diag), 0]; a duplicate set overwrites, add accumulates
+    M.set_values([0, 1, 2, 3], [2, 3, 2, 1], [0, 1, 2, 1, 0, 2, 3, 3],
+                 [4.0, -1.0, -1.0, 4.0, -1.0, 4.0, -1.0, 4.0])
+    M.set_values([1], [1], [2], [-2.0])
+    M.add_to_values([1], [1], [2], [0.5])
+    M.add_to_values([3], [1], [2], [-1.0])
+    M.assemble()
+    H = M.get_object().to_host()
+    assert H.ia.tolist() == [0, 2, 5, 7, 9]
+    # diagonal first, remaining entries in insertion order (aux path)
+    assert H.ja.tolist() == [0, 1, 1, 2, 0, 2, 3, 3, 2]
+    assert H.a.tolist() == [4.0, -1.0, 4.0, -1.5, -1.0, 4.0, -1.0, 4.0, -1.0]
+    # SetDiagOffdSizes selects the direct path: diagonal swapped with the first
+    M2 = hx.IJMatrix(0, 1)
+    M2.set_diag_offd_sizes([2, 2])
+    M2.initialize()
+    M2.set_values([0, 1], [2, 2], [1, 0, 0, 1], [-1.0, 2.0, -1.0, 2.0])
+    M2.assemble()
+    H2 = M2.get_object().to_host()
+    assert H2.ja.tolist() == [0, 1, 1, 0]
+
+
+def test_error_codes(hx):
+    say("error codes")
+    amg = hx.BoomerAMG()
+    with pytest.raises(hx.HypreError) as e:
+        amg.set(coarsen_type=6)
+    assert e.value.code & hx.ERROR_ARG
+    with pytest.raises(hx.HypreError):
+        amg.set(interp_type=0)
+    hx.clear_errors()
+    A = hx.ParCSRMatrix.laplacian7(8, 8, 8)
+    b = hx.ParVector(512, np.ones(512))
+    x = hx.ParVector(512)
+    pcg = hx.PCG(hx.BoomerAMG(), tol=1e-12, max_iter=2)
+    pcg.setup(A, b, x)
+    err = pcg.solve(A, b, x)
+    assert err & hx.ERROR_CONV
+    hx.clear_errors()
+    bn = np.ones(512)
+    bn[3] = np.nan
+    b2 = hx.ParVector(512, bn)
+    with pytest.raises(hx.HypreError) as e:
+        pcg.solve(A, b2, x)
+    assert e.value.code & hx.ERROR_GENERIC
+    hx.clear_errors()
+    # zero right-hand side: x = b, no iterations
+    z = hx.ParVector(512)
+    pcg.solve(A, z, x)
+    assert pcg.num_iterations() == 0
+    assert np.all(x.get() == 0.0)
