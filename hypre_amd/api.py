@@ -539,7 +539,8 @@ def spmv(A: HostCsr, x, alpha=1.0, beta=0.0, y=None) -> np.ndarray:
 
 
 # ------------------------------------------------------------------ timing
-def kernel_timing(on: bool = True):
+def kernel_timing(on: bool = True, only: str | None = None):
+    call("HYPRE_MI355X_SetKernelTimingClass", (only or "").encode())
     call("HYPRE_MI355X_SetKernelTiming", 1 if on else 0)
 
 

@@ -366,6 +366,7 @@ struct TimingRec
    double      bytes;
 };
 static bool                    g_timing = false;
+static std::string             g_timing_only;      // empty: every class
 static std::vector<TimingRec>  g_trecs;
 static std::vector<hipEvent_t> g_event_pool;
 
@@ -386,6 +387,7 @@ static hipEvent_t get_event()
 TimedLaunch::TimedLaunch(const char *cls, double bytes)
 {
    if (!g_timing || !cls) { return; }
+   if (!g_timing_only.empty() && g_timing_only != cls) { return; }
    TimingRec r;
    r.name = cls;
    r.e0 = get_event();
@@ -468,6 +470,12 @@ extern "C" {
 HYPRE_Int HYPRE_MI355X_SetKernelTiming(HYPRE_Int on)
 {
    hx::g_timing = (on != 0);
+   return hx::g_error_flag;
+}
+
+HYPRE_Int HYPRE_MI355X_SetKernelTimingClass(const char *name)
+{
+   hx::g_timing_only = name ? name : "";
    return hx::g_error_flag;
 }
 

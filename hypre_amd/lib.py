@@ -126,6 +126,7 @@ SIGNATURES = {
     "HYPRE_MI355X_GenerateAniso7": (P, [I, D, D]),
     "HYPRE_MI355X_ParVectorSetRandomPM": (I, [P, I]),
     "HYPRE_MI355X_SetKernelTiming": (I, [I]),
+    "HYPRE_MI355X_SetKernelTimingClass": (I, [C.c_char_p]),
     "HYPRE_MI355X_ResetKernelTiming": (I, []),
     "HYPRE_MI355X_GetKernelTiming": (I, [C.c_char_p, DP, IP, DP]),
 }

@@ -122,6 +122,8 @@ HYPRE_Int HYPRE_MI355X_ParVectorSetRandomPM(HYPRE_ParVector v, HYPRE_Int seed);
  * events on the stream it runs on.  Classes: "spmv_A0" (fine-level SpMV of
  * the solve phase), "relax_A0", "spmv", "relax", "setup", "solve". */
 HYPRE_Int HYPRE_MI355X_SetKernelTiming(HYPRE_Int on);
+/* restrict timing to one class (NULL or "" = all classes) */
+HYPRE_Int HYPRE_MI355X_SetKernelTimingClass(const char *name);
 HYPRE_Int HYPRE_MI355X_ResetKernelTiming(void);
 HYPRE_Int HYPRE_MI355X_GetKernelTiming(const char *name, HYPRE_Real *total_ms,
                                        HYPRE_Int *launches, HYPRE_Real *bytes);
