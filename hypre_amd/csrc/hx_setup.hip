@@ -55,72 +55,134 @@ __device__ __forceinline__ int xcd_blk(int bid, int nb)
    return (nb & 7) ? bid : (bid & 7) * (nb >> 3) + (bid >> 3);
 }
 
+// inclusive prefix sum over the 64 lanes of a wave
+__device__ __forceinline__ int wave_incl_scan(int v, int lane)
+{
+   for (int d = 1; d < WAVE; d <<= 1) {
+      const int t = __shfl_up(v, d, WAVE);
+      if (lane >= d) { v += t; }
+   }
+   return v;
+}
+
+// last index s in [0, nseg) with off[s] <= q (off non-decreasing, off[0] = 0 <= q)
+__device__ __forceinline__ int seg_find(const int *off, int nseg, int q)
+{
+   int lo = 0, hi = nseg - 1;
+   while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= q) { lo = mid; } else { hi = mid - 1; }
+   }
+   return lo;
+}
+
+// lanes per row for row-parallel kernels, from the mean row length
+static int lanes_per_row(long long nnz, int n)
+{
+   double m = n > 0 ? (double) nnz / n : 0.0;
+   if (m <= 6) { return 4; }
+   if (m <= 20) { return 8; }
+   if (m <= 48) { return 16; }
+   return 32;
+}
+
+template <class F>
+static void dispatch_k(int K, F f)
+{
+   switch (K) {
+      case 4: f(std::integral_constant<int, 4>{}); break;
+      case 8: f(std::integral_constant<int, 8>{}); break;
+      case 16: f(std::integral_constant<int, 16>{}); break;
+      default: f(std::integral_constant<int, 32>{}); break;
+   }
+}
+
+// Left-to-right sum of the values held by the K lanes of a group, one value
+// per lane and step; `acc` of every lane of the group receives the fold.
+template <int K>
+__device__ __forceinline__ double group_fold(double acc, double v, bool valid)
+{
+   for (int l = 0; l < K; l++) {
+      const double x = __shfl(v, l, K);
+      const int ok = __shfl((int) valid, l, K);
+      if (ok) { acc += x; }
+   }
+   return acc;
+}
+
+__global__ void k_rowlen_max(int n, const int *__restrict__ ia, int *__restrict__ out)
+{
+   int m = 0;
+   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+      m = max(m, ia[i + 1] - ia[i]);
+   }
+   for (int d = 32; d > 0; d >>= 1) { m = max(m, __shfl_xor(m, d, WAVE)); }
+   if ((threadIdx.x & 63) == 0 && m > 0) { atomicMax(out, m); }
+}
+
+static int max_row_len(const Csr &A)
+{
+   int *flag = device_flags() + 56;
+   dzero(flag, sizeof(int));
+   int nb = std::max(1, std::min((A.nrows + 255) / 256, 2048));
+   k_rowlen_max<<<nb, 256, 0, stream()>>>(A.nrows, A.ia, flag);
+   HX_LAUNCH_CHECK();
+   int h = 0;
+   d2h(&h, flag, sizeof(int));
+   return h;
+}
+
 // ================================================================== strength
-// src/parcsr_ls/par_strength.c:75-525 (CreateSHost, num_functions 1, one rank)
-__device__ __forceinline__ void strength_row(int i, const int *__restrict__ ia,
-                                             const double *__restrict__ a, double theta,
-                                             double max_row_sum, double &thr, int &mode)
+// src/parcsr_ls/par_strength.c:75-525 (CreateSHost, num_functions 1, one rank).
+// K lanes per row; row_scale is a min/max (order-free), the row sum is folded
+// in column order, S keeps the column order of A.
+template <int K, bool FILL>
+__global__ void k_strength(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                           const double *__restrict__ a, double theta, double mrs,
+                           int *__restrict__ S_ia_or_cnt, int *__restrict__ S_ja)
 {
-   // mode: 0 all weak, 1 strong iff a > thr (diag < 0), 2 strong iff a < thr
-   int s = ia[i], e = ia[i + 1];
-   double diag = a[s];
-   double row_scale = 0.0, row_sum = diag;
-   if (diag < 0) {
-      for (int j = s + 1; j < e; j++) {
-         double v = a[j];
-         row_scale = (row_scale < v) ? v : row_scale;
-         row_sum += v;
+   const int gl = threadIdx.x & (K - 1);
+   const int rows_per_block = blockDim.x / K;
+   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < n;
+        rb += (long long) gridDim.x * rows_per_block) {
+      const long long ir = rb + threadIdx.x / K;
+      const bool act = ir < n;
+      const int i = act ? (int) ir : 0;
+      const int s = act ? ia[i] : 0, e = act ? ia[i + 1] : 0;
+      const double diag = act ? a[s] : 0.0;
+      const bool neg = diag < 0;
+      // pass 1: row_scale and the ordered row sum
+      double scale = 0.0, rsum = diag;
+      for (int kb = s + 1; __any(kb < e); kb += K) {
+         const int k = kb + gl;
+         const bool valid = k < e;
+         const double v = valid ? a[k] : 0.0;
+         if (valid) { scale = neg ? ((scale < v) ? v : scale) : ((scale < v) ? scale : v); }
+         rsum = group_fold<K>(rsum, v, valid);
       }
-   } else {
-      for (int j = s + 1; j < e; j++) {
-         double v = a[j];
-         row_scale = (row_scale < v) ? row_scale : v;
-         row_sum += v;
+      for (int d = K / 2; d > 0; d >>= 1) {
+         const double o = __shfl_xor(scale, d, K);
+         scale = neg ? ((scale < o) ? o : scale) : ((scale < o) ? scale : o);
       }
-   }
-   if ((fabs(row_sum) > fabs(diag) * max_row_sum) && (max_row_sum < 1.0)) {
-      mode = 0;
-   } else {
-      mode = (diag < 0) ? 1 : 2;
-   }
-   thr = theta * row_scale;
-}
-
-__global__ void k_strength_count(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                                 const double *__restrict__ a, double theta, double mrs,
-                                 int *__restrict__ cnt)
-{
-   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-      double thr;
-      int mode;
-      strength_row(i, ia, a, theta, mrs, thr, mode);
+      const bool allweak = (fabs(rsum) > fabs(diag) * mrs) && (mrs < 1.0);
+      const double thr = theta * scale;
+      // pass 2: strong entries, positions by group ballot prefix
+      int o = FILL && act ? S_ia_or_cnt[i] : 0;
       int c = 0;
-      if (mode) {
-         for (int j = ia[i] + 1; j < ia[i + 1]; j++) {
-            double v = a[j];
-            bool strong = (mode == 1) ? !(v <= thr) : !(v >= thr);
-            c += strong;
+      const int gshift = (threadIdx.x & 63) & ~(K - 1);
+      const unsigned long long gmask = (K == 64) ? ~0ULL : ((1ULL << K) - 1);
+      for (int kb = s + 1; __any(kb < e); kb += K) {
+         const int k = kb + gl;
+         bool strong = false;
+         if (k < e && !allweak) {
+            const double v = a[k];
+            strong = neg ? !(v <= thr) : !(v >= thr);
          }
+         const unsigned long long m = (__ballot(strong) >> gshift) & gmask;
+         if (FILL && strong) { S_ja[o + c + __popcll(m & ((1ULL << gl) - 1))] = ja[k]; }
+         c += __popcll(m);
       }
-      cnt[i] = c;
-   }
-}
-
-__global__ void k_strength_fill(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                                const double *__restrict__ a, double theta, double mrs,
-                                const int *__restrict__ S_ia, int *__restrict__ S_ja)
-{
-   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-      double thr;
-      int mode;
-      strength_row(i, ia, a, theta, mrs, thr, mode);
-      if (!mode) { continue; }
-      int o = S_ia[i];
-      for (int j = ia[i] + 1; j < ia[i + 1]; j++) {
-         double v = a[j];
-         bool strong = (mode == 1) ? !(v <= thr) : !(v >= thr);
-         if (strong) { S_ja[o++] = ja[j]; }
-      }
+      if (!FILL && act && gl == 0) { S_ia_or_cnt[i] = c; }
    }
 }
 
@@ -131,28 +193,49 @@ void create_s(const Csr &A, double theta, double max_row_sum, Csr &S)
    S.nrows = n;
    S.ncols = A.ncols;
    S.ia = dalloc_n<int>((size_t) n + 1);
-   int nb = std::max(1, std::min((n + 255) / 256, 8192));
-   k_strength_count<<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, theta, max_row_sum, S.ia);
+   const int K = lanes_per_row(A.nnz, n);
+   const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
+   dispatch_k(K, [&](auto kc) {
+      k_strength<decltype(kc)::value, false><<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, theta,
+                                                                        max_row_sum, S.ia, nullptr);
+   });
    HX_LAUNCH_CHECK();
    S.nnz = exclusive_scan(S.ia, S.ia, n);
    S.ja = dalloc_n<int>((size_t) std::max<long long>(S.nnz, 1));
    S.a = nullptr;
-   k_strength_fill<<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, theta, max_row_sum, S.ia, S.ja);
+   dispatch_k(K, [&](auto kc) {
+      k_strength<decltype(kc)::value, true><<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, theta,
+                                                                       max_row_sum, S.ia, S.ja);
+   });
    HX_LAUNCH_CHECK();
 }
 
 // ================================================================== l1 norms
-// src/parcsr_ls/ams.c:4523-4600 option 1 + ams.c:770-797
+// src/parcsr_ls/ams.c:4523-4600 option 1 + ams.c:770-797; K lanes per row,
+// |a_ij| summed in column order.
+template <int K>
 __global__ void k_l1(int n, const int *__restrict__ ia, const double *__restrict__ a,
                      double *__restrict__ l1, int *__restrict__ zero_flag)
 {
-   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+   const int gl = threadIdx.x & (K - 1);
+   const int rows_per_block = blockDim.x / K;
+   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < n;
+        rb += (long long) gridDim.x * rows_per_block) {
+      const long long ir = rb + threadIdx.x / K;
+      const bool act = ir < n;
+      const int i = act ? (int) ir : 0;
+      const int b = act ? ia[i] : 0, e = act ? ia[i + 1] : 0;
       double s = 0.0;
-      int b = ia[i], e = ia[i + 1];
-      for (int j = b; j < e; j++) { s += fabs(a[j]); }
-      if (a[b] < 0.0) { s = -s; }
-      if (fabs(s) == 0.0) { *zero_flag = 1; }
-      l1[i] = s;
+      for (int kb = b; __any(kb < e); kb += K) {
+         const int k = kb + gl;
+         const bool valid = k < e;
+         s = group_fold<K>(s, valid ? fabs(a[k]) : 0.0, valid);
+      }
+      if (act && gl == 0) {
+         if (a[b] < 0.0) { s = -s; }
+         if (fabs(s) == 0.0) { *zero_flag = 1; }
+         l1[i] = s;
+      }
    }
 }
 
@@ -160,8 +243,12 @@ void l1_norms(const Csr &A, double *l1)
 {
    int *flag = device_flags() + 40;
    dzero(flag, sizeof(int));
-   int nb = std::max(1, std::min((A.nrows + 255) / 256, 8192));
-   k_l1<<<nb, 256, 0, stream()>>>(A.nrows, A.ia, A.a, l1, flag);
+   const int n = A.nrows;
+   const int K = lanes_per_row(A.nnz, n);
+   const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
+   dispatch_k(K, [&](auto kc) {
+      k_l1<decltype(kc)::value><<<nb, 256, 0, stream()>>>(n, A.ia, A.a, l1, flag);
+   });
    HX_LAUNCH_CHECK();
    int h = 0;
    d2h(&h, flag, sizeof(int));
@@ -721,12 +808,14 @@ __global__ __launch_bounds__(WAVE) void k_extpi(InterpArgs p)
 // global-memory slabs.
 template <class Args, class Launch>
 static void deferred_passes(Args a, int nwork_all, int *ndefer, const int *lds_caps, int nlds,
-                            int max_global_cap, int ntables, Launch launch)
+                            int max_global_cap, int ntables, Launch launch,
+                            const int *init_list = nullptr, int init_n = 0)
 {
    int *h = static_cast<int *>(pinned_scratch(64));
-   int *lists[2] = {dalloc_n<int>((size_t) nwork_all + 1), dalloc_n<int>((size_t) nwork_all + 1)};
-   a.rowlist = nullptr;
-   a.nwork = nwork_all;
+   const int nl = init_list ? init_n : nwork_all;
+   int *lists[2] = {dalloc_n<int>((size_t) nl + 1), dalloc_n<int>((size_t) nl + 1)};
+   a.rowlist = init_list;
+   a.nwork = nl;
    int cur = 0;
    int cap = lds_caps[0];
    int stage = 0;
@@ -773,7 +862,7 @@ static void deferred_passes(Args a, int nwork_all, int *ndefer, const int *lds_c
 }
 
 template <int MODE>
-static void extpi_pass(InterpArgs a)
+static void extpi_pass(InterpArgs a, const int *init_list = nullptr, int init_n = 0)
 {
    static const int caps[2] = {512, 4096};
    deferred_passes(a, a.n, device_flags() + 48, caps, 2, 1 << 22, 1,
@@ -781,7 +870,365 @@ static void extpi_pass(InterpArgs a)
                       size_t lds = x.gslab ? 0 : table_bytes(x.cap);
                       k_extpi<MODE><<<nb, WAVE, lds, stream()>>>(x);
                       HX_LAUNCH_CHECK();
-                   });
+                   },
+                   init_list, init_n);
+}
+
+// ------------------------------------------------------------------ ext+i, fast path
+// Same arithmetic as k_extpi, restructured so that a row costs a handful of
+// dependent global loads instead of one per neighbour:
+//   * C-hat is built over the flattened list "strong neighbours, with each
+//     strong F neighbour expanded to its strong row", 64 candidates per step;
+//     the first occurrence of a key is the one whose position wins an atomicMin
+//     on the slot's stamp, and first occurrences get their C-hat positions by a
+//     ballot prefix, which is exactly the serial first-seen order;
+//   * one lane per off-diagonal entry of row i; a strong-F lane walks row i1
+//     itself (denominator in column order, then the distributed values);
+//   * every contribution to weight v lands in M[r][v], r = rank of the
+//     contributing entry of row i, and lane v folds column v of M from r = 0,
+//     i.e. in the entry order of row i.  A missing contribution is +0.0, which
+//     leaves a sum unchanged (the sum starts at +0.0 and never becomes -0.0);
+//   * truncation runs afterwards in k_trunc_rows, one thread per row.
+// Rows with more than 64 strong neighbours, more than XF_MC C-hat points or
+// more than `mr` contributing entries are handed to k_extpi.
+constexpr int XF_HC = 256;   // hash slots (<= 64 F markers + <= XF_MC + 64 C keys)
+constexpr int XF_MC = 32;    // C-hat points per row on the fast path
+
+struct ExtpiFastArgs
+{
+   int           n;
+   const int    *A_ia, *A_ja;
+   const double *A_a;
+   const int    *S_ia, *S_ja;
+   const int    *CF, *f2c;
+   int           mr;          // rows of M
+   int          *raw_j;       // row i at i * XF_MC
+   double       *raw_a;
+   int          *cnt;         // entries of the raw row, -1 = deferred
+   int          *defer_list;
+   int          *ndefer;
+};
+
+static size_t extpi_fast_lds(int mr)
+{
+   return (size_t) mr * XF_MC * 8 + (size_t) XF_HC * 12 + (65 + 64 + 64) * 4 + XF_MC * 8;
+}
+
+__global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
+{
+   extern __shared__ double dyn_lds[];
+   const int lane = threadIdx.x;
+   const int nb = gridDim.x;
+   const int blk = xcd_blk(blockIdx.x, nb);
+   double *M = dyn_lds;
+   int *keys = reinterpret_cast<int *>(M + (size_t) p.mr * XF_MC);
+   int *vals = keys + XF_HC;
+   int *stamp = vals + XF_HC;
+   int *seg_off = stamp + XF_HC;   // 65
+   int *seg_b = seg_off + 65;      // 64
+   int *seg_kind = seg_b + 64;     // 64
+   int *chat_key = seg_kind + 64;  // XF_MC
+   int *chat_col = chat_key + XF_MC;
+   RowHash H{keys, vals, XF_HC, ilog2(XF_HC)};
+   const int per = (p.n + nb - 1) / nb;
+   const int r0 = blk * per, r1 = min(p.n, r0 + per);
+
+   for (int i = r0; i < r1; i++) {
+      const int cfi = p.CF[i];
+      if (cfi >= 0 || cfi == -3) {
+         if (lane == 0) {
+            if (cfi >= 0) {
+               p.raw_j[(size_t) i * XF_MC] = p.f2c[i];
+               p.raw_a[(size_t) i * XF_MC] = 1.0;
+               p.cnt[i] = 1;
+            } else {
+               p.cnt[i] = 0;
+            }
+         }
+         continue;
+      }
+      const int s0 = p.S_ia[i], ns = p.S_ia[i + 1] - s0;
+      const int a0 = p.A_ia[i], na = p.A_ia[i + 1] - a0 - 1;
+      bool defer = ns > WAVE;
+      if (!defer) {
+         for (int s = lane; s < XF_HC; s += WAVE) { keys[s] = -1; stamp[s] = 0x7fffffff; }
+         // ---- strong neighbours: kind 1 = C, 2 = F (par_lr_interp.c:1505-1560)
+         int i1 = -1, kind = 0, kb = 0, klen = 0;
+         if (lane < ns) {
+            i1 = p.S_ja[s0 + lane];
+            const int c1 = p.CF[i1];
+            if (c1 >= 0) {
+               kind = 1; klen = 1; kb = i1;
+            } else if (c1 != -3) {
+               kind = 2; kb = p.S_ia[i1]; klen = p.S_ia[i1 + 1] - kb;
+            }
+         }
+         wave_sync();
+         if (kind == 2) {
+            bool isn;
+            const int sl = H.insert(i1, isn);
+            if (sl >= 0) { vals[sl] = -2; }
+         }
+         const int incl = wave_incl_scan(klen, lane);
+         const int L = readlane_i(incl, WAVE - 1);
+         seg_off[lane] = incl - klen;
+         seg_b[lane] = kb;
+         seg_kind[lane] = kind;
+         wave_sync();
+         // ---- C-hat in first-seen order
+         int nch = 0;
+         const int nseg = max(ns, 1);
+         for (int cb = 0; cb < L; cb += WAVE) {
+            const int q = cb + lane;
+            int key = -1;
+            if (q < L) {
+               const int s = seg_find(seg_off, nseg, q);
+               if (seg_kind[s] == 1) {
+                  key = seg_b[s];
+               } else {
+                  const int k1 = p.S_ja[seg_b[s] + (q - seg_off[s])];
+                  if (p.CF[k1] >= 0) { key = k1; }
+               }
+            }
+            int sl = -1;
+            if (key >= 0) {
+               bool isn;
+               sl = H.insert(key, isn);
+               if (sl >= 0) { atomicMin(&stamp[sl], q); }
+            }
+            wave_sync();
+            const bool first = (sl >= 0) && stamp[sl] == q;
+            const unsigned long long fm = __ballot(first);
+            const int nnew = __popcll(fm);
+            if (nch + nnew > XF_MC) { defer = true; break; }
+            if (first) {
+               const int pos = nch + __popcll(fm & lanemask_lt(lane));
+               vals[sl] = pos;
+               chat_key[pos] = key;
+            }
+            nch += nnew;
+            wave_sync();
+         }
+         if (!defer) {
+            if (lane < nch) { chat_col[lane] = p.f2c[chat_key[lane]]; }
+            for (int t = lane; t < p.mr * XF_MC; t += WAVE) { M[t] = 0.0; }
+            wave_sync();
+            // ---- weights (par_lr_interp.c:1640-1805)
+            double diag = p.A_a[a0];
+            int mrow = 0;
+            for (int jb = 0; jb < na; jb += WAVE) {
+               const int j = jb + lane;
+               int c1 = 0, v = -1;
+               double aij = 0.0;
+               if (j < na) {
+                  c1 = p.A_ja[a0 + 1 + j];
+                  aij = p.A_a[a0 + 1 + j];
+                  const int sl = H.find(c1);
+                  v = (sl >= 0) ? vals[sl] : -1;
+               }
+               const bool contrib = (j < na) && (v >= 0 || v == -2);
+               const unsigned long long cm = __ballot(contrib);
+               if (mrow + __popcll(cm) > p.mr) { defer = true; break; }
+               const int myrow = mrow + __popcll(cm & lanemask_lt(lane));
+               mrow += __popcll(cm);
+               bool hasd = false;
+               double dval = 0.0;
+               if (j < na) {
+                  if (v >= 0) {
+                     M[myrow * XF_MC + v] = aij;
+                  } else if (v == -2) {
+                     const int b1 = p.A_ia[c1], e1 = p.A_ia[c1 + 1];
+                     const double sgn = (p.A_a[b1] < 0) ? -1.0 : 1.0;
+                     double sum = 0.0;
+                     for (int k = b1 + 1; k < e1; k++) {
+                        const int i2 = p.A_ja[k];
+                        const double a2 = p.A_a[k];
+                        if ((sgn * a2) < 0) {
+                           bool q2 = (i2 == i);
+                           if (!q2) {
+                              const int s2 = H.find(i2);
+                              q2 = (s2 >= 0) && vals[s2] >= 0;
+                           }
+                           if (q2) { sum += a2; }
+                        }
+                     }
+                     if (sum != 0) {
+                        const double distribute = aij / sum;
+                        for (int k = b1 + 1; k < e1; k++) {
+                           const int i2 = p.A_ja[k];
+                           const double a2 = p.A_a[k];
+                           if ((sgn * a2) < 0) {
+                              if (i2 == i) {
+                                 hasd = true;
+                                 dval = distribute * a2;
+                              } else {
+                                 const int s2 = H.find(i2);
+                                 const int v2 = (s2 >= 0) ? vals[s2] : -1;
+                                 if (v2 >= 0) { M[myrow * XF_MC + v2] = distribute * a2; }
+                              }
+                           }
+                        }
+                     } else {
+                        hasd = true;
+                        dval = aij;
+                     }
+                  } else if (p.CF[c1] != -3) {
+                     hasd = true;                     // weak connection
+                     dval = aij;
+                  }
+               }
+               unsigned long long dm = __ballot(hasd);
+               while (dm) {
+                  const int t = __ffsll((long long) dm) - 1;
+                  diag += readlane_d(dval, t);
+                  dm &= dm - 1;
+               }
+            }
+            if (!defer) {
+               wave_sync();
+               if (lane < nch) {
+                  double w = 0.0;
+                  for (int r = 0; r < mrow; r++) { w += M[r * XF_MC + lane]; }
+                  if (diag != 0.0) { w /= -diag; }
+                  p.raw_j[(size_t) i * XF_MC + lane] = chat_col[lane];
+                  p.raw_a[(size_t) i * XF_MC + lane] = w;
+               }
+               if (lane == 0) { p.cnt[i] = nch; }
+            }
+         }
+      }
+      if (defer && lane == 0) {
+         p.cnt[i] = -1;
+         const int d = atomicAdd(p.ndefer, 1);
+         p.defer_list[d] = i;
+      }
+      wave_sync();
+   }
+}
+
+// hypre_qsort2_abs (src/utilities/qsort.c:452) on a row stored with stride SS,
+// refined only as far as the first `keep` positions need; explicit stack.
+template <int SS>
+__device__ void qsort2_abs_strided(int *v, double *w, int cnt, int keep, int *stl, int *str,
+                                   int stack_cap)
+{
+   int sp = 0;
+   stl[0] = 0; str[0] = cnt - 1; sp = 1;
+   while (sp > 0) {
+      sp--;
+      const int left = stl[sp * SS], right = str[sp * SS];
+      if (left >= right || left >= keep) { continue; }
+      const int mid = (left + right) / 2;
+      { int t = v[left * SS]; v[left * SS] = v[mid * SS]; v[mid * SS] = t;
+        double tw = w[left * SS]; w[left * SS] = w[mid * SS]; w[mid * SS] = tw; }
+      int last = left;
+      const double piv = fabs(w[left * SS]);
+      for (int i = left + 1; i <= right; i++) {
+         if (fabs(w[i * SS]) > piv) {
+            ++last;
+            int t = v[last * SS]; v[last * SS] = v[i * SS]; v[i * SS] = t;
+            double tw = w[last * SS]; w[last * SS] = w[i * SS]; w[i * SS] = tw;
+         }
+      }
+      { int t = v[left * SS]; v[left * SS] = v[last * SS]; v[last * SS] = t;
+        double tw = w[left * SS]; w[left * SS] = w[last * SS]; w[last * SS] = tw; }
+      const int l1 = left, r1 = last - 1, l2 = last + 1, r2 = right;
+      // push the larger side first so the smaller one is refined next; a side
+      // that cannot reach the first `keep` positions is dropped here
+      const bool big1 = (r1 - l1) > (r2 - l2);
+      const int la = big1 ? l1 : l2, ra = big1 ? r1 : r2;
+      const int lb = big1 ? l2 : l1, rb = big1 ? r2 : r1;
+      if (la < ra && la < keep && sp < stack_cap) { stl[sp * SS] = la; str[sp * SS] = ra; sp++; }
+      if (lb < rb && lb < keep && sp < stack_cap) { stl[sp * SS] = lb; str[sp * SS] = rb; sp++; }
+   }
+}
+
+// par_csr_matrix.c:3040-3320 (trunc_factor drop + rescale, then P_max_elmts
+// with qsort2_abs and rescale), one thread per row, rows staged in LDS with a
+// stride of 65 so that a wave's accesses fall into distinct banks.
+constexpr int TR_ROWS = 64;
+constexpr int TR_SS = 65;
+constexpr int TR_STACK = 16;
+
+__global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__restrict__ cnt_raw,
+                                                        const int *__restrict__ raw_j,
+                                                        const double *__restrict__ raw_a,
+                                                        double trunc_factor, int max_elmts,
+                                                        int out_stride, int *__restrict__ out_cnt,
+                                                        int *__restrict__ out_j,
+                                                        double *__restrict__ out_a)
+{
+   __shared__ double sw[XF_MC * TR_SS];
+   __shared__ int sj[XF_MC * TR_SS];
+   __shared__ int stl[TR_STACK * TR_SS], str[TR_STACK * TR_SS];
+   __shared__ int sc[TR_ROWS];
+   const int t = threadIdx.x;
+   for (long long base = (long long) blockIdx.x * TR_ROWS; base < n;
+        base += (long long) gridDim.x * TR_ROWS) {
+      const long long i = base + t;
+      sc[t] = (i < n) ? cnt_raw[i] : -1;
+      __syncthreads();
+      for (int k = t; k < TR_ROWS * XF_MC; k += TR_ROWS) {
+         const int r = k / XF_MC, e = k % XF_MC;
+         if (e < sc[r]) {
+            sj[e * TR_SS + r] = raw_j[(size_t) (base + r) * XF_MC + e];
+            sw[e * TR_SS + r] = raw_a[(size_t) (base + r) * XF_MC + e];
+         }
+      }
+      __syncthreads();
+      const int c = sc[t];
+      if (c >= 0) {
+         int *vj = sj + t;
+         double *vw = sw + t;
+         int kept = c;
+         if (trunc_factor > 0) {
+            double row_nrm = 0.0;
+            for (int k = 0; k < c; k++) {
+               const double a = fabs(vw[k * TR_SS]);
+               row_nrm = (row_nrm < a) ? a : row_nrm;
+            }
+            const double drop_coeff = trunc_factor * row_nrm;
+            double row_sum = 0.0, scale = 0.0;
+            int o = 0;
+            for (int k = 0; k < c; k++) {
+               const double w = vw[k * TR_SS];
+               row_sum += w;
+               if (fabs(w) < drop_coeff) { continue; }
+               scale += w;
+               vw[o * TR_SS] = w;
+               vj[o * TR_SS] = vj[k * TR_SS];
+               o++;
+            }
+            if (scale != 0.) {
+               if (scale != row_sum) {
+                  scale = row_sum / scale;
+                  for (int k = 0; k < o; k++) { vw[k * TR_SS] *= scale; }
+               }
+            }
+            kept = o;
+         }
+         if (max_elmts > 0 && kept > max_elmts) {
+            double row_sum = 0.0;
+            for (int k = 0; k < kept; k++) { row_sum += vw[k * TR_SS]; }
+            qsort2_abs_strided<TR_SS>(vj, vw, kept, max_elmts, stl + t, str + t, TR_STACK);
+            double scale = 0.0;
+            for (int k = 0; k < max_elmts; k++) { scale += vw[k * TR_SS]; }
+            if (scale != 0.) {
+               if (scale != row_sum) {
+                  scale = row_sum / scale;
+                  for (int k = 0; k < max_elmts; k++) { vw[k * TR_SS] *= scale; }
+               }
+            }
+            kept = max_elmts;
+         }
+         out_cnt[i] = kept;
+         for (int k = 0; k < kept; k++) {
+            out_j[(size_t) i * out_stride + k] = vj[k * TR_SS];
+            out_a[(size_t) i * out_stride + k] = vw[k * TR_SS];
+         }
+      }
+      __syncthreads();
+   }
 }
 
 __global__ void k_compact_rows(int n, const int *__restrict__ base, const int *__restrict__ cnt,
@@ -823,23 +1270,66 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
    int *tmp_j = nullptr;
    double *tmp_a = nullptr;
    int stride = 0;
-   if (max_elmts > 0) {
+   if (max_elmts > 0 && max_elmts <= XF_MC) {
+      // fast path into raw rows, truncation per thread, leftovers by k_extpi
       stride = max_elmts;
-      tmp_j = dalloc_n<int>((size_t) n * stride);
-      tmp_a = dalloc_n<double>((size_t) n * stride);
+      const int mr = std::max(1, std::min(64, max_row_len(A) - 1));
+      ExtpiFastArgs f{};
+      f.n = n;
+      f.A_ia = A.ia; f.A_ja = A.ja; f.A_a = A.a;
+      f.S_ia = S.ia; f.S_ja = S.ja;
+      f.CF = CF; f.f2c = f2c;
+      f.mr = mr;
+      int *raw_cnt = dalloc_n<int>((size_t) n + 1);
+      f.raw_j = dalloc_n<int>((size_t) std::max(n, 1) * XF_MC);
+      f.raw_a = dalloc_n<double>((size_t) std::max(n, 1) * XF_MC);
+      f.cnt = raw_cnt;
+      int *dlist = dalloc_n<int>((size_t) n + 1);
+      f.defer_list = dlist;
+      f.ndefer = device_flags() + 60;
+      dzero(f.ndefer, sizeof(int));
+      k_extpi_fast<<<launch_blocks(n), WAVE, extpi_fast_lds(mr), stream()>>>(f);
+      HX_LAUNCH_CHECK();
+      tmp_j = dalloc_n<int>((size_t) std::max(n, 1) * stride);
+      tmp_a = dalloc_n<double>((size_t) std::max(n, 1) * stride);
+      const int nbt = (int) std::max<long long>(1, std::min<long long>((n + TR_ROWS - 1) / TR_ROWS, 16384));
+      k_trunc_rows<<<nbt, TR_ROWS, 0, stream()>>>(n, raw_cnt, f.raw_j, f.raw_a, trunc_factor,
+                                                  max_elmts, stride, cnt, tmp_j, tmp_a);
+      HX_LAUNCH_CHECK();
+      int *h = static_cast<int *>(pinned_scratch(64));
+      HX_HIP(hipMemcpyAsync(h, f.ndefer, sizeof(int), hipMemcpyDeviceToHost, stream()));
+      sync();
+      const int nd = h[0];
+      dfree(f.raw_j);
+      dfree(f.raw_a);
+      dfree(raw_cnt);
+      if (nd > 0) {
+         a.base = nullptr;
+         a.out_j = tmp_j;
+         a.out_a = tmp_a;
+         a.cnt = cnt;
+         extpi_pass<MODE_NUM>(a, dlist, nd);
+      }
+      dfree(dlist);
    } else {
+      if (max_elmts > 0) {
+         stride = max_elmts;
+         tmp_j = dalloc_n<int>((size_t) n * stride);
+         tmp_a = dalloc_n<double>((size_t) n * stride);
+      } else {
+         a.cnt = cnt;
+         extpi_pass<MODE_SYM>(a);
+         base = dalloc_n<int>((size_t) n + 1);
+         long long tot = exclusive_scan(cnt, base, n);
+         tmp_j = dalloc_n<int>((size_t) std::max<long long>(tot, 1));
+         tmp_a = dalloc_n<double>((size_t) std::max<long long>(tot, 1));
+      }
+      a.base = base;
+      a.out_j = tmp_j;
+      a.out_a = tmp_a;
       a.cnt = cnt;
-      extpi_pass<MODE_SYM>(a);
-      base = dalloc_n<int>((size_t) n + 1);
-      long long tot = exclusive_scan(cnt, base, n);
-      tmp_j = dalloc_n<int>((size_t) std::max<long long>(tot, 1));
-      tmp_a = dalloc_n<double>((size_t) std::max<long long>(tot, 1));
+      extpi_pass<MODE_NUM>(a);
    }
-   a.base = base;
-   a.out_j = tmp_j;
-   a.out_a = tmp_a;
-   a.cnt = cnt;
-   extpi_pass<MODE_NUM>(a);
    P.nrows = n;
    P.ncols = nc;
    P.ia = dalloc_n<int>((size_t) n + 1);
@@ -873,23 +1363,50 @@ __global__ void k_t_fill(int n, const int *__restrict__ ia, const int *__restric
    }
 }
 
-__global__ void k_t_sort(int nc, const int *__restrict__ Ria, int *__restrict__ Rj,
-                         double *__restrict__ Ra)
+// Rows of R hold distinct fine indices: each entry's final place is its rank
+// in the row.  One wave per row, the row staged in LDS; rows longer than the
+// stage are sorted by insertion in lane 0.
+constexpr int TS_MAX = 512;
+
+__global__ __launch_bounds__(WAVE) void k_t_sort(int nc, const int *__restrict__ Ria,
+                                                 int *__restrict__ Rj, double *__restrict__ Ra)
 {
-   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nc; r += gridDim.x * blockDim.x) {
-      int b = Ria[r], e = Ria[r + 1];
-      for (int k = b + 1; k < e; k++) {
-         int key = Rj[k];
-         double v = Ra[k];
-         int m = k - 1;
-         while (m >= b && Rj[m] > key) {
-            Rj[m + 1] = Rj[m];
-            Ra[m + 1] = Ra[m];
-            m--;
+   __shared__ int sk[TS_MAX];
+   __shared__ double sv[TS_MAX];
+   const int lane = threadIdx.x;
+   for (int r = blockIdx.x; r < nc; r += gridDim.x) {
+      const int b = Ria[r], e = Ria[r + 1], L = e - b;
+      if (L <= 1) { continue; }
+      if (L > TS_MAX) {
+         if (lane == 0) {
+            for (int k = b + 1; k < e; k++) {
+               const int key = Rj[k];
+               const double v = Ra[k];
+               int m = k - 1;
+               while (m >= b && Rj[m] > key) {
+                  Rj[m + 1] = Rj[m];
+                  Ra[m + 1] = Ra[m];
+                  m--;
+               }
+               Rj[m + 1] = key;
+               Ra[m + 1] = v;
+            }
          }
-         Rj[m + 1] = key;
-         Ra[m + 1] = v;
+         continue;
       }
+      for (int k = lane; k < L; k += WAVE) {
+         sk[k] = Rj[b + k];
+         sv[k] = Ra[b + k];
+      }
+      wave_sync();
+      for (int k = lane; k < L; k += WAVE) {
+         const int key = sk[k];
+         int rank = 0;
+         for (int m = 0; m < L; m++) { rank += (sk[m] < key) ? 1 : 0; }
+         Rj[b + rank] = key;
+         Ra[b + rank] = sv[k];
+      }
+      wave_sync();
    }
 }
 
@@ -913,8 +1430,7 @@ void transpose(const Csr &P, Csr &R)
    d2d(fillp, R.ia, sizeof(int) * ((size_t) nc + 1));
    int nb = std::max(1, std::min((n + 255) / 256, 8192));
    k_t_fill<<<nb, 256, 0, stream()>>>(n, P.ia, P.ja, P.a, fillp, R.ja, R.a);
-   int nbc = std::max(1, std::min((nc + 255) / 256, 8192));
-   k_t_sort<<<nbc, 256, 0, stream()>>>(nc, R.ia, R.ja, R.a);
+   k_t_sort<<<launch_blocks(nc), WAVE, 0, stream()>>>(nc, R.ia, R.ja, R.a);
    HX_LAUNCH_CHECK();
    dfree(fillp);
 }
@@ -1071,7 +1587,7 @@ __global__ __launch_bounds__(WAVE) void k_rap(RapArgs p)
 }
 
 template <int MODE>
-static void rap_pass(RapArgs a)
+static void rap_pass(RapArgs a, const int *init_list = nullptr, int init_n = 0)
 {
    static const int caps[2] = {512, 2048};
    deferred_passes(a, a.nc, device_flags() + 52, caps, 2, 1 << 22, 2,
@@ -1079,7 +1595,249 @@ static void rap_pass(RapArgs a)
                       size_t lds = x.gslab ? 0 : 2 * table_bytes(x.cap);
                       k_rap<MODE><<<nb, WAVE, lds, stream()>>>(x);
                       HX_LAUNCH_CHECK();
-                   });
+                   },
+                   init_list, init_n);
+}
+
+// ------------------------------------------------------------------ RAP, fast path
+// One numeric pass per coarse row with the arithmetic of k_rap:
+//   * RA row: the A rows named by R(ic,:) are flattened and read 64 entries
+//     per step; the step is applied one A row (segment) at a time, in R order,
+//     so a new column is created by the first product and later products add
+//     to it in the serial order; no two lanes of one segment share a column;
+//   * Ac row: diagonal first, then the P rows of the RA entries in first-seen
+//     order, flattened the same way.  Products of one step that hit the same
+//     coarse column are applied in rounds: each round, the lowest pending lane
+//     of every column (atomicMin on the slot) applies its product, so each
+//     column receives its products in traversal order, and columns created in
+//     a step get positions in the order of their first product.
+// Output rows go to a stride-RF_ACL buffer; rows whose RA row exceeds `rcl`
+// entries or whose Ac row exceeds RF_ACL entries are deferred.
+constexpr int RF_AC = 256;
+constexpr int RF_ACL = 128;
+
+struct RapFastArgs
+{
+   int           nc;
+   const int    *R_ia, *R_ja;
+   const double *R_a;
+   const int    *A_ia, *A_ja;
+   const double *A_a;
+   const int    *P_ia, *P_ja;
+   const double *P_a;
+   const int    *rowlist;
+   int           nwork;
+   int           rc;          // RA hash slots (power of two)
+   int           rcl;         // RA entries (<= rc - 64)
+   int          *tmp_j;       // row ic at ic * RF_ACL
+   double       *tmp_a;
+   int          *cnt;         // -1 = deferred
+   int          *defer_list;
+   int          *ndefer;
+};
+
+static size_t rap_fast_lds(int rc, int rcl)
+{
+   return (size_t) rc * 12 + (size_t) rcl * 4 + (size_t) RF_AC * 20 + RF_ACL * 4 + 64 * 8 + 129 * 4;
+}
+
+__global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
+{
+   extern __shared__ double dyn_lds[];
+   const int lane = threadIdx.x;
+   const int nb = gridDim.x;
+   const int blk = xcd_blk(blockIdx.x, nb);
+   double *ra_val = dyn_lds;                    // rc
+   double *ac_val = ra_val + p.rc;              // RF_AC
+   double *seg_r = ac_val + RF_AC;              // 64
+   int *ra_keys = reinterpret_cast<int *>(seg_r + 64);
+   int *ra_order = ra_keys + p.rc;              // rcl
+   int *ac_keys = ra_order + p.rcl;             // RF_AC
+   int *ac_pos = ac_keys + RF_AC;
+   int *ac_mins = ac_pos + RF_AC;
+   int *ac_order = ac_mins + RF_AC;             // RF_ACL
+   int *seg_off = ac_order + RF_ACL;            // 65
+   int *seg_b = seg_off + 65;                   // 64
+   RowHash HR{ra_keys, nullptr, p.rc, ilog2(p.rc)};
+   RowHash HC{ac_keys, nullptr, RF_AC, ilog2(RF_AC)};
+   const int per = (p.nwork + nb - 1) / nb;
+   const int w0 = blk * per, w1 = min(p.nwork, w0 + per);
+
+   for (int wi = w0; wi < w1; wi++) {
+      const int ic = p.rowlist ? p.rowlist[wi] : wi;
+      for (int s = lane; s < p.rc; s += WAVE) { ra_keys[s] = -1; }
+      for (int s = lane; s < RF_AC; s += WAVE) {
+         ac_keys[s] = -1;
+         ac_pos[s] = -1;
+         ac_mins[s] = 0x7fffffff;
+      }
+      wave_sync();
+      bool defer = false;
+      // ---- row ic of RA (par_rap.c:1712-1800)
+      int nra = 0;
+      const int rb0 = p.R_ia[ic], rb1 = p.R_ia[ic + 1];
+      for (int rb = rb0; rb < rb1 && !defer; rb += WAVE) {
+         const int jr = rb + lane;
+         int len = 0, ab = 0;
+         double rv = 0.0;
+         if (jr < rb1) {
+            const int i1 = p.R_ja[jr];
+            rv = p.R_a[jr];
+            ab = p.A_ia[i1];
+            len = p.A_ia[i1 + 1] - ab;
+         }
+         const int incl = wave_incl_scan(len, lane);
+         const int L = readlane_i(incl, WAVE - 1);
+         const int nseg = min(WAVE, rb1 - rb);
+         seg_off[lane] = incl - len;
+         seg_b[lane] = ab;
+         seg_r[lane] = rv;
+         wave_sync();
+         for (int cb = 0; cb < L && !defer; cb += WAVE) {
+            const int q = cb + lane;
+            const bool valid = q < L;
+            int s = 0, key = -1;
+            double x = 0.0;
+            if (valid) {
+               s = seg_find(seg_off, nseg, q);
+               const int k = seg_b[s] + (q - seg_off[s]);
+               key = p.A_ja[k];
+               x = seg_r[s] * p.A_a[k];
+            }
+            const int s_lo = readlane_i(s, 0);
+            const int s_hi = readlane_i(s, min(WAVE - 1, L - 1 - cb));
+            for (int sg = s_lo; sg <= s_hi; sg++) {
+               const bool act = valid && s == sg;
+               bool isn = false;
+               int sl = -1;
+               if (act) { sl = HR.insert(key, isn); }
+               const unsigned long long nm = __ballot(isn);
+               if (nra + __popcll(nm) > p.rcl) { defer = true; break; }
+               if (act && sl >= 0) {
+                  if (isn) {
+                     const int pos = nra + __popcll(nm & lanemask_lt(lane));
+                     ra_order[pos] = sl;
+                     ra_val[sl] = x;
+                  } else {
+                     ra_val[sl] += x;
+                  }
+               }
+               nra += __popcll(nm);
+               wave_sync();
+            }
+         }
+         wave_sync();
+      }
+      // ---- (RA) x P, diagonal first (par_rap.c:1612-1620, 1860-1920)
+      int nac = 0;
+      if (!defer) {
+         if (lane == 0) {
+            bool isn;
+            const int sl = HC.insert(ic, isn);
+            ac_pos[sl] = 0;
+            ac_order[0] = sl;
+            ac_val[sl] = 0.0;
+         }
+         nac = 1;
+         wave_sync();
+         for (int kb = 0; kb < nra && !defer; kb += WAVE) {
+            const int k = kb + lane;
+            int len = 0, pb = 0;
+            double rav = 0.0;
+            if (k < nra) {
+               const int sl = ra_order[k];
+               const int i1 = ra_keys[sl];
+               rav = ra_val[sl];
+               pb = p.P_ia[i1];
+               len = p.P_ia[i1 + 1] - pb;
+            }
+            const int incl = wave_incl_scan(len, lane);
+            const int L = readlane_i(incl, WAVE - 1);
+            const int nseg = min(WAVE, nra - kb);
+            wave_sync();                         // seg arrays of the previous step are dead
+            seg_off[lane] = incl - len;
+            seg_b[lane] = pb;
+            seg_r[lane] = rav;
+            wave_sync();
+            for (int cb = 0; cb < L && !defer; cb += WAVE) {
+               const int q = cb + lane;
+               const bool valid = q < L;
+               int key = -1;
+               double x = 0.0;
+               if (valid) {
+                  const int s = seg_find(seg_off, nseg, q);
+                  const int e = seg_b[s] + (q - seg_off[s]);
+                  key = p.P_ja[e];
+                  x = seg_r[s] * p.P_a[e];
+               }
+               bool isn = false;
+               int sl = -1;
+               if (valid) { sl = HC.insert(key, isn); }
+               if (nac + __popcll(__ballot(isn)) > RF_ACL) { defer = true; break; }
+               bool pending = valid && sl >= 0;
+               while (__ballot(pending)) {
+                  if (pending) { atomicMin(&ac_mins[sl], lane); }
+                  wave_sync();
+                  const bool apply = pending && ac_mins[sl] == lane;
+                  const bool first = apply && ac_pos[sl] < 0;
+                  const unsigned long long fm = __ballot(first);
+                  if (apply) {
+                     if (first) {
+                        const int pos = nac + __popcll(fm & lanemask_lt(lane));
+                        ac_pos[sl] = pos;
+                        ac_order[pos] = sl;
+                        ac_val[sl] = x;
+                     } else {
+                        ac_val[sl] += x;
+                     }
+                     ac_mins[sl] = 0x7fffffff;
+                     pending = false;
+                  }
+                  nac += __popcll(fm);
+                  wave_sync();
+               }
+            }
+         }
+      }
+      if (!defer) {
+         for (int k = lane; k < nac; k += WAVE) {
+            const int sl = ac_order[k];
+            p.tmp_j[(size_t) ic * RF_ACL + k] = ac_keys[sl];
+            p.tmp_a[(size_t) ic * RF_ACL + k] = ac_val[sl];
+         }
+         if (lane == 0) { p.cnt[ic] = nac; }
+      } else if (lane == 0) {
+         p.cnt[ic] = -1;
+         const int d = atomicAdd(p.ndefer, 1);
+         p.defer_list[d] = ic;
+      }
+      wave_sync();
+   }
+}
+
+__global__ void k_rap_compact(int nc, const int *__restrict__ cnt, const int *__restrict__ out_ia,
+                              const int *__restrict__ tmp_j, const double *__restrict__ tmp_a,
+                              int *__restrict__ oj, double *__restrict__ oa)
+{
+   // 16 lanes per row
+   const int g = threadIdx.x & 15;
+   for (long long r = ((long long) blockIdx.x * blockDim.x + threadIdx.x) >> 4; r < nc;
+        r += ((long long) gridDim.x * blockDim.x) >> 4) {
+      const int c = cnt[r];
+      if (c < 0) { continue; }
+      const int o = out_ia[r];
+      for (int k = g; k < c; k += 16) {
+         oj[o + k] = tmp_j[(size_t) r * RF_ACL + k];
+         oa[o + k] = tmp_a[(size_t) r * RF_ACL + k];
+      }
+   }
+}
+
+__global__ void k_clamp0(int n, const int *__restrict__ in, int *__restrict__ out)
+{
+   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+      out[i] = max(in[i], 0);
+   }
 }
 
 void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
@@ -1091,20 +1849,72 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
    a.R_ia = R.ia; a.R_ja = R.ja; a.R_a = R.a;
    a.A_ia = A.ia; a.A_ja = A.ja; a.A_a = A.a;
    a.P_ia = P.ia; a.P_ja = P.ja; a.P_a = P.a;
-   int *cnt = dalloc_n<int>((size_t) nc + 1);
-   a.cnt = cnt;
-   rap_pass<MODE_SYM>(a);
    Ac.nrows = nc;
    Ac.ncols = nc;
+   Ac.a32 = nullptr;
    Ac.ia = dalloc_n<int>((size_t) nc + 1);
+
+   // fast path: RA tables of 512, 1024, 2048 slots
+   RapFastArgs f{};
+   f.nc = nc;
+   f.R_ia = R.ia; f.R_ja = R.ja; f.R_a = R.a;
+   f.A_ia = A.ia; f.A_ja = A.ja; f.A_a = A.a;
+   f.P_ia = P.ia; f.P_ja = P.ja; f.P_a = P.a;
+   int *fcnt = dalloc_n<int>((size_t) nc + 1);
+   int *tmp_j = dalloc_n<int>((size_t) std::max(nc, 1) * RF_ACL);
+   double *tmp_a = dalloc_n<double>((size_t) std::max(nc, 1) * RF_ACL);
+   int *lists[2] = {dalloc_n<int>((size_t) nc + 1), dalloc_n<int>((size_t) nc + 1)};
+   int *ndefer = device_flags() + 60;
+   int *h = static_cast<int *>(pinned_scratch(64));
+   f.tmp_j = tmp_j;
+   f.tmp_a = tmp_a;
+   f.cnt = fcnt;
+   f.ndefer = ndefer;
+   const int *list = nullptr;
+   int nwork = nc;
+   int cur = 0;
+   static const int rcs[3] = {512, 1024, 2048};
+   for (int stage = 0; stage < 3 && nwork > 0; stage++) {
+      f.rowlist = list;
+      f.nwork = nwork;
+      f.rc = rcs[stage];
+      f.rcl = rcs[stage] * 5 / 8;
+      f.defer_list = lists[cur];
+      dzero(ndefer, sizeof(int));
+      k_rap_fast<<<launch_blocks(nwork), WAVE, rap_fast_lds(f.rc, f.rcl), stream()>>>(f);
+      HX_LAUNCH_CHECK();
+      HX_HIP(hipMemcpyAsync(h, ndefer, sizeof(int), hipMemcpyDeviceToHost, stream()));
+      sync();
+      list = lists[cur];
+      nwork = h[0];
+      cur ^= 1;
+   }
+   // rows beyond the fast path: counts by the general kernel
+   int *cnt = dalloc_n<int>((size_t) nc + 1);
+   int nb = std::max(1, std::min((nc + 255) / 256, 8192));
+   k_clamp0<<<nb, 256, 0, stream()>>>(nc, fcnt, cnt);
+   HX_LAUNCH_CHECK();
+   if (nwork > 0) {
+      a.cnt = cnt;
+      rap_pass<MODE_SYM>(a, list, nwork);
+   }
    Ac.nnz = exclusive_scan(cnt, Ac.ia, nc);
    Ac.ja = dalloc_n<int>((size_t) std::max<long long>(Ac.nnz, 1));
    Ac.a = dalloc_n<double>((size_t) std::max<long long>(Ac.nnz, 1));
-   Ac.a32 = nullptr;
-   a.out_ia = Ac.ia;
-   a.out_j = Ac.ja;
-   a.out_a = Ac.a;
-   rap_pass<MODE_NUM>(a);
+   int nbc = (int) std::max<long long>(1, std::min<long long>(((long long) nc * 16 + 255) / 256, 16384));
+   k_rap_compact<<<nbc, 256, 0, stream()>>>(nc, fcnt, Ac.ia, tmp_j, tmp_a, Ac.ja, Ac.a);
+   HX_LAUNCH_CHECK();
+   if (nwork > 0) {
+      a.out_ia = Ac.ia;
+      a.out_j = Ac.ja;
+      a.out_a = Ac.a;
+      rap_pass<MODE_NUM>(a, list, nwork);
+   }
+   dfree(lists[0]);
+   dfree(lists[1]);
+   dfree(tmp_j);
+   dfree(tmp_a);
+   dfree(fcnt);
    dfree(cnt);
 }
 
