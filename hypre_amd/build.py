@@ -63,7 +63,27 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_examples(verbose)
     return LIB
+
+
+EXAMPLES = os.path.join(HERE, "examples")
+
+
+def build_examples(verbose: bool = True) -> None:
+    """Plain-C drivers that use only hypre's public API, linked against the library."""
+    for src in sorted(os.listdir(EXAMPLES)) if os.path.isdir(EXAMPLES) else []:
+        if not src.endswith(".c"):
+            continue
+        exe = os.path.join(EXAMPLES, src[:-2])
+        cmd = ["gcc", "-O2", "-std=c99", "-Wall", "-I", os.path.join(ROOT, "include"),
+               os.path.join(EXAMPLES, src), "-L", HERE, "-lhypre_mi355x",
+               "-Wl,-rpath,$ORIGIN/..", "-lm", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"example build failed: {src}\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print(f"built {exe}")
 
 
 def clean() -> None:

@@ -50,6 +50,16 @@ static int launch_blocks(long long work_items)
    return (int) b;
 }
 
+// one-wave blocks that all fit on the chip at once (256 CUs, 160 KB LDS and
+// 32 wave slots per CU), so that grid-strided rows advance together
+static int resident_blocks(long long work_items, size_t lds_per_block)
+{
+   long long per_cu = lds_per_block ? std::min<long long>(32, 163840 / (long long) lds_per_block) : 32;
+   long long b = std::min<long long>(std::max<long long>(work_items, 1), 256 * std::max<long long>(per_cu, 1));
+   if (b >= 64) { b = b / 8 * 8; }
+   return (int) b;
+}
+
 __device__ __forceinline__ int xcd_blk(int bid, int nb)
 {
    return (nb & 7) ? bid : (bid & 7) * (nb >> 3) + (bid >> 3);
@@ -579,10 +589,10 @@ __global__ __launch_bounds__(WAVE) void k_extpi(InterpArgs p)
    carve(base, p.cap, keys, vals, s_col, s_w);
    RowHash H{keys, vals, p.cap, ilog2(p.cap)};
    const int capc = p.cap / 2;
-   const int per = (p.nwork + nb - 1) / nb;
-   const int w0 = blk * per, w1 = min(p.nwork, w0 + per);
 
-   for (int wi = w0; wi < w1; wi++) {
+   // rows strided over the grid: co-resident waves work on one window of
+   // neighbouring rows, whose neighbour rows stay in the XCD's L2
+   for (int wi = blk; wi < p.nwork; wi += nb) {
       const int i = p.rowlist ? p.rowlist[wi] : wi;
       const int cfi = p.CF[i];
       if (cfi >= 0 || cfi == -3) {
@@ -875,24 +885,29 @@ static void extpi_pass(InterpArgs a, const int *init_list = nullptr, int init_n 
 }
 
 // ------------------------------------------------------------------ ext+i, fast path
-// Same arithmetic as k_extpi, restructured so that a row costs a handful of
-// dependent global loads instead of one per neighbour:
-//   * C-hat is built over the flattened list "strong neighbours, with each
-//     strong F neighbour expanded to its strong row", 64 candidates per step;
-//     the first occurrence of a key is the one whose position wins an atomicMin
-//     on the slot's stamp, and first occurrences get their C-hat positions by a
-//     ballot prefix, which is exactly the serial first-seen order;
-//   * one lane per off-diagonal entry of row i; a strong-F lane walks row i1
-//     itself (denominator in column order, then the distributed values);
-//   * every contribution to weight v lands in M[r][v], r = rank of the
-//     contributing entry of row i, and lane v folds column v of M from r = 0,
-//     i.e. in the entry order of row i.  A missing contribution is +0.0, which
-//     leaves a sum unchanged (the sum starts at +0.0 and never becomes -0.0);
+// Same arithmetic as k_extpi, restructured so that a row costs a few dependent
+// global loads instead of one chain per neighbour, with ~2.5 KB of LDS:
+//   * a row's work is cut into segments held one per lane (segment s <-> lane
+//     s); the flattened entries of all segments are read 64 per chunk and
+//     XF_U chunks per batch of independent loads; a lane finds its segment by
+//     a binary search over the lanes' offsets (__shfl);
+//   * C-hat is built over "strong neighbours, each strong F neighbour
+//     expanded to its strong row": the first occurrence of a key is the
+//     position that wins an atomicMin on the slot's value (used as a stamp
+//     until the key gets its C-hat position), and first occurrences get
+//     positions by ballot prefix, i.e. the serial first-seen order;
+//   * weights: segments are the off-diagonal entries of row i (a C-hat entry
+//     is a one-entry segment, a strong F neighbour i1 spans row i1).  Pass A
+//     folds each denominator left to right over the qualifying entries of row
+//     i1; pass B adds the distributed values into the weights one segment per
+//     round, so every weight receives its terms in the serial order (the
+//     columns within one segment are distinct);
 //   * truncation runs afterwards in k_trunc_rows, one thread per row.
-// Rows with more than 64 strong neighbours, more than XF_MC C-hat points or
-// more than `mr` contributing entries are handed to k_extpi.
+// Rows with more than 64 strong neighbours or more than XF_MC C-hat points
+// are handed to k_extpi.
 constexpr int XF_HC = 256;   // hash slots (<= 64 F markers + <= XF_MC + 64 C keys)
 constexpr int XF_MC = 32;    // C-hat points per row on the fast path
+constexpr int XF_U = 3;      // chunks of 64 entries per batch of loads
 
 struct ExtpiFastArgs
 {
@@ -901,7 +916,6 @@ struct ExtpiFastArgs
    const double *A_a;
    const int    *S_ia, *S_ja;
    const int    *CF, *f2c;
-   int           mr;          // rows of M
    int          *raw_j;       // row i at i * XF_MC
    double       *raw_a;
    int          *cnt;         // entries of the raw row, -1 = deferred
@@ -909,31 +923,41 @@ struct ExtpiFastArgs
    int          *ndefer;
 };
 
-static size_t extpi_fast_lds(int mr)
+// last segment s in [0, nseg) with off_s <= q; segment s's offset is held by
+// lane s (non-decreasing, off_0 = 0 <= q).  Uniform: every lane runs 6 steps.
+__device__ __forceinline__ int seg_find_lanes(int off_reg, int nseg, int q)
 {
-   return (size_t) mr * XF_MC * 8 + (size_t) XF_HC * 12 + (65 + 64 + 64) * 4 + XF_MC * 8;
+   int lo = 0, hi = nseg - 1;
+#pragma unroll
+   for (int it = 0; it < 6; it++) {
+      const int mid = (lo + hi + 1) >> 1;
+      const int om = __shfl(off_reg, mid, WAVE);
+      if (lo < hi) {
+         if (om <= q) { lo = mid; } else { hi = mid - 1; }
+      }
+   }
+   return lo;
+}
+
+__device__ __forceinline__ double shfl_d(double v, int src)
+{
+   return __shfl(v, src, WAVE);
 }
 
 __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
 {
-   extern __shared__ double dyn_lds[];
+   __shared__ int keys[XF_HC];
+   __shared__ int vals[XF_HC];
+   __shared__ double wgt[XF_MC];
+   __shared__ int chat_key[XF_MC];
+   __shared__ int chat_col[XF_MC];
    const int lane = threadIdx.x;
    const int nb = gridDim.x;
    const int blk = xcd_blk(blockIdx.x, nb);
-   double *M = dyn_lds;
-   int *keys = reinterpret_cast<int *>(M + (size_t) p.mr * XF_MC);
-   int *vals = keys + XF_HC;
-   int *stamp = vals + XF_HC;
-   int *seg_off = stamp + XF_HC;   // 65
-   int *seg_b = seg_off + 65;      // 64
-   int *seg_kind = seg_b + 64;     // 64
-   int *chat_key = seg_kind + 64;  // XF_MC
-   int *chat_col = chat_key + XF_MC;
    RowHash H{keys, vals, XF_HC, ilog2(XF_HC)};
-   const int per = (p.n + nb - 1) / nb;
-   const int r0 = blk * per, r1 = min(p.n, r0 + per);
-
-   for (int i = r0; i < r1; i++) {
+   // rows strided over the grid: co-resident waves work on one window of
+   // neighbouring rows, whose neighbour rows stay in the XCD's L2
+   for (int i = blk; i < p.n; i += nb) {
       const int cfi = p.CF[i];
       if (cfi >= 0 || cfi == -3) {
          if (lane == 0) {
@@ -951,7 +975,7 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
       const int a0 = p.A_ia[i], na = p.A_ia[i + 1] - a0 - 1;
       bool defer = ns > WAVE;
       if (!defer) {
-         for (int s = lane; s < XF_HC; s += WAVE) { keys[s] = -1; stamp[s] = 0x7fffffff; }
+         for (int s = lane; s < XF_HC; s += WAVE) { keys[s] = -1; vals[s] = 0x7fffffff; }
          // ---- strong neighbours: kind 1 = C, 2 = F (par_lr_interp.c:1505-1560)
          int i1 = -1, kind = 0, kb = 0, klen = 0;
          if (lane < ns) {
@@ -971,111 +995,178 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
          }
          const int incl = wave_incl_scan(klen, lane);
          const int L = readlane_i(incl, WAVE - 1);
-         seg_off[lane] = incl - klen;
-         seg_b[lane] = kb;
-         seg_kind[lane] = kind;
+         const int off = incl - klen;
          wave_sync();
-         // ---- C-hat in first-seen order
+         // ---- C-hat in first-seen order.  vals[] of a C key holds the
+         // smallest position seen (>= 64 after the first chunk) until the key
+         // receives its C-hat position (< XF_MC).
          int nch = 0;
          const int nseg = max(ns, 1);
-         for (int cb = 0; cb < L; cb += WAVE) {
-            const int q = cb + lane;
-            int key = -1;
-            if (q < L) {
-               const int s = seg_find(seg_off, nseg, q);
-               if (seg_kind[s] == 1) {
-                  key = seg_b[s];
-               } else {
-                  const int k1 = p.S_ja[seg_b[s] + (q - seg_off[s])];
-                  if (p.CF[k1] >= 0) { key = k1; }
+         for (int cb = 0; cb < L && !defer; cb += WAVE * XF_U) {
+            int key[XF_U], k1[XF_U];
+#pragma unroll
+            for (int u = 0; u < XF_U; u++) {
+               const int q = cb + u * WAVE + lane;
+               const int s = seg_find_lanes(off, nseg, q);
+               const int kind_s = __shfl(kind, s, WAVE);
+               const int b_s = __shfl(kb, s, WAVE);
+               const int off_s = __shfl(off, s, WAVE);
+               key[u] = -1;
+               k1[u] = -1;
+               if (q < L) {
+                  if (kind_s == 1) {
+                     key[u] = b_s;
+                  } else if (kind_s == 2) {
+                     k1[u] = p.S_ja[b_s + (q - off_s)];
+                  }
                }
             }
-            int sl = -1;
-            if (key >= 0) {
-               bool isn;
-               sl = H.insert(key, isn);
-               if (sl >= 0) { atomicMin(&stamp[sl], q); }
+            int cf1[XF_U];
+#pragma unroll
+            for (int u = 0; u < XF_U; u++) { cf1[u] = (k1[u] >= 0) ? p.CF[k1[u]] : -1; }
+#pragma unroll
+            for (int u = 0; u < XF_U; u++) {
+               if (cf1[u] >= 0) { key[u] = k1[u]; }
             }
-            wave_sync();
-            const bool first = (sl >= 0) && stamp[sl] == q;
-            const unsigned long long fm = __ballot(first);
-            const int nnew = __popcll(fm);
-            if (nch + nnew > XF_MC) { defer = true; break; }
-            if (first) {
-               const int pos = nch + __popcll(fm & lanemask_lt(lane));
-               vals[sl] = pos;
-               chat_key[pos] = key;
+#pragma unroll
+            for (int u = 0; u < XF_U; u++) {
+               if (cb + u * WAVE >= L || defer) { break; }
+               const int q = cb + u * WAVE + lane;
+               int sl = -1;
+               if (key[u] >= 0) {
+                  bool isn;
+                  sl = H.insert(key[u], isn);
+                  if (sl >= 0) { atomicMin(&vals[sl], q); }
+               }
+               wave_sync();
+               const bool first = (sl >= 0) && vals[sl] == q;
+               const unsigned long long fm = __ballot(first);
+               const int nnew = __popcll(fm);
+               if (nch + nnew > XF_MC) {
+                  defer = true;
+               } else if (first) {
+                  const int pos = nch + __popcll(fm & lanemask_lt(lane));
+                  vals[sl] = pos;
+                  chat_key[pos] = key[u];
+               }
+               nch += nnew;
+               wave_sync();
             }
-            nch += nnew;
-            wave_sync();
          }
          if (!defer) {
-            if (lane < nch) { chat_col[lane] = p.f2c[chat_key[lane]]; }
-            for (int t = lane; t < p.mr * XF_MC; t += WAVE) { M[t] = 0.0; }
+            if (lane < nch) {
+               chat_col[lane] = p.f2c[chat_key[lane]];
+               wgt[lane] = 0.0;
+            }
             wave_sync();
-            // ---- weights (par_lr_interp.c:1640-1805)
+            // ---- weights (par_lr_interp.c:1640-1805), 64 entries of row i per block
             double diag = p.A_a[a0];
-            int mrow = 0;
             for (int jb = 0; jb < na; jb += WAVE) {
                const int j = jb + lane;
-               int c1 = 0, v = -1;
+               // segment of entry j: kind 1 C-hat (one entry, b = weight index),
+               // 2 strong F (row c1 without its diagonal), 3 weak, 0 none
+               int c1 = 0, v = -1, sk = 0, sb = 0, slen = 0;
+               bool sneg = false;
                double aij = 0.0;
                if (j < na) {
                   c1 = p.A_ja[a0 + 1 + j];
                   aij = p.A_a[a0 + 1 + j];
                   const int sl = H.find(c1);
                   v = (sl >= 0) ? vals[sl] : -1;
-               }
-               const bool contrib = (j < na) && (v >= 0 || v == -2);
-               const unsigned long long cm = __ballot(contrib);
-               if (mrow + __popcll(cm) > p.mr) { defer = true; break; }
-               const int myrow = mrow + __popcll(cm & lanemask_lt(lane));
-               mrow += __popcll(cm);
-               bool hasd = false;
-               double dval = 0.0;
-               if (j < na) {
                   if (v >= 0) {
-                     M[myrow * XF_MC + v] = aij;
+                     sk = 1; sb = v; slen = 1;
                   } else if (v == -2) {
-                     const int b1 = p.A_ia[c1], e1 = p.A_ia[c1 + 1];
-                     const double sgn = (p.A_a[b1] < 0) ? -1.0 : 1.0;
-                     double sum = 0.0;
-                     for (int k = b1 + 1; k < e1; k++) {
-                        const int i2 = p.A_ja[k];
-                        const double a2 = p.A_a[k];
-                        if ((sgn * a2) < 0) {
-                           bool q2 = (i2 == i);
-                           if (!q2) {
-                              const int s2 = H.find(i2);
-                              q2 = (s2 >= 0) && vals[s2] >= 0;
-                           }
-                           if (q2) { sum += a2; }
-                        }
-                     }
-                     if (sum != 0) {
-                        const double distribute = aij / sum;
-                        for (int k = b1 + 1; k < e1; k++) {
-                           const int i2 = p.A_ja[k];
-                           const double a2 = p.A_a[k];
-                           if ((sgn * a2) < 0) {
-                              if (i2 == i) {
-                                 hasd = true;
-                                 dval = distribute * a2;
-                              } else {
-                                 const int s2 = H.find(i2);
-                                 const int v2 = (s2 >= 0) ? vals[s2] : -1;
-                                 if (v2 >= 0) { M[myrow * XF_MC + v2] = distribute * a2; }
-                              }
-                           }
-                        }
-                     } else {
-                        hasd = true;
-                        dval = aij;
-                     }
+                     const int b1 = p.A_ia[c1];
+                     sk = 2; sb = b1 + 1; slen = p.A_ia[c1 + 1] - b1 - 1;
+                     sneg = p.A_a[b1] < 0;
                   } else if (p.CF[c1] != -3) {
-                     hasd = true;                     // weak connection
+                     sk = 3;
+                  }
+               }
+               const int sincl = wave_incl_scan(slen, lane);
+               const int SL = readlane_i(sincl, WAVE - 1);
+               const int soff = sincl - slen;
+               const int nseg2 = min(WAVE, na - jb);
+               // pass A: denominators sum over row c1 of the entries of opposite
+               // sign whose column is in C-hat or is i, in column order
+               double ssum = 0.0, sai = 0.0;
+               bool shas_i = false;
+               double acc = 0.0;
+               int acc_s = -1;
+               for (int cb = 0; cb < SL; cb += WAVE * XF_U) {
+                  int i2[XF_U], sq[XF_U];
+                  double a2[XF_U];
+                  bool sf[XF_U], sn[XF_U];
+#pragma unroll
+                  for (int u = 0; u < XF_U; u++) {
+                     const int q = cb + u * WAVE + lane;
+                     const int s = seg_find_lanes(soff, nseg2, q);
+                     const int k_s = __shfl(sk, s, WAVE);
+                     const int b_s = __shfl(sb, s, WAVE);
+                     const int o_s = __shfl(soff, s, WAVE);
+                     sn[u] = __shfl((int) sneg, s, WAVE) != 0;
+                     sq[u] = s;
+                     sf[u] = (q < SL) && k_s == 2;
+                     i2[u] = -1;
+                     a2[u] = 0.0;
+                     if (sf[u]) {
+                        i2[u] = p.A_ja[b_s + (q - o_s)];
+                        a2[u] = p.A_a[b_s + (q - o_s)];
+                     }
+                  }
+#pragma unroll
+                  for (int u = 0; u < XF_U; u++) {
+                     bool qual = false, isi = false;
+                     if (sf[u]) {
+                        const bool opp = sn[u] ? (a2[u] > 0) : (a2[u] < 0);
+                        if (opp) {
+                           if (i2[u] == i) {
+                              qual = true;
+                              isi = true;
+                           } else {
+                              const int s2 = H.find(i2[u]);
+                              qual = (s2 >= 0) && vals[s2] >= 0;
+                           }
+                        }
+                     }
+                     unsigned long long m = __ballot(qual);
+                     while (m) {
+                        const int t = __ffsll((long long) m) - 1;
+                        const int st = readlane_i(sq[u], t);
+                        if (st != acc_s) {
+                           if (acc_s >= 0 && lane == acc_s) { ssum = acc; }
+                           acc_s = st;
+                           acc = 0.0;
+                        }
+                        acc += readlane_d(a2[u], t);
+                        m &= m - 1;
+                     }
+                     unsigned long long mi = __ballot(isi);
+                     while (mi) {
+                        const int t = __ffsll((long long) mi) - 1;
+                        const int st = readlane_i(sq[u], t);
+                        const double av = readlane_d(a2[u], t);
+                        if (lane == st) { sai = av; shas_i = true; }
+                        mi &= mi - 1;
+                     }
+                  }
+               }
+               if (acc_s >= 0 && lane == acc_s) { ssum = acc; }
+               // per entry: distribution factor and diagonal term, in entry order
+               bool hasd = false, sdist = false;
+               double dval = 0.0, dist = 0.0;
+               if (sk == 2) {
+                  if (ssum != 0) {
+                     dist = aij / ssum;
+                     sdist = true;
+                     if (shas_i) { hasd = true; dval = dist * sai; }
+                  } else {
+                     hasd = true;
                      dval = aij;
                   }
+               } else if (sk == 3) {
+                  hasd = true;                        // weak connection
+                  dval = aij;
                }
                unsigned long long dm = __ballot(hasd);
                while (dm) {
@@ -1083,18 +1174,70 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                   diag += readlane_d(dval, t);
                   dm &= dm - 1;
                }
-            }
-            if (!defer) {
-               wave_sync();
-               if (lane < nch) {
-                  double w = 0.0;
-                  for (int r = 0; r < mrow; r++) { w += M[r * XF_MC + lane]; }
-                  if (diag != 0.0) { w /= -diag; }
-                  p.raw_j[(size_t) i * XF_MC + lane] = chat_col[lane];
-                  p.raw_a[(size_t) i * XF_MC + lane] = w;
+               // pass B: a_ij into its weight, distributed values into theirs,
+               // one segment per round
+               const double segx = (sk == 1) ? aij : dist;
+               for (int cb = 0; cb < SL; cb += WAVE * XF_U) {
+                  int tgt[XF_U], sq[XF_U], i2[XF_U];
+                  double x[XF_U], a2[XF_U];
+                  bool ld[XF_U], sn[XF_U];
+#pragma unroll
+                  for (int u = 0; u < XF_U; u++) {
+                     const int q = cb + u * WAVE + lane;
+                     const int s = seg_find_lanes(soff, nseg2, q);
+                     const int k_s = __shfl(sk, s, WAVE);
+                     const int b_s = __shfl(sb, s, WAVE);
+                     const int o_s = __shfl(soff, s, WAVE);
+                     const int d_s = __shfl((int) sdist, s, WAVE);
+                     const double x_s = shfl_d(segx, s);
+                     sn[u] = __shfl((int) sneg, s, WAVE) != 0;
+                     sq[u] = s;
+                     tgt[u] = -1;
+                     x[u] = x_s;
+                     ld[u] = false;
+                     i2[u] = -1;
+                     a2[u] = 0.0;
+                     if (q < SL) {
+                        if (k_s == 1) {
+                           tgt[u] = b_s;
+                        } else if (k_s == 2 && d_s) {
+                           ld[u] = true;
+                           i2[u] = p.A_ja[b_s + (q - o_s)];
+                           a2[u] = p.A_a[b_s + (q - o_s)];
+                        }
+                     }
+                  }
+#pragma unroll
+                  for (int u = 0; u < XF_U; u++) {
+                     if (cb + u * WAVE >= SL) { break; }
+                     if (ld[u]) {
+                        const bool opp = sn[u] ? (a2[u] > 0) : (a2[u] < 0);
+                        if (opp && i2[u] != i) {
+                           const int s2 = H.find(i2[u]);
+                           const int v2 = (s2 >= 0) ? vals[s2] : -1;
+                           if (v2 >= 0) {
+                              tgt[u] = v2;
+                              x[u] = x[u] * a2[u];
+                           }
+                        }
+                     }
+                     const int s_lo = readlane_i(sq[u], 0);
+                     const int s_hi = readlane_i(sq[u], min(WAVE - 1, SL - 1 - (cb + u * WAVE)));
+                     for (int sg = s_lo; sg <= s_hi; sg++) {
+                        if (tgt[u] >= 0 && sq[u] == sg) { wgt[tgt[u]] += x[u]; }
+                        wave_sync();
+                     }
+                  }
                }
-               if (lane == 0) { p.cnt[i] = nch; }
             }
+            wave_sync();
+            if (lane < nch) {
+               double w = wgt[lane];
+               if (diag != 0.0) { w /= -diag; }
+               p.raw_j[(size_t) i * XF_MC + lane] = chat_col[lane];
+               p.raw_a[(size_t) i * XF_MC + lane] = w;
+            }
+            if (lane == 0) { p.cnt[i] = nch; }
          }
       }
       if (defer && lane == 0) {
@@ -1273,13 +1416,11 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
    if (max_elmts > 0 && max_elmts <= XF_MC) {
       // fast path into raw rows, truncation per thread, leftovers by k_extpi
       stride = max_elmts;
-      const int mr = std::max(1, std::min(64, max_row_len(A) - 1));
       ExtpiFastArgs f{};
       f.n = n;
       f.A_ia = A.ia; f.A_ja = A.ja; f.A_a = A.a;
       f.S_ia = S.ia; f.S_ja = S.ja;
       f.CF = CF; f.f2c = f2c;
-      f.mr = mr;
       int *raw_cnt = dalloc_n<int>((size_t) n + 1);
       f.raw_j = dalloc_n<int>((size_t) std::max(n, 1) * XF_MC);
       f.raw_a = dalloc_n<double>((size_t) std::max(n, 1) * XF_MC);
@@ -1288,7 +1429,7 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
       f.defer_list = dlist;
       f.ndefer = device_flags() + 60;
       dzero(f.ndefer, sizeof(int));
-      k_extpi_fast<<<launch_blocks(n), WAVE, extpi_fast_lds(mr), stream()>>>(f);
+      k_extpi_fast<<<launch_blocks(n), WAVE, 0, stream()>>>(f);
       HX_LAUNCH_CHECK();
       tmp_j = dalloc_n<int>((size_t) std::max(n, 1) * stride);
       tmp_a = dalloc_n<double>((size_t) std::max(n, 1) * stride);
@@ -1478,10 +1619,10 @@ __global__ __launch_bounds__(WAVE) void k_rap(RapArgs p)
    RowHash HR{ra_keys, ra_vals, p.cap, bits};
    RowHash HC{ac_keys, ac_vals, p.cap, bits};
    const int capc = p.cap / 2;
-   const int per = (p.nwork + nb - 1) / nb;
-   const int w0 = blk * per, w1 = min(p.nwork, w0 + per);
 
-   for (int wi = w0; wi < w1; wi++) {
+   // rows strided over the grid: co-resident waves work on one window of
+   // neighbouring rows, whose neighbour rows stay in the XCD's L2
+   for (int wi = blk; wi < p.nwork; wi += nb) {
       const int ic = p.rowlist ? p.rowlist[wi] : wi;
       HR.clear(lane);
       HC.clear(lane);
@@ -1660,10 +1801,10 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
    int *seg_b = seg_off + 65;                   // 64
    RowHash HR{ra_keys, nullptr, p.rc, ilog2(p.rc)};
    RowHash HC{ac_keys, nullptr, RF_AC, ilog2(RF_AC)};
-   const int per = (p.nwork + nb - 1) / nb;
-   const int w0 = blk * per, w1 = min(p.nwork, w0 + per);
 
-   for (int wi = w0; wi < w1; wi++) {
+   // rows strided over the grid: co-resident waves work on one window of
+   // neighbouring rows, whose neighbour rows stay in the XCD's L2
+   for (int wi = blk; wi < p.nwork; wi += nb) {
       const int ic = p.rowlist ? p.rowlist[wi] : wi;
       for (int s = lane; s < p.rc; s += WAVE) { ra_keys[s] = -1; }
       for (int s = lane; s < RF_AC; s += WAVE) {
@@ -1881,7 +2022,8 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
       f.rcl = rcs[stage] * 5 / 8;
       f.defer_list = lists[cur];
       dzero(ndefer, sizeof(int));
-      k_rap_fast<<<launch_blocks(nwork), WAVE, rap_fast_lds(f.rc, f.rcl), stream()>>>(f);
+      const size_t lds = rap_fast_lds(f.rc, f.rcl);
+      k_rap_fast<<<resident_blocks(nwork, lds), WAVE, lds, stream()>>>(f);
       HX_LAUNCH_CHECK();
       HX_HIP(hipMemcpyAsync(h, ndefer, sizeof(int), hipMemcpyDeviceToHost, stream()));
       sync();

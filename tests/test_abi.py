@@ -41,3 +41,20 @@ def test_python_signatures_match_exports():
     L = lib()
     for name in SIGNATURES:
         assert hasattr(L, name), name
+
+
+EXAMPLE = os.path.join(ROOT, "hypre_amd", "examples", "ij_amg_pcg")
+
+
+def test_c_driver_uses_only_the_declared_api():
+    """The plain-C driver (hypre's public API only) links against the library,
+    and every library symbol it imports is a hypre entry point in include/*.h."""
+    import subprocess
+    assert os.path.exists(EXAMPLE), "build with python -m hypre_amd.build"
+    out = subprocess.run(["nm", "-D", "--undefined-only", EXAMPLE], capture_output=True,
+                         text=True, check=True).stdout
+    imported = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    hypre = {s for s in imported if s.startswith("HYPRE_") or s.startswith("Generate")}
+    assert {"HYPRE_IJMatrixSetValues", "HYPRE_ParCSRPCGSolve", "HYPRE_BoomerAMGSetup"} <= hypre
+    assert hypre <= declared_functions()
+    assert not any(s.startswith("HYPRE_MI355X") for s in hypre)

@@ -329,3 +329,23 @@ def test_error_codes(hx):
     pcg.solve(A, z, x)
     assert pcg.num_iterations() == 0
     assert np.all(x.get() == 0.0)
+
+
+# ------------------------------------------------------------------ drop-in C driver
+def test_c_driver_reproduces_reference_ij_run(golden):
+    """hypre_amd/examples/ij_amg_pcg.c, a C program written against hypre's
+    public API only, run on 30^3 7-point with ij's settings: the reference's
+    ij printed 12 iterations and final relative residual 7.124590e-09."""
+    import os
+    import subprocess
+    say("C driver 30^3 7pt")
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "hypre_amd", "examples", "ij_amg_pcg")
+    r = subprocess.run([exe, "30", "7", "1.0"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = dict(ln.split(" = ") for ln in r.stdout.splitlines() if " = " in ln)
+    g = golden["ij_30_7pt"]
+    assert int(out["Iterations"]) == g["iterations"]
+    assert float(out["Final Relative Residual Norm"]) == g["final_rel_res"]
+    assert float(out["True Relative Residual Norm"]) < 1e-8
+    assert int(out["error flag"]) == 0
