@@ -23,7 +23,36 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_N = 1 << 27          # doubles copied by the counter-calibration kernel (1 GiB)
 METRIC = "AMG-PCG setup+solve sec to 1e-8 FP64 rel. residual; SpMV HBM GB/s"
+
+
+PROBLEMS = {
+    "lap27": {"n": 200, "rhs_seed": 0, "desc": "3D 27-point Laplacian",
+              "make": lambda hx, N: hx.ParCSRMatrix.laplacian27(N, N, N)},
+    "lap7": {"n": 256, "rhs_seed": 0, "desc": "3D 7-point Laplacian",
+             "make": lambda hx, N: hx.ParCSRMatrix.laplacian7(N, N, N)},
+    "vardiff7": {"n": 256, "rhs_seed": 2, "desc": "3D 7-point diffusion, kappa=10^(-3+6U) on 8^3-node cells (seed 1)",
+                 "make": lambda hx, N: hx.ParCSRMatrix.var_diffusion(N, 7, 8, -3.0, 3.0, 1)},
+    "jump27": {"n": 320, "rhs_seed": 2, "desc": "3D 27-point diffusion, kappa=10^(-4+8U) on 16^3-node cells (seed 1)",
+               "make": lambda hx, N: hx.ParCSRMatrix.var_diffusion(N, 27, 16, -4.0, 4.0, 1)},
+}
+
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "spmv_A0_200_27pt_pmc.md")
+
+
+def measured_traffic(n_grid: int, problem: str = "lap27"):
+    """HBM bytes per launch of the fine-level SpMV from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same
+    counter on a copy kernel of known size), or None when there is none for
+    this grid."""
+    if problem != "lap27" or n_grid != 200 or not os.path.exists(PMC_SUMMARY):
+        return None
+    with open(PMC_SUMMARY) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(sample_n: int = 100):
@@ -53,9 +82,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=200, help="grid edge (27-point stencil)")
+    ap.add_argument("--n", type=int, default=None, help="grid edge (default per problem)")
+    ap.add_argument("--problem", default="lap27", choices=sorted(PROBLEMS),
+                    help="lap27: 27-point Laplacian (default, 200^3); vardiff7: 7-point "
+                         "variable-coefficient diffusion, kappa on 8^3 cells, 10^[-3,3] (256^3); "
+                         "jump27: 27-point jumping coefficients, 16^3 cells, 10^[-4,4] (320^3); "
+                         "lap7: 7-point Laplacian (256^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp32-cycle", action="store_true", help="FP32 V-cycle inside FP64 PCG")
+    ap.add_argument("--probe", action="store_true",
+                    help="short run for hardware-counter passes: one setup, 3 PCG iterations, "
+                         "then 3 copies of a 1 GiB vector (the bytes reference of the counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not "
@@ -66,16 +103,31 @@ def main():
     from hypre_amd import api as hx
 
     hx.initialize()
-    N = args.n
-    A = hx.ParCSRMatrix.laplacian27(N, N, N)
+    prob = PROBLEMS[args.problem]
+    N = args.n or prob["n"]
+    A = prob["make"](hx, N)
     n, nnz = A.dims()
     b = hx.ParVector(n)
-    b.set_random(0)                       # 2*hypre_Rand()-1, hypre_SeedRand(0) -> seed 1
+    b.set_random(prob["rhs_seed"])        # 2*hypre_Rand()-1 from hypre_SeedRand(seed)
     x = hx.ParVector(n)
     amg = hx.BoomerAMG(coarsen_type=8, interp_type=6, strong_threshold=0.25, max_row_sum=0.9,
                        P_max_elmts=4, relax_type=18, num_sweeps=1, max_coarse_size=9,
                        fp32_cycle=1 if args.fp32_cycle else 0)
-    pcg = hx.PCG(amg, tol=1e-8, max_iter=1000, two_norm=1)
+    pcg = hx.PCG(amg, tol=1e-8, max_iter=3 if args.probe else 1000, two_norm=1)
+
+    if args.probe:
+        x.fill(0.0)
+        pcg.setup(A, b, x)
+        pcg.solve(A, b, x)
+        hx.clear_errors()                 # 3 iterations leave hypre's sticky CONV flag set
+        v1 = hx.ParVector(COPY_N)
+        v2 = hx.ParVector(COPY_N)
+        v1.fill(1.0)
+        for _ in range(3):
+            v1.copy_to(v2)
+        print(json.dumps({"probe": "done", "rows": n, "nnz": nnz, "copy_bytes_read": 8 * COPY_N,
+                          "copy_bytes_written": 8 * COPY_N}), flush=True)
+        return
 
     def step():
         x.fill(0.0)
@@ -126,9 +178,10 @@ def main():
         "dtype": "f32-cycle/f64" if args.fp32_cycle else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"3D 27-point Laplacian {N}^3 ({n} rows, {nnz} nnz), Dirichlet; PMIS, "
+            "workload": f"{prob['desc']} {N}^3 ({n} rows, {nnz} nnz), Dirichlet; PMIS, "
                         "ext+i (Pmx 4), theta 0.25, max_row_sum 0.9, l1-Jacobi V(1,1), GE coarsest, "
-                        "two-norm PCG to 1e-8, x0 = 0, RHS uniform[-1,1] seed 0",
+                        f"two-norm PCG to 1e-8, x0 = 0, RHS uniform[-1,1] seed {prob['rhs_seed']}",
+            "problem": args.problem,
             "grid": N,
             "rows": n,
             "nnz": nnz,
@@ -146,18 +199,18 @@ def main():
         "spmv_gbs": round(achieved, 1) if achieved else None,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_spmv<32,double,double,DOT> (PCG s = A p on the fine level)",
+            "kernel": "k_spmv_w<RW,double,double,DOT> (PCG s = A p on the fine level)",
             "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None,
+            "traffic": measured_traffic(N, args.problem),
             "algorithmic_bytes_per_launch": bytes_per,
             "avg_launch_ms": round(avg_ms, 5),
             "launches": launches,
         },
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.problem == "lap27" and N == 200:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
 

@@ -36,8 +36,8 @@ def _dp(a):
 def check(ret: int, where: str, allow: int = 0):
     """Raise on a non-zero error flag (bits in `allow` are cleared and ignored)."""
     if ret:
+        lib().HYPRE_ClearAllErrors()      # hypre's flag is sticky: clear it either way
         if ret & ~allow:
-            lib().HYPRE_ClearAllErrors()
             raise HypreError(ret, where)
     return ret
 
@@ -183,6 +183,10 @@ class ParVector:
         d = C.c_double()
         call("HYPRE_ParVectorInnerProd", self.h, other.h, C.byref(d))
         return d.value
+
+    def copy_to(self, other: "ParVector"):
+        """other <- self (HYPRE_ParVectorCopy)."""
+        call("HYPRE_ParVectorCopy", self.h, other.h)
 
     def destroy(self):
         if self.h:
@@ -375,6 +379,13 @@ class BoomerAMG:
         v = C.c_double()
         call("HYPRE_BoomerAMGGetFinalRelativeResidualNorm", self.h, C.byref(v))
         return v.value
+
+    def level_dims(self, lev: int, which: str):
+        """(rows, cols, nnz) of a level matrix without copying it."""
+        n, nc, nnz = C.c_int(), C.c_int(), C.c_int()
+        call("HYPRE_MI355X_BoomerAMGGetLevelMatrix", self.h, lev, C.c_char(which.encode()),
+             C.byref(n), C.byref(nc), C.byref(nnz), None, None, None)
+        return n.value, nc.value, nnz.value
 
     def level_matrix(self, lev: int, which: str) -> HostCsr:
         n, nc, nnz = C.c_int(), C.c_int(), C.c_int()

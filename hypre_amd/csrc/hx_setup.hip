@@ -2023,7 +2023,7 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
       f.defer_list = lists[cur];
       dzero(ndefer, sizeof(int));
       const size_t lds = rap_fast_lds(f.rc, f.rcl);
-      k_rap_fast<<<resident_blocks(nwork, lds), WAVE, lds, stream()>>>(f);
+      k_rap_fast<<<launch_blocks(nwork), WAVE, lds, stream()>>>(f);
       HX_LAUNCH_CHECK();
       HX_HIP(hipMemcpyAsync(h, ndefer, sizeof(int), hipMemcpyDeviceToHost, stream()));
       sync();

@@ -2,12 +2,13 @@
 // relaxation, deterministic dot products, fused PCG vector updates and the
 // coarsest-grid Gaussian elimination.
 //
-// SpMV layout: K lanes of a wave64 share one row (K = 4..64 picked from the
-// average row length), loads of ja/a are contiguous per row, partial sums are
-// combined with lane shuffles.  Each block owns a contiguous slice of rows and
-// block ids are remapped so that every XCD (blocks are dealt round-robin to the
-// 8 XCDs) streams one contiguous eighth of the matrix and its x-gathers stay
-// in that XCD's L2.
+// SpMV and l1-Jacobi layout: a wave owns a group of consecutive rows and
+// streams their nonzeros in coalesced windows (products staged in LDS, each
+// row summed by one lane in storage order).  Each block owns a contiguous
+// slice of row groups and block ids are remapped so that every XCD (blocks are
+// dealt round-robin to the 8 XCDs) streams one contiguous eighth of the
+// matrix and its x-gathers stay in that XCD's L2.  Weighted Jacobi (relax
+// type 0, off the default path) keeps K lanes per row with a shuffle sum.
 #include "hx_internal.h"
 
 #include <algorithm>
@@ -53,26 +54,81 @@ __device__ __forceinline__ X row_dot(int s, int e, int lane, const int *__restri
    return t;
 }
 
-// ---------------------------------------------------------------- SpMV
-template <int K, typename V, typename X, bool DOT>
-__global__ __launch_bounds__(BS) void k_spmv(int n, const int *__restrict__ ia,
-                                             const int *__restrict__ ja, const V *__restrict__ a,
-                                             const X *__restrict__ x, X alpha, X beta,
-                                             const X *b, X *y, double *__restrict__ part)
+// ---------------------------------------------------------------- windowed row sums
+// A wave owns RW consecutive rows.  Their nonzeros form one contiguous range,
+// read in windows of SW entries with fully coalesced loads (SW/64 independent
+// a/ja loads and x gathers per lane in flight); the products go to a per-wave
+// LDS buffer and lane l (< RW) adds up row l's products in storage order,
+// i.e. t = ((0 + a_0 x_0) + a_1 x_1) + ..., the order of hypre's host CSR
+// matvec (src/seq_mv/csr_matvec.c).
+constexpr int SW = 512;
+constexpr int WPB = BS / 64;   // waves per block
+
+static int pick_rw(long long nnz, int n)
 {
-   constexpr int G = BS / K;
+   double avg = n > 0 ? (double) nnz / n : 1.0;
+   int rw = 64;
+   while (rw > 1 && rw * avg > SW) { rw >>= 1; }
+   return rw;
+}
+
+__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_wave_barrier(); }
+
+template <int RW, typename V, typename X>
+__device__ __forceinline__ X rows_dot_window(int r0, int nr, int lane, const int *__restrict__ ia,
+                                             const int *__restrict__ ja, const V *__restrict__ a,
+                                             const X *__restrict__ x, X *sbuf)
+{
+   int rs = 0, re = 0;
+   if (lane < nr) {
+      rs = ia[r0 + lane];
+      re = ia[r0 + lane + 1];
+   }
+   const int base = __shfl(rs, 0, 64);
+   const int end = __shfl(re, nr - 1, 64);
+   X t = 0;
+   for (int wb = base; wb < end; wb += SW) {
+      const int we = min(end, wb + SW);
+      X p[SW / 64];
+#pragma unroll
+      for (int u = 0; u < SW / 64; u++) {
+         const int j = wb + u * 64 + lane;
+         p[u] = (j < we) ? (X) a[j] * x[ja[j]] : (X) 0;
+      }
+#pragma unroll
+      for (int u = 0; u < SW / 64; u++) { sbuf[u * 64 + lane] = p[u]; }
+      wave_fence();
+      if (lane < nr) {
+         const int s = max(rs, wb), e = min(re, we);
+         for (int k = s; k < e; k++) { t += sbuf[k - wb]; }
+      }
+      wave_fence();
+   }
+   return t;
+}
+
+template <int RW, typename V, typename X, bool DOT>
+__global__ __launch_bounds__(BS) void k_spmv_w(int n, const int *__restrict__ ia,
+                                               const int *__restrict__ ja, const V *__restrict__ a,
+                                               const X *__restrict__ x, X alpha, X beta,
+                                               const X *b, X *y, double *__restrict__ part)
+{
+   __shared__ X sbuf_all[WPB * SW];
    const int nb = gridDim.x;
    const int blk = xcd_block(blockIdx.x, nb);
-   const int rows_per_block = (n + nb - 1) / nb;
-   const int r0 = blk * rows_per_block;
-   const int r1 = min(n, r0 + rows_per_block);
-   const int lane = threadIdx.x & (K - 1);
-   const int g = threadIdx.x / K;
+   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   X *sbuf = sbuf_all + wv * SW;
+   const int ng = (n + RW - 1) / RW;
+   const int gpb = (ng + nb - 1) / nb;
+   const int g0 = blk * gpb, g1 = min(ng, g0 + gpb);
    double acc = 0.0;
-   for (int row = r0 + g; row < r1; row += G) {
-      X t = row_dot<K, V, X>(ia[row], ia[row + 1], lane, ja, a, x);
-      if (lane == 0) {
-         X v = (beta == (X) 0) ? alpha * t : alpha * t + beta * b[row];
+   for (int g = g0 + wv; g < g1; g += WPB) {
+      const int r0 = g * RW;
+      const int nr = min(RW, n - r0);
+      const X t = rows_dot_window<RW, V, X>(r0, nr, lane, ia, ja, a, x, sbuf);
+      if (lane < nr) {
+         const int row = r0 + lane;
+         const X v = (beta == (X) 0) ? alpha * t : alpha * t + beta * b[row];
          y[row] = v;
          if (DOT) { acc += (double) v * (double) x[row]; }
       }
@@ -111,25 +167,47 @@ static double *parts()
    return g_part;
 }
 
-template <typename V, typename X, bool DOT>
-static void launch_spmv(const Csr &A, const V *vals, X alpha, const X *x, X beta, const X *b,
-                        X *y, double *part, const char *cls, double vbytes)
+static int grid_for_groups(int n, int rw)
 {
-   if (A.nrows <= 0) { return; }
-   int K = pick_k(A.nnz, A.nrows);
-   int nb = grid_for_rows(A.nrows, K);
+   long long ng = ((long long) n + rw - 1) / rw;
+   long long blocks = (ng + WPB - 1) / WPB;
+   blocks = std::min<long long>(blocks, 8192);
+   blocks = std::max<long long>(blocks, 1);
+   if (blocks >= 64) { blocks = (blocks + 7) / 8 * 8; }
+   return (int) blocks;
+}
+
+template <class F>
+static void dispatch_rw(int rw, F f)
+{
+   switch (rw) {
+      case 1: f(std::integral_constant<int, 1>{}); break;
+      case 2: f(std::integral_constant<int, 2>{}); break;
+      case 4: f(std::integral_constant<int, 4>{}); break;
+      case 8: f(std::integral_constant<int, 8>{}); break;
+      case 16: f(std::integral_constant<int, 16>{}); break;
+      case 32: f(std::integral_constant<int, 32>{}); break;
+      default: f(std::integral_constant<int, 64>{}); break;
+   }
+}
+
+template <typename V, typename X, bool DOT>
+static int launch_spmv(const Csr &A, const V *vals, X alpha, const X *x, X beta, const X *b,
+                       X *y, double *part, const char *cls, double vbytes)
+{
+   if (A.nrows <= 0) { return 0; }
+   const int rw = pick_rw(A.nnz, A.nrows);
+   const int nb = grid_for_groups(A.nrows, rw);
    double bytes = (double) A.nnz * (4.0 + vbytes) + 4.0 * (A.nrows + 1) +
                   (double) sizeof(X) * (A.ncols + A.nrows) + (beta != (X) 0 ? sizeof(X) * (double) A.nrows : 0.0);
    TimedLaunch tl(cls, bytes);
    hipStream_t s = stream();
-   switch (K) {
-      case 4: k_spmv<4, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, x, alpha, beta, b, y, part); break;
-      case 8: k_spmv<8, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, x, alpha, beta, b, y, part); break;
-      case 16: k_spmv<16, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, x, alpha, beta, b, y, part); break;
-      case 32: k_spmv<32, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, x, alpha, beta, b, y, part); break;
-      default: k_spmv<64, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, x, alpha, beta, b, y, part); break;
-   }
+   dispatch_rw(rw, [&](auto c) {
+      k_spmv_w<decltype(c)::value, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, x, alpha,
+                                                               beta, b, y, part);
+   });
    HX_LAUNCH_CHECK();
+   return nb;
 }
 
 void spmv(const Csr &A, double alpha, const double *x, double beta, const double *b, double *y,
@@ -147,42 +225,14 @@ void spmv32(const Csr &A, float alpha, const float *x, float beta, const float *
 
 void spmv_dot(const Csr &A, const double *p, double *s, double *dot_out_dev, const char *cls)
 {
-   int K = pick_k(A.nnz, A.nrows);
-   int nb = grid_for_rows(A.nrows, K);
-   launch_spmv<double, double, true>(A, A.a, 1.0, p, 0.0, nullptr, s, parts(), cls, 8.0);
-   k_reduce_parts<<<1, BS, 0, stream()>>>(parts(), nb, dot_out_dev);
+   const int nb = launch_spmv<double, double, true>(A, A.a, 1.0, p, 0.0, nullptr, s, parts(), cls, 8.0);
+   k_reduce_parts<<<1, BS, 0, stream()>>>(parts(), std::max(nb, 0), dot_out_dev);
    HX_LAUNCH_CHECK();
 }
 
 // ---------------------------------------------------------------- relaxation
-// l1-Jacobi (src/parcsr_ls/par_relax.c:1178-1240): v = w*f - w*(A u); u_out = u + v/l1
-template <int K, typename V, typename X, bool ZERO>
-__global__ __launch_bounds__(BS) void k_relax_l1(int n, const int *__restrict__ ia,
-                                                 const int *__restrict__ ja,
-                                                 const V *__restrict__ a, const X *__restrict__ f,
-                                                 const X *__restrict__ u, const X *__restrict__ l1,
-                                                 X w, X *__restrict__ uo)
-{
-   constexpr int G = BS / K;
-   const int nb = gridDim.x;
-   const int blk = xcd_block(blockIdx.x, nb);
-   const int rows_per_block = (n + nb - 1) / nb;
-   const int r0 = blk * rows_per_block;
-   const int r1 = min(n, r0 + rows_per_block);
-   const int lane = threadIdx.x & (K - 1);
-   const int g = threadIdx.x / K;
-   for (int row = r0 + g; row < r1; row += G) {
-      if (ZERO) {
-         if (lane == 0) { uo[row] = (X) 0 + (w * f[row]) / l1[row]; }
-      } else {
-         X t = row_dot<K, V, X>(ia[row], ia[row + 1], lane, ja, a, u);
-         if (lane == 0) {
-            X v = w * f[row] - w * t;
-            uo[row] = u[row] + v / l1[row];
-         }
-      }
-   }
-}
+// l1-Jacobi (src/parcsr_ls/par_relax.c:1178-1240): v = w*f - w*(A u);
+// u_out = u + v/l1, in k_relax_l1_w / k_relax_l1_zero below.
 
 // weighted Jacobi, relax type 0 (par_relax.c:180-320, Skip_diag = 1)
 template <int K, typename V, typename X>
@@ -215,29 +265,65 @@ __global__ __launch_bounds__(BS) void k_relax_jac(int n, const int *__restrict__
    }
 }
 
+// windowed l1-Jacobi sweep: t = (A u)_row in storage order, then the update
+template <int RW, typename V, typename X>
+__global__ __launch_bounds__(BS) void k_relax_l1_w(int n, const int *__restrict__ ia,
+                                                   const int *__restrict__ ja,
+                                                   const V *__restrict__ a, const X *__restrict__ f,
+                                                   const X *__restrict__ u, const X *__restrict__ l1,
+                                                   X w, X *__restrict__ uo)
+{
+   __shared__ X sbuf_all[WPB * SW];
+   const int nb = gridDim.x;
+   const int blk = xcd_block(blockIdx.x, nb);
+   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   X *sbuf = sbuf_all + wv * SW;
+   const int ng = (n + RW - 1) / RW;
+   const int gpb = (ng + nb - 1) / nb;
+   const int g0 = blk * gpb, g1 = min(ng, g0 + gpb);
+   for (int g = g0 + wv; g < g1; g += WPB) {
+      const int r0 = g * RW;
+      const int nr = min(RW, n - r0);
+      const X t = rows_dot_window<RW, V, X>(r0, nr, lane, ia, ja, a, u, sbuf);
+      if (lane < nr) {
+         const int row = r0 + lane;
+         const X v = w * f[row] - w * t;
+         uo[row] = u[row] + v / l1[row];
+      }
+   }
+}
+
+// zero initial guess: u_out = (0 + w f) / l1, one thread per row
+template <typename X>
+__global__ void k_relax_l1_zero(int n, const X *__restrict__ f, const X *__restrict__ l1, X w,
+                                X *__restrict__ uo)
+{
+   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+      uo[i] = (X) 0 + (w * f[i]) / l1[i];
+   }
+}
+
 template <typename V, typename X>
 static void launch_relax_l1(const Csr &A, const V *vals, const X *f, const X *u, const X *l1, X w,
                             X *uo, bool zero, const char *cls, double vbytes)
 {
    if (A.nrows <= 0) { return; }
-   int K = zero ? 64 : pick_k(A.nnz, A.nrows);
-   int nb = grid_for_rows(A.nrows, K);
    double xs = sizeof(X);
    double bytes = zero ? xs * 3.0 * A.nrows
                        : (double) A.nnz * (4.0 + vbytes) + 4.0 * (A.nrows + 1) + xs * 4.0 * A.nrows;
    TimedLaunch tl(cls, bytes);
    hipStream_t s = stream();
-#define HX_RL(KK)                                                                                   \
-   if (zero) k_relax_l1<KK, V, X, true><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, f, u, l1, w, uo); \
-   else k_relax_l1<KK, V, X, false><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, f, u, l1, w, uo);
-   switch (K) {
-      case 4: HX_RL(4); break;
-      case 8: HX_RL(8); break;
-      case 16: HX_RL(16); break;
-      case 32: HX_RL(32); break;
-      default: HX_RL(64); break;
+   if (zero) {
+      const int nb = std::max(1, std::min((A.nrows + BS - 1) / BS, 16384));
+      k_relax_l1_zero<X><<<nb, BS, 0, s>>>(A.nrows, f, l1, w, uo);
+   } else {
+      const int rw = pick_rw(A.nnz, A.nrows);
+      const int nb = grid_for_groups(A.nrows, rw);
+      dispatch_rw(rw, [&](auto c) {
+         k_relax_l1_w<decltype(c)::value, V, X><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, f, u,
+                                                                  l1, w, uo);
+      });
    }
-#undef HX_RL
    HX_LAUNCH_CHECK();
 }
 

@@ -14,6 +14,7 @@ D = C.c_double
 P = C.c_void_p
 IP = C.POINTER(C.c_int)
 DP = C.POINTER(C.c_double)
+FP = C.POINTER(C.c_float)
 PP = C.POINTER(C.c_void_p)
 
 # name: (restype, [argtypes]); handles are passed as void*
@@ -125,6 +126,16 @@ SIGNATURES = {
     "HYPRE_MI355X_GenerateVarDiffusion": (P, [I, I, I, D, D, I]),
     "HYPRE_MI355X_GenerateAniso7": (P, [I, D, D]),
     "HYPRE_MI355X_ParVectorSetRandomPM": (I, [P, I]),
+    "HYPRE_MI355X_ParamModelCreate": (I, [I, I, I, I, PP]),
+    "HYPRE_MI355X_ParamModelDestroy": (I, [P]),
+    "HYPRE_MI355X_ParamModelNumParams": (I, [P, IP]),
+    "HYPRE_MI355X_ParamModelSetParams": (I, [P, FP]),
+    "HYPRE_MI355X_ParamModelGetParams": (I, [P, FP]),
+    "HYPRE_MI355X_ParamModelForward": (I, [P, I, FP, FP]),
+    "HYPRE_MI355X_ParamModelGradient": (I, [P, I, FP, FP, FP, FP]),
+    "HYPRE_MI355X_ParamModelTrain": (I, [P, I, FP, FP, I, C.c_float, FP]),
+    "HYPRE_MI355X_MatrixFeatures": (I, [P, DP]),
+    "HYPRE_MI355X_BoomerAMGSetParamModel": (I, [P, P, P]),
     "HYPRE_MI355X_SetKernelTiming": (I, [I]),
     "HYPRE_MI355X_SetKernelTimingClass": (I, [C.c_char_p]),
     "HYPRE_MI355X_ResetKernelTiming": (I, []),

@@ -117,6 +117,41 @@ HYPRE_ParCSRMatrix HYPRE_MI355X_GenerateAniso7(HYPRE_Int N, HYPRE_Real eps, HYPR
 /* x_i = 2*U_i - 1 with U the Park-Miller stream from `seed` (seed < 1 -> 1) */
 HYPRE_Int HYPRE_MI355X_ParVectorSetRandomPM(HYPRE_ParVector v, HYPRE_Int seed);
 
+/* ---- learned per-level parameters ---------------------------------------
+ * A perceptron 8 -> nh1 -> nh2 -> nout (ReLU hidden layers, sigmoid output)
+ * on the device, FP32, trained with mean-squared error and Adam.  Parameters
+ * are one float array: W1[nin][nh1], b1, W2[nh1][nh2], b2, W3[nh2][nout], b3
+ * (row-major, one sample per row of X).  Output y of level l (nout = 3 L):
+ * theta = 0.1 + 0.8 y[3l], omega = 0.6 + 0.8 y[3l+1],
+ * sweeps = 1 + floor(2 y[3l+2] + 0.5). */
+typedef struct hx_param_model_struct *HYPRE_MI355X_ParamModel;
+HYPRE_Int HYPRE_MI355X_ParamModelCreate(HYPRE_Int nin, HYPRE_Int nh1, HYPRE_Int nh2, HYPRE_Int nout,
+                                        HYPRE_MI355X_ParamModel *model);
+HYPRE_Int HYPRE_MI355X_ParamModelDestroy(HYPRE_MI355X_ParamModel model);
+HYPRE_Int HYPRE_MI355X_ParamModelNumParams(HYPRE_MI355X_ParamModel model, HYPRE_Int *np);
+/* host arrays of NumParams floats; SetParams also resets the Adam state */
+HYPRE_Int HYPRE_MI355X_ParamModelSetParams(HYPRE_MI355X_ParamModel model, const float *params);
+HYPRE_Int HYPRE_MI355X_ParamModelGetParams(HYPRE_MI355X_ParamModel model, float *params);
+/* Y[batch][nout] = sigmoid outputs for X[batch][nin] */
+HYPRE_Int HYPRE_MI355X_ParamModelForward(HYPRE_MI355X_ParamModel model, HYPRE_Int batch,
+                                         const float *X, float *Y);
+/* gradient of mean((Y - T)^2) with respect to the parameters, and the loss */
+HYPRE_Int HYPRE_MI355X_ParamModelGradient(HYPRE_MI355X_ParamModel model, HYPRE_Int batch,
+                                          const float *X, const float *T, float *grad, float *loss);
+/* `steps` full-batch Adam steps (beta1 0.9, beta2 0.999, eps 1e-8); loss
+ * before each step into loss_hist[steps] (may be NULL) */
+HYPRE_Int HYPRE_MI355X_ParamModelTrain(HYPRE_MI355X_ParamModel model, HYPRE_Int batch,
+                                       const float *X, const float *T, HYPRE_Int steps, float lr,
+                                       float *loss_hist);
+/* the 8 matrix features the model reads: log10(n)/8, mean row length/27,
+ * mean max|a_ij|/|a_ii|, mean min/max |a_ij|, mean 2nd/1st largest |a_ij|,
+ * mean strong fraction at theta 0.25, mean |row sum|/|a_ii|,
+ * std(log10|a_ii|)/4 */
+HYPRE_Int HYPRE_MI355X_MatrixFeatures(HYPRE_ParCSRMatrix A, HYPRE_Real *features);
+/* features of A -> model -> per-level theta, omega, sweeps of the solver */
+HYPRE_Int HYPRE_MI355X_BoomerAMGSetParamModel(HYPRE_Solver solver, HYPRE_MI355X_ParamModel model,
+                                              HYPRE_ParCSRMatrix A);
+
 /* ---- kernel timing with HIP events --------------------------------------
  * While enabled, every launch of the named kernel class is bracketed by HIP
  * events on the stream it runs on.  Classes: "spmv_A0" (fine-level SpMV of
