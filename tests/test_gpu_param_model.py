@@ -114,3 +114,26 @@ def test_model_sets_per_level_parameters(hx, L, PM):
         M1, M2 = a1.level_matrix(l, "A"), a2.level_matrix(l, "A")
         assert np.array_equal(M1.ja, M2.ja) and np.array_equal(M1.a, M2.a)
     m.destroy()
+
+
+def test_learned_parameters_reduce_work_on_held_out_problems(hx, L):
+    """The committed model (trained on seeds 1..64 of the anisotropic family)
+    on held-out seeds: with the fallback guard the chosen settings never need
+    more work per digit than the defaults, and on average they need less."""
+    say("held-out gain")
+    model = L.ParamModel.load()
+    ratios, picks = [], []
+    for seed in (1001, 1002, 1003, 1004, 1005, 1006):
+        A, b, eps, th = L.aniso_problem(64, seed)
+        amg = hx.BoomerAMG()
+        which, learned, default = L.apply_with_fallback(amg, model, A, b)
+        chosen = learned if which == "learned" else default
+        assert chosen.rel_res < 1e-8
+        assert chosen.work_per_digit <= default.work_per_digit
+        ratios.append(chosen.work_per_digit / default.work_per_digit)
+        picks.append(which)
+        say(seed, f"eps={eps:.3g} rot={th:.3f}", which, f"its {learned.iterations} vs {default.iterations}",
+            f"ratio {ratios[-1]:.3f}")
+        amg.destroy()
+    assert np.mean(ratios) < 0.97, ratios
+    assert "learned" in picks
