@@ -220,6 +220,62 @@ void create_s(const Csr &A, double theta, double max_row_sum, Csr &S)
    HX_LAUNCH_CHECK();
 }
 
+// The strong C neighbours of every row, in S order (count or fill pass).
+template <int K, bool FILL>
+__global__ void k_strong_c(int n, const int *__restrict__ S_ia, const int *__restrict__ S_ja,
+                           const int *__restrict__ CF, int *__restrict__ SC_ia_or_cnt,
+                           int *__restrict__ SC_ja)
+{
+   const int gl = threadIdx.x & (K - 1);
+   const int rows_per_block = blockDim.x / K;
+   const int gshift = (threadIdx.x & 63) & ~(K - 1);
+   const unsigned long long gmask = (K == 64) ? ~0ULL : ((1ULL << K) - 1);
+   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < n;
+        rb += (long long) gridDim.x * rows_per_block) {
+      const long long ir = rb + threadIdx.x / K;
+      const bool act = ir < n;
+      const int i = act ? (int) ir : 0;
+      const int s = act ? S_ia[i] : 0, e = act ? S_ia[i + 1] : 0;
+      int o = FILL && act ? SC_ia_or_cnt[i] : 0;
+      int c = 0;
+      for (int kb = s; __any(kb < e); kb += K) {
+         const int k = kb + gl;
+         int j = -1;
+         bool isc = false;
+         if (k < e) {
+            j = S_ja[k];
+            isc = CF[j] >= 0;
+         }
+         const unsigned long long m = (__ballot(isc) >> gshift) & gmask;
+         if (FILL && isc) { SC_ja[o + c + __popcll(m & ((1ULL << gl) - 1))] = j; }
+         c += __popcll(m);
+      }
+      if (!FILL && act && gl == 0) { SC_ia_or_cnt[i] = c; }
+   }
+}
+
+static void strong_c_lists(const Csr &S, const int *CF, Csr &SC)
+{
+   const int n = S.nrows;
+   SC.nrows = n;
+   SC.ncols = S.ncols;
+   SC.ia = dalloc_n<int>((size_t) n + 1);
+   SC.a = nullptr;
+   SC.a32 = nullptr;
+   const int K = lanes_per_row(S.nnz, n);
+   const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
+   dispatch_k(K, [&](auto kc) {
+      k_strong_c<decltype(kc)::value, false><<<nb, 256, 0, stream()>>>(n, S.ia, S.ja, CF, SC.ia, nullptr);
+   });
+   HX_LAUNCH_CHECK();
+   SC.nnz = exclusive_scan(SC.ia, SC.ia, n);
+   SC.ja = dalloc_n<int>((size_t) std::max<long long>(SC.nnz, 1));
+   dispatch_k(K, [&](auto kc) {
+      k_strong_c<decltype(kc)::value, true><<<nb, 256, 0, stream()>>>(n, S.ia, S.ja, CF, SC.ia, SC.ja);
+   });
+   HX_LAUNCH_CHECK();
+}
+
 // ================================================================== l1 norms
 // src/parcsr_ls/ams.c:4523-4600 option 1 + ams.c:770-797; K lanes per row,
 // |a_ij| summed in column order.
@@ -908,6 +964,7 @@ static void extpi_pass(InterpArgs a, const int *init_list = nullptr, int init_n 
 constexpr int XF_HC = 256;   // hash slots (<= 64 F markers + <= XF_MC + 64 C keys)
 constexpr int XF_MC = 32;    // C-hat points per row on the fast path
 constexpr int XF_U = 3;      // chunks of 64 entries per batch of loads
+constexpr int XF_QC = 192;   // C-hat terms of 64 entries of row i kept for pass B
 
 struct ExtpiFastArgs
 {
@@ -915,6 +972,7 @@ struct ExtpiFastArgs
    const int    *A_ia, *A_ja;
    const double *A_a;
    const int    *S_ia, *S_ja;
+   const int    *SC_ia, *SC_ja;   // strong C neighbours of every row, S order
    const int    *CF, *f2c;
    int          *raw_j;       // row i at i * XF_MC
    double       *raw_a;
@@ -951,6 +1009,10 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
    __shared__ double wgt[XF_MC];
    __shared__ int chat_key[XF_MC];
    __shared__ int chat_col[XF_MC];
+   __shared__ int mins[XF_MC];
+   __shared__ double q_a[XF_QC];
+   __shared__ int q_tgt[XF_QC];
+   __shared__ int q_seg[XF_QC];
    const int lane = threadIdx.x;
    const int nb = gridDim.x;
    const int blk = xcd_blk(blockIdx.x, nb);
@@ -984,7 +1046,8 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
             if (c1 >= 0) {
                kind = 1; klen = 1; kb = i1;
             } else if (c1 != -3) {
-               kind = 2; kb = p.S_ia[i1]; klen = p.S_ia[i1 + 1] - kb;
+               // the candidates of a strong F neighbour are its strong C neighbours
+               kind = 2; kb = p.SC_ia[i1]; klen = p.SC_ia[i1 + 1] - kb;
             }
          }
          wave_sync();
@@ -1003,7 +1066,7 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
          int nch = 0;
          const int nseg = max(ns, 1);
          for (int cb = 0; cb < L && !defer; cb += WAVE * XF_U) {
-            int key[XF_U], k1[XF_U];
+            int key[XF_U];
 #pragma unroll
             for (int u = 0; u < XF_U; u++) {
                const int q = cb + u * WAVE + lane;
@@ -1012,21 +1075,13 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                const int b_s = __shfl(kb, s, WAVE);
                const int off_s = __shfl(off, s, WAVE);
                key[u] = -1;
-               k1[u] = -1;
                if (q < L) {
                   if (kind_s == 1) {
                      key[u] = b_s;
                   } else if (kind_s == 2) {
-                     k1[u] = p.S_ja[b_s + (q - off_s)];
+                     key[u] = p.SC_ja[b_s + (q - off_s)];
                   }
                }
-            }
-            int cf1[XF_U];
-#pragma unroll
-            for (int u = 0; u < XF_U; u++) { cf1[u] = (k1[u] >= 0) ? p.CF[k1[u]] : -1; }
-#pragma unroll
-            for (int u = 0; u < XF_U; u++) {
-               if (cf1[u] >= 0) { key[u] = k1[u]; }
             }
 #pragma unroll
             for (int u = 0; u < XF_U; u++) {
@@ -1058,6 +1113,7 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                chat_col[lane] = p.f2c[chat_key[lane]];
                wgt[lane] = 0.0;
             }
+            if (lane < XF_MC) { mins[lane] = 0x7fffffff; }
             wave_sync();
             // ---- weights (par_lr_interp.c:1640-1805), 64 entries of row i per block
             double diag = p.A_a[a0];
@@ -1093,7 +1149,8 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                bool shas_i = false;
                double acc = 0.0;
                int acc_s = -1;
-               for (int cb = 0; cb < SL; cb += WAVE * XF_U) {
+               int qn = 0;
+               for (int cb = 0; cb < SL && !defer; cb += WAVE * XF_U) {
                   int i2[XF_U], sq[XF_U];
                   double a2[XF_U];
                   bool sf[XF_U], sn[XF_U];
@@ -1117,6 +1174,7 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
 #pragma unroll
                   for (int u = 0; u < XF_U; u++) {
                      bool qual = false, isi = false;
+                     int tq = -1;
                      if (sf[u]) {
                         const bool opp = sn[u] ? (a2[u] > 0) : (a2[u] < 0);
                         if (opp) {
@@ -1125,10 +1183,31 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                               isi = true;
                            } else {
                               const int s2 = H.find(i2[u]);
-                              qual = (s2 >= 0) && vals[s2] >= 0;
+                              tq = (s2 >= 0) ? vals[s2] : -1;
+                              qual = tq >= 0;
+                              if (!qual) { tq = -1; }
                            }
                         }
                      }
+                     // C-hat terms of this step, in order, for pass B: a direct
+                     // entry (segment of kind 1) or a distributed one
+                     // (cross-lane reads with every lane active: a lane-permute
+                     // returns nothing from an inactive source lane)
+                     const int q = cb + u * WAVE + lane;
+                     const int k_sq = __shfl(sk, sq[u], WAVE);
+                     const int b_sq = __shfl(sb, sq[u], WAVE);
+                     const double a_dir = shfl_d(aij, sq[u]);
+                     const bool dir = (q < SL) && k_sq == 1;
+                     if (dir) { tq = b_sq; }
+                     const unsigned long long qm = __ballot(tq >= 0);
+                     if (qn + __popcll(qm) > XF_QC) { defer = true; break; }
+                     if (tq >= 0) {
+                        const int pos = qn + __popcll(qm & lanemask_lt(lane));
+                        q_tgt[pos] = tq;
+                        q_seg[pos] = sq[u] | (dir ? 0x100 : 0);
+                        q_a[pos] = dir ? a_dir : a2[u];
+                     }
+                     qn += __popcll(qm);
                      unsigned long long m = __ballot(qual);
                      while (m) {
                         const int t = __ffsll((long long) m) - 1;
@@ -1151,6 +1230,7 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                      }
                   }
                }
+               if (defer) { break; }
                if (acc_s >= 0 && lane == acc_s) { ssum = acc; }
                // per entry: distribution factor and diagonal term, in entry order
                bool hasd = false, sdist = false;
@@ -1174,70 +1254,46 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                   diag += readlane_d(dval, t);
                   dm &= dm - 1;
                }
-               // pass B: a_ij into its weight, distributed values into theirs,
-               // one segment per round
-               const double segx = (sk == 1) ? aij : dist;
-               for (int cb = 0; cb < SL; cb += WAVE * XF_U) {
-                  int tgt[XF_U], sq[XF_U], i2[XF_U];
-                  double x[XF_U], a2[XF_U];
-                  bool ld[XF_U], sn[XF_U];
-#pragma unroll
-                  for (int u = 0; u < XF_U; u++) {
-                     const int q = cb + u * WAVE + lane;
-                     const int s = seg_find_lanes(soff, nseg2, q);
-                     const int k_s = __shfl(sk, s, WAVE);
-                     const int b_s = __shfl(sb, s, WAVE);
-                     const int o_s = __shfl(soff, s, WAVE);
-                     const int d_s = __shfl((int) sdist, s, WAVE);
-                     const double x_s = shfl_d(segx, s);
-                     sn[u] = __shfl((int) sneg, s, WAVE) != 0;
-                     sq[u] = s;
-                     tgt[u] = -1;
-                     x[u] = x_s;
-                     ld[u] = false;
-                     i2[u] = -1;
-                     a2[u] = 0.0;
-                     if (q < SL) {
-                        if (k_s == 1) {
-                           tgt[u] = b_s;
-                        } else if (k_s == 2 && d_s) {
-                           ld[u] = true;
-                           i2[u] = p.A_ja[b_s + (q - o_s)];
-                           a2[u] = p.A_a[b_s + (q - o_s)];
-                        }
+               // pass B: the C-hat terms collected in pass A, in order: a_ij of a
+               // C-hat entry, distribute * a of a strong F neighbour with a
+               // nonzero denominator.  Terms of one step that hit the same weight
+               // are applied lowest lane first (atomicMin on mins[weight]).
+               wave_sync();
+               for (int qb = 0; qb < qn; qb += WAVE) {
+                  const int e = qb + lane;
+                  const bool valid = e < qn;
+                  const int tg = valid ? q_tgt[e] : 0;
+                  const int sgw = valid ? q_seg[e] : 0;
+                  const int sg = sgw & 0xff;
+                  const bool dr = (sgw & 0x100) != 0;
+                  const double a = valid ? q_a[e] : 0.0;
+                  const int d_s = __shfl((int) sdist, sg, WAVE);
+                  const double dist_s = shfl_d(dist, sg);
+                  bool pending = valid && (dr || d_s);
+                  const double x = dr ? a : dist_s * a;
+                  while (__ballot(pending)) {
+                     if (pending) { atomicMin(&mins[tg], lane); }
+                     wave_sync();
+                     if (pending && mins[tg] == lane) {
+                        wgt[tg] += x;
+                        mins[tg] = 0x7fffffff;
+                        pending = false;
                      }
-                  }
-#pragma unroll
-                  for (int u = 0; u < XF_U; u++) {
-                     if (cb + u * WAVE >= SL) { break; }
-                     if (ld[u]) {
-                        const bool opp = sn[u] ? (a2[u] > 0) : (a2[u] < 0);
-                        if (opp && i2[u] != i) {
-                           const int s2 = H.find(i2[u]);
-                           const int v2 = (s2 >= 0) ? vals[s2] : -1;
-                           if (v2 >= 0) {
-                              tgt[u] = v2;
-                              x[u] = x[u] * a2[u];
-                           }
-                        }
-                     }
-                     const int s_lo = readlane_i(sq[u], 0);
-                     const int s_hi = readlane_i(sq[u], min(WAVE - 1, SL - 1 - (cb + u * WAVE)));
-                     for (int sg = s_lo; sg <= s_hi; sg++) {
-                        if (tgt[u] >= 0 && sq[u] == sg) { wgt[tgt[u]] += x[u]; }
-                        wave_sync();
-                     }
+                     wave_sync();
                   }
                }
+               wave_sync();
             }
             wave_sync();
-            if (lane < nch) {
-               double w = wgt[lane];
-               if (diag != 0.0) { w /= -diag; }
-               p.raw_j[(size_t) i * XF_MC + lane] = chat_col[lane];
-               p.raw_a[(size_t) i * XF_MC + lane] = w;
+            if (!defer) {
+               if (lane < nch) {
+                  double w = wgt[lane];
+                  if (diag != 0.0) { w /= -diag; }
+                  p.raw_j[(size_t) i * XF_MC + lane] = chat_col[lane];
+                  p.raw_a[(size_t) i * XF_MC + lane] = w;
+               }
+               if (lane == 0) { p.cnt[i] = nch; }
             }
-            if (lane == 0) { p.cnt[i] = nch; }
          }
       }
       if (defer && lane == 0) {
@@ -1416,10 +1472,13 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
    if (max_elmts > 0 && max_elmts <= XF_MC) {
       // fast path into raw rows, truncation per thread, leftovers by k_extpi
       stride = max_elmts;
+      Csr SC;
+      strong_c_lists(S, CF, SC);
       ExtpiFastArgs f{};
       f.n = n;
       f.A_ia = A.ia; f.A_ja = A.ja; f.A_a = A.a;
       f.S_ia = S.ia; f.S_ja = S.ja;
+      f.SC_ia = SC.ia; f.SC_ja = SC.ja;
       f.CF = CF; f.f2c = f2c;
       int *raw_cnt = dalloc_n<int>((size_t) n + 1);
       f.raw_j = dalloc_n<int>((size_t) std::max(n, 1) * XF_MC);
@@ -1444,6 +1503,7 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
       dfree(f.raw_j);
       dfree(f.raw_a);
       dfree(raw_cnt);
+      SC.free_all();
       if (nd > 0) {
          a.base = nullptr;
          a.out_j = tmp_j;

@@ -64,21 +64,27 @@ __device__ __forceinline__ X row_dot(int s, int e, int lane, const int *__restri
 constexpr int SW = 512;
 constexpr int WPB = BS / 64;   // waves per block
 
+// rows per wave: one row per lane (windows are pipelined), fewer only for
+// very long rows so that a lane's serial sum stays short
 static int pick_rw(long long nnz, int n)
 {
    double avg = n > 0 ? (double) nnz / n : 1.0;
    int rw = 64;
-   while (rw > 1 && rw * avg > SW) { rw >>= 1; }
+   while (rw > 1 && rw * avg > 8 * SW) { rw >>= 1; }
    return rw;
 }
 
 __device__ __forceinline__ void wave_fence() { __builtin_amdgcn_wave_barrier(); }
 
+// The matrix is streamed once: its loads are non-temporal so that x stays in
+// L2.  The a/ja loads of the next window are issued before the current
+// window's products are summed, so a wave always has a window in flight.
 template <int RW, typename V, typename X>
 __device__ __forceinline__ X rows_dot_window(int r0, int nr, int lane, const int *__restrict__ ia,
                                              const int *__restrict__ ja, const V *__restrict__ a,
                                              const X *__restrict__ x, X *sbuf)
 {
+   constexpr int U = SW / 64;
    int rs = 0, re = 0;
    if (lane < nr) {
       rs = ia[r0 + lane];
@@ -87,16 +93,31 @@ __device__ __forceinline__ X rows_dot_window(int r0, int nr, int lane, const int
    const int base = __shfl(rs, 0, 64);
    const int end = __shfl(re, nr - 1, 64);
    X t = 0;
+   int jn[U];
+   V an[U];
+#pragma unroll
+   for (int u = 0; u < U; u++) {
+      const int j = base + u * 64 + lane;
+      jn[u] = (j < end) ? __builtin_nontemporal_load(ja + j) : 0;
+      an[u] = (j < end) ? __builtin_nontemporal_load(a + j) : (V) 0;
+   }
    for (int wb = base; wb < end; wb += SW) {
       const int we = min(end, wb + SW);
-      X p[SW / 64];
+      X p[U];
 #pragma unroll
-      for (int u = 0; u < SW / 64; u++) {
+      for (int u = 0; u < U; u++) {
          const int j = wb + u * 64 + lane;
-         p[u] = (j < we) ? (X) a[j] * x[ja[j]] : (X) 0;
+         p[u] = (j < we) ? (X) an[u] * x[jn[u]] : (X) 0;
+      }
+      const int nxt = wb + SW;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+         const int j = nxt + u * 64 + lane;
+         jn[u] = (j < end) ? __builtin_nontemporal_load(ja + j) : 0;
+         an[u] = (j < end) ? __builtin_nontemporal_load(a + j) : (V) 0;
       }
 #pragma unroll
-      for (int u = 0; u < SW / 64; u++) { sbuf[u * 64 + lane] = p[u]; }
+      for (int u = 0; u < U; u++) { sbuf[u * 64 + lane] = p[u]; }
       wave_fence();
       if (lane < nr) {
          const int s = max(rs, wb), e = min(re, we);
