@@ -254,6 +254,86 @@ __global__ void k_strong_c(int n, const int *__restrict__ S_ia, const int *__res
    }
 }
 
+// Off-diagonal entries of every row of A whose sign is opposite to the row's
+// diagonal and whose column is a C point, in storage order: absolute entry
+// index, column and value (count or fill pass).
+template <int K, bool FILL>
+__global__ void k_opp_c(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                        const double *__restrict__ a, const int *__restrict__ CF,
+                        int *__restrict__ OC_ia_or_cnt, int *__restrict__ OC_k,
+                        int *__restrict__ OC_j, double *__restrict__ OC_a)
+{
+   const int gl = threadIdx.x & (K - 1);
+   const int rows_per_block = blockDim.x / K;
+   const int gshift = (threadIdx.x & 63) & ~(K - 1);
+   const unsigned long long gmask = (K == 64) ? ~0ULL : ((1ULL << K) - 1);
+   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < n;
+        rb += (long long) gridDim.x * rows_per_block) {
+      const long long ir = rb + threadIdx.x / K;
+      const bool act = ir < n;
+      const int i = act ? (int) ir : 0;
+      const int s = act ? ia[i] : 0, e = act ? ia[i + 1] : 0;
+      const bool neg = act && a[s] < 0;
+      int o = FILL && act ? OC_ia_or_cnt[i] : 0;
+      int c = 0;
+      for (int kb = s + 1; __any(kb < e); kb += K) {
+         const int k = kb + gl;
+         bool take = false;
+         int j = 0;
+         double v = 0.0;
+         if (k < e) {
+            v = a[k];
+            if (neg ? (v > 0) : (v < 0)) {
+               j = ja[k];
+               take = CF[j] >= 0;
+            }
+         }
+         const unsigned long long m = (__ballot(take) >> gshift) & gmask;
+         if (FILL && take) {
+            const int pos = o + c + __popcll(m & ((1ULL << gl) - 1));
+            OC_k[pos] = k;
+            OC_j[pos] = j;
+            OC_a[pos] = v;
+         }
+         c += __popcll(m);
+      }
+      if (!FILL && act && gl == 0) { OC_ia_or_cnt[i] = c; }
+   }
+}
+
+struct OppCLists
+{
+   int    *ia = nullptr, *k = nullptr, *j = nullptr;
+   double *a = nullptr;
+   void    free_all()
+   {
+      dfree(ia); dfree(k); dfree(j); dfree(a);
+      ia = k = j = nullptr; a = nullptr;
+   }
+};
+
+static void opp_c_lists(const Csr &A, const int *CF, OppCLists &OC)
+{
+   const int n = A.nrows;
+   OC.ia = dalloc_n<int>((size_t) n + 1);
+   const int K = lanes_per_row(A.nnz, n);
+   const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
+   dispatch_k(K, [&](auto kc) {
+      k_opp_c<decltype(kc)::value, false><<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, CF, OC.ia,
+                                                                     nullptr, nullptr, nullptr);
+   });
+   HX_LAUNCH_CHECK();
+   const long long tot = exclusive_scan(OC.ia, OC.ia, n);
+   OC.k = dalloc_n<int>((size_t) std::max<long long>(tot, 1));
+   OC.j = dalloc_n<int>((size_t) std::max<long long>(tot, 1));
+   OC.a = dalloc_n<double>((size_t) std::max<long long>(tot, 1));
+   dispatch_k(K, [&](auto kc) {
+      k_opp_c<decltype(kc)::value, true><<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, CF, OC.ia,
+                                                                    OC.k, OC.j, OC.a);
+   });
+   HX_LAUNCH_CHECK();
+}
+
 static void strong_c_lists(const Csr &S, const int *CF, Csr &SC)
 {
    const int n = S.nrows;
@@ -973,6 +1053,8 @@ struct ExtpiFastArgs
    const double *A_a;
    const int    *S_ia, *S_ja;
    const int    *SC_ia, *SC_ja;   // strong C neighbours of every row, S order
+   const int    *OC_ia, *OC_k, *OC_j;   // opposite-sign C entries: entry index, column
+   const double *OC_a;                  // and value
    const int    *CF, *f2c;
    int          *raw_j;       // row i at i * XF_MC
    double       *raw_a;
@@ -1010,6 +1092,7 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
    __shared__ int chat_key[XF_MC];
    __shared__ int chat_col[XF_MC];
    __shared__ int mins[XF_MC];
+   __shared__ int seg_ki[WAVE];
    __shared__ double q_a[XF_QC];
    __shared__ int q_tgt[XF_QC];
    __shared__ int q_seg[XF_QC];
@@ -1119,9 +1202,11 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
             double diag = p.A_a[a0];
             for (int jb = 0; jb < na; jb += WAVE) {
                const int j = jb + lane;
-               // segment of entry j: kind 1 C-hat (one entry, b = weight index),
-               // 2 strong F (row c1 without its diagonal), 3 weak, 0 none
-               int c1 = 0, v = -1, sk = 0, sb = 0, slen = 0;
+               // entry j of row i: kind 1 C-hat (weight index v), 2 strong F
+               // neighbour c1, 3 weak, 0 none
+               int c1 = 0, v = -1, sk = 0;
+               int ib = 0, ilen = 0;          // row c1 without its diagonal (i search)
+               int ob = 0, olen = 0;          // opposite-sign C entries of row c1
                bool sneg = false;
                double aij = 0.0;
                if (j < na) {
@@ -1130,108 +1215,98 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                   const int sl = H.find(c1);
                   v = (sl >= 0) ? vals[sl] : -1;
                   if (v >= 0) {
-                     sk = 1; sb = v; slen = 1;
+                     sk = 1; olen = 1;
                   } else if (v == -2) {
                      const int b1 = p.A_ia[c1];
-                     sk = 2; sb = b1 + 1; slen = p.A_ia[c1 + 1] - b1 - 1;
+                     sk = 2;
+                     ib = b1 + 1;
+                     ilen = p.A_ia[c1 + 1] - b1 - 1;
                      sneg = p.A_a[b1] < 0;
+                     ob = p.OC_ia[c1];
+                     olen = p.OC_ia[c1 + 1] - ob;
                   } else if (p.CF[c1] != -3) {
                      sk = 3;
                   }
                }
-               const int sincl = wave_incl_scan(slen, lane);
-               const int SL = readlane_i(sincl, WAVE - 1);
-               const int soff = sincl - slen;
                const int nseg2 = min(WAVE, na - jb);
-               // pass A: denominators sum over row c1 of the entries of opposite
-               // sign whose column is in C-hat or is i, in column order
-               double ssum = 0.0, sai = 0.0;
-               bool shas_i = false;
-               double acc = 0.0;
-               int acc_s = -1;
-               int qn = 0;
-               for (int cb = 0; cb < SL && !defer; cb += WAVE * XF_U) {
-                  int i2[XF_U], sq[XF_U];
-                  double a2[XF_U];
-                  bool sf[XF_U], sn[XF_U];
+               // pass I: the entry of column i in each strong F neighbour row
+               const int iincl = wave_incl_scan(ilen, lane);
+               const int IL = readlane_i(iincl, WAVE - 1);
+               const int ioff = iincl - ilen;
+               seg_ki[lane] = -1;
+               wave_sync();
+               for (int cb = 0; cb < IL; cb += WAVE * XF_U) {
+                  int kk[XF_U], sq[XF_U];
+                  bool hit[XF_U];
 #pragma unroll
                   for (int u = 0; u < XF_U; u++) {
                      const int q = cb + u * WAVE + lane;
-                     const int s = seg_find_lanes(soff, nseg2, q);
-                     const int k_s = __shfl(sk, s, WAVE);
-                     const int b_s = __shfl(sb, s, WAVE);
-                     const int o_s = __shfl(soff, s, WAVE);
-                     sn[u] = __shfl((int) sneg, s, WAVE) != 0;
+                     const int s = seg_find_lanes(ioff, nseg2, q);
+                     const int b_s = __shfl(ib, s, WAVE);
+                     const int o_s = __shfl(ioff, s, WAVE);
                      sq[u] = s;
-                     sf[u] = (q < SL) && k_s == 2;
-                     i2[u] = -1;
-                     a2[u] = 0.0;
-                     if (sf[u]) {
-                        i2[u] = p.A_ja[b_s + (q - o_s)];
-                        a2[u] = p.A_a[b_s + (q - o_s)];
-                     }
+                     kk[u] = b_s + (q - o_s);
+                     hit[u] = (q < IL) && p.A_ja[kk[u]] == i;
                   }
 #pragma unroll
                   for (int u = 0; u < XF_U; u++) {
-                     bool qual = false, isi = false;
-                     int tq = -1;
-                     if (sf[u]) {
-                        const bool opp = sn[u] ? (a2[u] > 0) : (a2[u] < 0);
-                        if (opp) {
-                           if (i2[u] == i) {
-                              qual = true;
-                              isi = true;
-                           } else {
-                              const int s2 = H.find(i2[u]);
-                              tq = (s2 >= 0) ? vals[s2] : -1;
-                              qual = tq >= 0;
-                              if (!qual) { tq = -1; }
-                           }
+                     if (hit[u]) { seg_ki[sq[u]] = kk[u]; }   // one hit per row at most
+                  }
+               }
+               wave_sync();
+               const int ki = seg_ki[lane];
+               double sai = 0.0;
+               bool shas_i = false;
+               if (sk == 2 && ki >= 0) {
+                  sai = p.A_a[ki];
+                  shas_i = sneg ? (sai > 0) : (sai < 0);
+               }
+               // pass A, one lane per strong F neighbour c1: the denominator
+               // sums, in column order, the opposite-sign entries of row c1
+               // whose column is in C-hat, with the entry of column i merged
+               // in by its storage index.  The C-hat terms for pass B are
+               // counted here and written below at prefix-sum offsets.
+               double ssum = 0.0;
+               int nq = (sk == 1) ? 1 : 0;
+               if (sk == 2) {
+                  double acc = 0.0;
+                  bool pend = shas_i;
+                  for (int e = ob; e < ob + olen; e++) {
+                     const int s2 = H.find(p.OC_j[e]);
+                     if (s2 >= 0 && vals[s2] >= 0) {
+                        if (pend && ki < p.OC_k[e]) {
+                           acc += sai;
+                           pend = false;
                         }
+                        acc += p.OC_a[e];
+                        nq++;
                      }
-                     // C-hat terms of this step, in order, for pass B: a direct
-                     // entry (segment of kind 1) or a distributed one
-                     // (cross-lane reads with every lane active: a lane-permute
-                     // returns nothing from an inactive source lane)
-                     const int q = cb + u * WAVE + lane;
-                     const int k_sq = __shfl(sk, sq[u], WAVE);
-                     const int b_sq = __shfl(sb, sq[u], WAVE);
-                     const double a_dir = shfl_d(aij, sq[u]);
-                     const bool dir = (q < SL) && k_sq == 1;
-                     if (dir) { tq = b_sq; }
-                     const unsigned long long qm = __ballot(tq >= 0);
-                     if (qn + __popcll(qm) > XF_QC) { defer = true; break; }
-                     if (tq >= 0) {
-                        const int pos = qn + __popcll(qm & lanemask_lt(lane));
-                        q_tgt[pos] = tq;
-                        q_seg[pos] = sq[u] | (dir ? 0x100 : 0);
-                        q_a[pos] = dir ? a_dir : a2[u];
-                     }
-                     qn += __popcll(qm);
-                     unsigned long long m = __ballot(qual);
-                     while (m) {
-                        const int t = __ffsll((long long) m) - 1;
-                        const int st = readlane_i(sq[u], t);
-                        if (st != acc_s) {
-                           if (acc_s >= 0 && lane == acc_s) { ssum = acc; }
-                           acc_s = st;
-                           acc = 0.0;
+                  }
+                  if (pend) { acc += sai; }
+                  ssum = acc;
+               }
+               const int qincl = wave_incl_scan(nq, lane);
+               const int qn = readlane_i(qincl, WAVE - 1);
+               if (qn > XF_QC) { defer = true; break; }
+               {
+                  int pos = qincl - nq;
+                  if (sk == 1) {
+                     q_tgt[pos] = v;
+                     q_seg[pos] = lane | 0x100;
+                     q_a[pos] = aij;
+                  } else if (sk == 2) {
+                     for (int e = ob; e < ob + olen; e++) {
+                        const int s2 = H.find(p.OC_j[e]);
+                        const int v2 = (s2 >= 0) ? vals[s2] : -1;
+                        if (v2 >= 0) {
+                           q_tgt[pos] = v2;
+                           q_seg[pos] = lane;
+                           q_a[pos] = p.OC_a[e];
+                           pos++;
                         }
-                        acc += readlane_d(a2[u], t);
-                        m &= m - 1;
-                     }
-                     unsigned long long mi = __ballot(isi);
-                     while (mi) {
-                        const int t = __ffsll((long long) mi) - 1;
-                        const int st = readlane_i(sq[u], t);
-                        const double av = readlane_d(a2[u], t);
-                        if (lane == st) { sai = av; shas_i = true; }
-                        mi &= mi - 1;
                      }
                   }
                }
-               if (defer) { break; }
-               if (acc_s >= 0 && lane == acc_s) { ssum = acc; }
                // per entry: distribution factor and diagonal term, in entry order
                bool hasd = false, sdist = false;
                double dval = 0.0, dist = 0.0;
@@ -1474,11 +1549,14 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
       stride = max_elmts;
       Csr SC;
       strong_c_lists(S, CF, SC);
+      OppCLists OC;
+      opp_c_lists(A, CF, OC);
       ExtpiFastArgs f{};
       f.n = n;
       f.A_ia = A.ia; f.A_ja = A.ja; f.A_a = A.a;
       f.S_ia = S.ia; f.S_ja = S.ja;
       f.SC_ia = SC.ia; f.SC_ja = SC.ja;
+      f.OC_ia = OC.ia; f.OC_k = OC.k; f.OC_j = OC.j; f.OC_a = OC.a;
       f.CF = CF; f.f2c = f2c;
       int *raw_cnt = dalloc_n<int>((size_t) n + 1);
       f.raw_j = dalloc_n<int>((size_t) std::max(n, 1) * XF_MC);
@@ -1504,6 +1582,7 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
       dfree(f.raw_a);
       dfree(raw_cnt);
       SC.free_all();
+      OC.free_all();
       if (nd > 0) {
          a.base = nullptr;
          a.out_j = tmp_j;
@@ -1816,6 +1895,7 @@ static void rap_pass(RapArgs a, const int *init_list = nullptr, int init_n = 0)
 // entries or whose Ac row exceeds RF_ACL entries are deferred.
 constexpr int RF_AC = 256;
 constexpr int RF_ACL = 128;
+constexpr int RF_U = 3;      // chunks of 64 products per batch of loads
 
 struct RapFastArgs
 {
@@ -1839,7 +1919,7 @@ struct RapFastArgs
 
 static size_t rap_fast_lds(int rc, int rcl)
 {
-   return (size_t) rc * 12 + (size_t) rcl * 4 + (size_t) RF_AC * 20 + RF_ACL * 4 + 64 * 8 + 129 * 4;
+   return (size_t) rc * 12 + (size_t) rcl * 4 + (size_t) RF_AC * 20 + RF_ACL * 4;
 }
 
 __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
@@ -1850,15 +1930,12 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
    const int blk = xcd_blk(blockIdx.x, nb);
    double *ra_val = dyn_lds;                    // rc
    double *ac_val = ra_val + p.rc;              // RF_AC
-   double *seg_r = ac_val + RF_AC;              // 64
-   int *ra_keys = reinterpret_cast<int *>(seg_r + 64);
+   int *ra_keys = reinterpret_cast<int *>(ac_val + RF_AC);
    int *ra_order = ra_keys + p.rc;              // rcl
    int *ac_keys = ra_order + p.rcl;             // RF_AC
    int *ac_pos = ac_keys + RF_AC;
    int *ac_mins = ac_pos + RF_AC;
    int *ac_order = ac_mins + RF_AC;             // RF_ACL
-   int *seg_off = ac_order + RF_ACL;            // 65
-   int *seg_b = seg_off + 65;                   // 64
    RowHash HR{ra_keys, nullptr, p.rc, ilog2(p.rc)};
    RowHash HC{ac_keys, nullptr, RF_AC, ilog2(RF_AC)};
 
@@ -1889,45 +1966,54 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
          }
          const int incl = wave_incl_scan(len, lane);
          const int L = readlane_i(incl, WAVE - 1);
+         const int off = incl - len;
          const int nseg = min(WAVE, rb1 - rb);
-         seg_off[lane] = incl - len;
-         seg_b[lane] = ab;
-         seg_r[lane] = rv;
-         wave_sync();
-         for (int cb = 0; cb < L && !defer; cb += WAVE) {
-            const int q = cb + lane;
-            const bool valid = q < L;
-            int s = 0, key = -1;
-            double x = 0.0;
-            if (valid) {
-               s = seg_find(seg_off, nseg, q);
-               const int k = seg_b[s] + (q - seg_off[s]);
-               key = p.A_ja[k];
-               x = seg_r[s] * p.A_a[k];
-            }
-            const int s_lo = readlane_i(s, 0);
-            const int s_hi = readlane_i(s, min(WAVE - 1, L - 1 - cb));
-            for (int sg = s_lo; sg <= s_hi; sg++) {
-               const bool act = valid && s == sg;
-               bool isn = false;
-               int sl = -1;
-               if (act) { sl = HR.insert(key, isn); }
-               const unsigned long long nm = __ballot(isn);
-               if (nra + __popcll(nm) > p.rcl) { defer = true; break; }
-               if (act && sl >= 0) {
-                  if (isn) {
-                     const int pos = nra + __popcll(nm & lanemask_lt(lane));
-                     ra_order[pos] = sl;
-                     ra_val[sl] = x;
-                  } else {
-                     ra_val[sl] += x;
-                  }
+         for (int cb = 0; cb < L && !defer; cb += WAVE * RF_U) {
+            int key[RF_U], sq[RF_U];
+            double x[RF_U];
+#pragma unroll
+            for (int u = 0; u < RF_U; u++) {
+               const int q = cb + u * WAVE + lane;
+               const int s = seg_find_lanes(off, nseg, q);
+               const int b_s = __shfl(ab, s, WAVE);
+               const int o_s = __shfl(off, s, WAVE);
+               const double r_s = shfl_d(rv, s);
+               sq[u] = s;
+               key[u] = -1;
+               x[u] = 0.0;
+               if (q < L) {
+                  const int k = b_s + (q - o_s);
+                  key[u] = p.A_ja[k];
+                  x[u] = r_s * p.A_a[k];
                }
-               nra += __popcll(nm);
-               wave_sync();
+            }
+#pragma unroll
+            for (int u = 0; u < RF_U; u++) {
+               if (cb + u * WAVE >= L || defer) { break; }
+               const bool valid = cb + u * WAVE + lane < L;
+               const int s_lo = readlane_i(sq[u], 0);
+               const int s_hi = readlane_i(sq[u], min(WAVE - 1, L - 1 - (cb + u * WAVE)));
+               for (int sg = s_lo; sg <= s_hi; sg++) {
+                  const bool act = valid && sq[u] == sg;
+                  bool isn = false;
+                  int sl = -1;
+                  if (act) { sl = HR.insert(key[u], isn); }
+                  const unsigned long long nm = __ballot(isn);
+                  if (nra + __popcll(nm) > p.rcl) { defer = true; break; }
+                  if (act && sl >= 0) {
+                     if (isn) {
+                        const int pos = nra + __popcll(nm & lanemask_lt(lane));
+                        ra_order[pos] = sl;
+                        ra_val[sl] = x[u];
+                     } else {
+                        ra_val[sl] += x[u];
+                     }
+                  }
+                  nra += __popcll(nm);
+                  wave_sync();
+               }
             }
          }
-         wave_sync();
       }
       // ---- (RA) x P, diagonal first (par_rap.c:1612-1620, 1860-1920)
       int nac = 0;
@@ -1954,48 +2040,56 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
             }
             const int incl = wave_incl_scan(len, lane);
             const int L = readlane_i(incl, WAVE - 1);
+            const int off = incl - len;
             const int nseg = min(WAVE, nra - kb);
-            wave_sync();                         // seg arrays of the previous step are dead
-            seg_off[lane] = incl - len;
-            seg_b[lane] = pb;
-            seg_r[lane] = rav;
-            wave_sync();
-            for (int cb = 0; cb < L && !defer; cb += WAVE) {
-               const int q = cb + lane;
-               const bool valid = q < L;
-               int key = -1;
-               double x = 0.0;
-               if (valid) {
-                  const int s = seg_find(seg_off, nseg, q);
-                  const int e = seg_b[s] + (q - seg_off[s]);
-                  key = p.P_ja[e];
-                  x = seg_r[s] * p.P_a[e];
-               }
-               bool isn = false;
-               int sl = -1;
-               if (valid) { sl = HC.insert(key, isn); }
-               if (nac + __popcll(__ballot(isn)) > RF_ACL) { defer = true; break; }
-               bool pending = valid && sl >= 0;
-               while (__ballot(pending)) {
-                  if (pending) { atomicMin(&ac_mins[sl], lane); }
-                  wave_sync();
-                  const bool apply = pending && ac_mins[sl] == lane;
-                  const bool first = apply && ac_pos[sl] < 0;
-                  const unsigned long long fm = __ballot(first);
-                  if (apply) {
-                     if (first) {
-                        const int pos = nac + __popcll(fm & lanemask_lt(lane));
-                        ac_pos[sl] = pos;
-                        ac_order[pos] = sl;
-                        ac_val[sl] = x;
-                     } else {
-                        ac_val[sl] += x;
-                     }
-                     ac_mins[sl] = 0x7fffffff;
-                     pending = false;
+            for (int cb = 0; cb < L && !defer; cb += WAVE * RF_U) {
+               int key[RF_U];
+               double x[RF_U];
+#pragma unroll
+               for (int u = 0; u < RF_U; u++) {
+                  const int q = cb + u * WAVE + lane;
+                  const int s = seg_find_lanes(off, nseg, q);
+                  const int b_s = __shfl(pb, s, WAVE);
+                  const int o_s = __shfl(off, s, WAVE);
+                  const double r_s = shfl_d(rav, s);
+                  key[u] = -1;
+                  x[u] = 0.0;
+                  if (q < L) {
+                     const int e = b_s + (q - o_s);
+                     key[u] = p.P_ja[e];
+                     x[u] = r_s * p.P_a[e];
                   }
-                  nac += __popcll(fm);
-                  wave_sync();
+               }
+#pragma unroll
+               for (int u = 0; u < RF_U; u++) {
+                  if (cb + u * WAVE >= L || defer) { break; }
+                  const bool valid = cb + u * WAVE + lane < L;
+                  bool isn = false;
+                  int sl = -1;
+                  if (valid) { sl = HC.insert(key[u], isn); }
+                  if (nac + __popcll(__ballot(isn)) > RF_ACL) { defer = true; break; }
+                  bool pending = valid && sl >= 0;
+                  while (__ballot(pending)) {
+                     if (pending) { atomicMin(&ac_mins[sl], lane); }
+                     wave_sync();
+                     const bool apply = pending && ac_mins[sl] == lane;
+                     const bool first = apply && ac_pos[sl] < 0;
+                     const unsigned long long fm = __ballot(first);
+                     if (apply) {
+                        if (first) {
+                           const int pos = nac + __popcll(fm & lanemask_lt(lane));
+                           ac_pos[sl] = pos;
+                           ac_order[pos] = sl;
+                           ac_val[sl] = x[u];
+                        } else {
+                           ac_val[sl] += x[u];
+                        }
+                        ac_mins[sl] = 0x7fffffff;
+                        pending = false;
+                     }
+                     nac += __popcll(fm);
+                     wave_sync();
+                  }
                }
             }
          }
