@@ -18,6 +18,8 @@
 #include "hx_internal.h"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 
 namespace hx {
 
@@ -48,6 +50,16 @@ static int launch_blocks(long long work_items)
    long long b = std::min<long long>(std::max<long long>(work_items, 1), 256 * 32);
    if (b >= 64) { b = b / 8 * 8; }
    return (int) b;
+}
+
+// HYPRE_MI355X_VERBOSE=1 prints how many rows each setup stage hands on
+static bool setup_verbose()
+{
+   static const bool v = [] {
+      const char *e = std::getenv("HYPRE_MI355X_VERBOSE");
+      return e && std::atoi(e) > 0;
+   }();
+   return v;
 }
 
 // one-wave blocks that all fit on the chip at once (256 CUs, 160 KB LDS and
@@ -475,40 +487,72 @@ __global__ void k_pmis_init(int n, const int *__restrict__ S_ia, const int *__re
 }
 
 // par_coarsen.c:2506-2550: drop candidates with a larger-measure S-neighbour
+// (K lanes per row; every write is a 0, so their order does not matter)
+template <int K>
 __global__ void k_pmis_remove(int n, const int *__restrict__ S_ia, const int *__restrict__ S_ja,
                               const double *__restrict__ m, int *CF)
 {
-   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-      double mi = m[i];
-      if (!(mi > 1)) { continue; }
-      for (int k = S_ia[i]; k < S_ia[i + 1]; k++) {
-         int j = S_ja[k];
-         double mj = m[j];
-         if (mj > 1) {
-            if (mi > mj) { CF[j] = 0; }
-            else if (mj > mi) { CF[i] = 0; }
+   const int gl = threadIdx.x & (K - 1);
+   const int rows_per_block = blockDim.x / K;
+   const int gshift = (threadIdx.x & 63) & ~(K - 1);
+   const unsigned long long gmask = (K == 64) ? ~0ULL : ((1ULL << K) - 1);
+   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < n;
+        rb += (long long) gridDim.x * rows_per_block) {
+      const long long ir = rb + threadIdx.x / K;
+      const bool act = ir < n;
+      const int i = act ? (int) ir : 0;
+      const double mi = act ? m[i] : 0.0;
+      const bool cand = mi > 1;
+      const int s = (act && cand) ? S_ia[i] : 0, e = (act && cand) ? S_ia[i + 1] : 0;
+      bool lose = false;
+      for (int kb = s; __any(kb < e); kb += K) {
+         const int k = kb + gl;
+         if (k < e) {
+            const int j = S_ja[k];
+            const double mj = m[j];
+            if (mj > 1) {
+               if (mi > mj) { CF[j] = 0; }
+               else if (mj > mi) { lose = true; }
+            }
          }
       }
+      const unsigned long long lm = (__ballot(lose) >> gshift) & gmask;
+      if (lm && gl == 0) { CF[i] = 0; }
    }
 }
 
-// par_coarsen.c:2603-2660: C points, and F points depending strongly on a C point
+// par_coarsen.c:2603-2660: C points, and F points depending strongly on a C
+// point.  A neighbour's "CF > 0" does not change within this pass (survivors
+// stay 1, everything else is 0 or -1), so the concurrent updates are safe.
+template <int K>
 __global__ void k_pmis_cf(int n, const int *__restrict__ S_ia, const int *__restrict__ S_ja,
                           const double *__restrict__ m, int *CF)
 {
-   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-      double mi = m[i];
-      if (!(mi > 0)) { continue; }   // not in the graph
-      int c = CF[i];
-      if (mi < 1) { c = -1; }
-      if (c > 0) {
-         c = 1;
-      } else {
-         for (int k = S_ia[i]; k < S_ia[i + 1]; k++) {
-            if (CF[S_ja[k]] > 0) { c = -1; break; }
-         }
+   const int gl = threadIdx.x & (K - 1);
+   const int rows_per_block = blockDim.x / K;
+   const int gshift = (threadIdx.x & 63) & ~(K - 1);
+   const unsigned long long gmask = (K == 64) ? ~0ULL : ((1ULL << K) - 1);
+   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < n;
+        rb += (long long) gridDim.x * rows_per_block) {
+      const long long ir = rb + threadIdx.x / K;
+      const bool act = ir < n;
+      const int i = act ? (int) ir : 0;
+      const double mi = act ? m[i] : 0.0;
+      const bool in_graph = act && mi > 0;
+      int c = in_graph ? CF[i] : 0;
+      if (in_graph && mi < 1) { c = -1; }
+      const bool scan = in_graph && !(c > 0);
+      const int s = scan ? S_ia[i] : 0, e = scan ? S_ia[i + 1] : 0;
+      bool hitc = false;
+      for (int kb = s; __any(kb < e); kb += K) {
+         const int k = kb + gl;
+         if (k < e && CF[S_ja[k]] > 0) { hitc = true; }
       }
-      CF[i] = c;
+      const unsigned long long hm = (__ballot(hitc) >> gshift) & gmask;
+      if (in_graph && gl == 0) {
+         if (c > 0) { c = 1; } else if (hm) { c = -1; }
+         CF[i] = c;
+      }
    }
 }
 
@@ -558,10 +602,14 @@ int pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_ke
    int *remaining = device_flags() + 44;
    int *h = static_cast<int *>(pinned_scratch(64));
    int rounds = 0;
+   const int K = lanes_per_row(S.nnz, n);
+   const int nbk = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
    while (true) {
       dzero(remaining, sizeof(int));
-      k_pmis_remove<<<nb, 256, 0, st>>>(n, S.ia, S.ja, m, CF);
-      k_pmis_cf<<<nb, 256, 0, st>>>(n, S.ia, S.ja, m, CF);
+      dispatch_k(K, [&](auto kc) {
+         k_pmis_remove<decltype(kc)::value><<<nbk, 256, 0, st>>>(n, S.ia, S.ja, m, CF);
+         k_pmis_cf<decltype(kc)::value><<<nbk, 256, 0, st>>>(n, S.ia, S.ja, m, CF);
+      });
       k_pmis_update<<<nb, 256, 0, st>>>(n, m, CF, remaining);
       HX_LAUNCH_CHECK();
       rounds++;
@@ -1056,9 +1104,11 @@ struct ExtpiFastArgs
    const int    *OC_ia, *OC_k, *OC_j;   // opposite-sign C entries: entry index, column
    const double *OC_a;                  // and value
    const int    *CF, *f2c;
-   int          *raw_j;       // row i at i * XF_MC
+   const int    *rowlist;     // rows to process (nullptr: 0..n-1)
+   int           nwork;
+   int          *raw_j;       // work item w at w * MC
    double       *raw_a;
-   int          *cnt;         // entries of the raw row, -1 = deferred
+   int          *cnt;         // entries of the raw row of work item w, -1 = deferred
    int          *defer_list;
    int          *ndefer;
 };
@@ -1084,34 +1134,38 @@ __device__ __forceinline__ double shfl_d(double v, int src)
    return __shfl(v, src, WAVE);
 }
 
+template <int MC, int QC>
 __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
 {
    __shared__ int keys[XF_HC];
    __shared__ int vals[XF_HC];
-   __shared__ double wgt[XF_MC];
-   __shared__ int chat_key[XF_MC];
-   __shared__ int chat_col[XF_MC];
-   __shared__ int mins[XF_MC];
+   __shared__ double wgt[MC];
+   __shared__ int chat_key[MC];
+   __shared__ int chat_col[MC];
+   __shared__ int mins[MC];
    __shared__ int seg_ki[WAVE];
-   __shared__ double q_a[XF_QC];
-   __shared__ int q_tgt[XF_QC];
-   __shared__ int q_seg[XF_QC];
+   __shared__ double q_a[QC];
+   __shared__ int q_tgt[QC];
+   __shared__ int q_seg[QC];
    const int lane = threadIdx.x;
    const int nb = gridDim.x;
    const int blk = xcd_blk(blockIdx.x, nb);
    RowHash H{keys, vals, XF_HC, ilog2(XF_HC)};
    // rows strided over the grid: co-resident waves work on one window of
    // neighbouring rows, whose neighbour rows stay in the XCD's L2
-   for (int i = blk; i < p.n; i += nb) {
+   for (int wi = blk; wi < p.nwork; wi += nb) {
+      // raw row and count are indexed by row, or by list position on a row list
+      const int i = p.rowlist ? p.rowlist[wi] : wi;
+      const size_t ro = (size_t) wi * MC;
       const int cfi = p.CF[i];
       if (cfi >= 0 || cfi == -3) {
          if (lane == 0) {
             if (cfi >= 0) {
-               p.raw_j[(size_t) i * XF_MC] = p.f2c[i];
-               p.raw_a[(size_t) i * XF_MC] = 1.0;
-               p.cnt[i] = 1;
+               p.raw_j[ro] = p.f2c[i];
+               p.raw_a[ro] = 1.0;
+               p.cnt[wi] = 1;
             } else {
-               p.cnt[i] = 0;
+               p.cnt[wi] = 0;
             }
          }
          continue;
@@ -1145,7 +1199,7 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
          wave_sync();
          // ---- C-hat in first-seen order.  vals[] of a C key holds the
          // smallest position seen (>= 64 after the first chunk) until the key
-         // receives its C-hat position (< XF_MC).
+         // receives its C-hat position (< MC).
          int nch = 0;
          const int nseg = max(ns, 1);
          for (int cb = 0; cb < L && !defer; cb += WAVE * XF_U) {
@@ -1180,7 +1234,7 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                const bool first = (sl >= 0) && vals[sl] == q;
                const unsigned long long fm = __ballot(first);
                const int nnew = __popcll(fm);
-               if (nch + nnew > XF_MC) {
+               if (nch + nnew > MC) {
                   defer = true;
                } else if (first) {
                   const int pos = nch + __popcll(fm & lanemask_lt(lane));
@@ -1196,7 +1250,7 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                chat_col[lane] = p.f2c[chat_key[lane]];
                wgt[lane] = 0.0;
             }
-            if (lane < XF_MC) { mins[lane] = 0x7fffffff; }
+            if (lane < MC) { mins[lane] = 0x7fffffff; }
             wave_sync();
             // ---- weights (par_lr_interp.c:1640-1805), 64 entries of row i per block
             double diag = p.A_a[a0];
@@ -1287,7 +1341,7 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                }
                const int qincl = wave_incl_scan(nq, lane);
                const int qn = readlane_i(qincl, WAVE - 1);
-               if (qn > XF_QC) { defer = true; break; }
+               if (qn > QC) { defer = true; break; }
                {
                   int pos = qincl - nq;
                   if (sk == 1) {
@@ -1364,15 +1418,15 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                if (lane < nch) {
                   double w = wgt[lane];
                   if (diag != 0.0) { w /= -diag; }
-                  p.raw_j[(size_t) i * XF_MC + lane] = chat_col[lane];
-                  p.raw_a[(size_t) i * XF_MC + lane] = w;
+                  p.raw_j[ro + lane] = chat_col[lane];
+                  p.raw_a[ro + lane] = w;
                }
-               if (lane == 0) { p.cnt[i] = nch; }
+               if (lane == 0) { p.cnt[wi] = nch; }
             }
          }
       }
       if (defer && lane == 0) {
-         p.cnt[i] = -1;
+         p.cnt[wi] = -1;
          const int d = atomicAdd(p.ndefer, 1);
          p.defer_list[d] = i;
       }
@@ -1424,7 +1478,9 @@ constexpr int TR_ROWS = 64;
 constexpr int TR_SS = 65;
 constexpr int TR_STACK = 16;
 
-__global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__restrict__ cnt_raw,
+template <int MC>
+__global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__restrict__ rowlist,
+                                                        const int *__restrict__ cnt_raw,
                                                         const int *__restrict__ raw_j,
                                                         const double *__restrict__ raw_a,
                                                         double trunc_factor, int max_elmts,
@@ -1432,8 +1488,8 @@ __global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__rest
                                                         int *__restrict__ out_j,
                                                         double *__restrict__ out_a)
 {
-   __shared__ double sw[XF_MC * TR_SS];
-   __shared__ int sj[XF_MC * TR_SS];
+   __shared__ double sw[MC * TR_SS];
+   __shared__ int sj[MC * TR_SS];
    __shared__ int stl[TR_STACK * TR_SS], str[TR_STACK * TR_SS];
    __shared__ int sc[TR_ROWS];
    const int t = threadIdx.x;
@@ -1442,11 +1498,11 @@ __global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__rest
       const long long i = base + t;
       sc[t] = (i < n) ? cnt_raw[i] : -1;
       __syncthreads();
-      for (int k = t; k < TR_ROWS * XF_MC; k += TR_ROWS) {
-         const int r = k / XF_MC, e = k % XF_MC;
+      for (int k = t; k < TR_ROWS * MC; k += TR_ROWS) {
+         const int r = k / MC, e = k % MC;
          if (e < sc[r]) {
-            sj[e * TR_SS + r] = raw_j[(size_t) (base + r) * XF_MC + e];
-            sw[e * TR_SS + r] = raw_a[(size_t) (base + r) * XF_MC + e];
+            sj[e * TR_SS + r] = raw_j[(size_t) (base + r) * MC + e];
+            sw[e * TR_SS + r] = raw_a[(size_t) (base + r) * MC + e];
          }
       }
       __syncthreads();
@@ -1495,10 +1551,11 @@ __global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__rest
             }
             kept = max_elmts;
          }
-         out_cnt[i] = kept;
+         const long long row = rowlist ? rowlist[i] : i;
+         out_cnt[row] = kept;
          for (int k = 0; k < kept; k++) {
-            out_j[(size_t) i * out_stride + k] = vj[k * TR_SS];
-            out_a[(size_t) i * out_stride + k] = vw[k * TR_SS];
+            out_j[(size_t) row * out_stride + k] = vj[k * TR_SS];
+            out_a[(size_t) row * out_stride + k] = vw[k * TR_SS];
          }
       }
       __syncthreads();
@@ -1558,39 +1615,60 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
       f.SC_ia = SC.ia; f.SC_ja = SC.ja;
       f.OC_ia = OC.ia; f.OC_k = OC.k; f.OC_j = OC.j; f.OC_a = OC.a;
       f.CF = CF; f.f2c = f2c;
-      int *raw_cnt = dalloc_n<int>((size_t) n + 1);
-      f.raw_j = dalloc_n<int>((size_t) std::max(n, 1) * XF_MC);
-      f.raw_a = dalloc_n<double>((size_t) std::max(n, 1) * XF_MC);
-      f.cnt = raw_cnt;
-      int *dlist = dalloc_n<int>((size_t) n + 1);
-      f.defer_list = dlist;
-      f.ndefer = device_flags() + 60;
-      dzero(f.ndefer, sizeof(int));
-      k_extpi_fast<<<launch_blocks(n), WAVE, 0, stream()>>>(f);
-      HX_LAUNCH_CHECK();
       tmp_j = dalloc_n<int>((size_t) std::max(n, 1) * stride);
       tmp_a = dalloc_n<double>((size_t) std::max(n, 1) * stride);
-      const int nbt = (int) std::max<long long>(1, std::min<long long>((n + TR_ROWS - 1) / TR_ROWS, 16384));
-      k_trunc_rows<<<nbt, TR_ROWS, 0, stream()>>>(n, raw_cnt, f.raw_j, f.raw_a, trunc_factor,
-                                                  max_elmts, stride, cnt, tmp_j, tmp_a);
-      HX_LAUNCH_CHECK();
       int *h = static_cast<int *>(pinned_scratch(64));
-      HX_HIP(hipMemcpyAsync(h, f.ndefer, sizeof(int), hipMemcpyDeviceToHost, stream()));
-      sync();
-      const int nd = h[0];
-      dfree(f.raw_j);
-      dfree(f.raw_a);
-      dfree(raw_cnt);
+      int *lists[2] = {dalloc_n<int>((size_t) n + 1), dalloc_n<int>((size_t) n + 1)};
+      const int *list = nullptr;
+      int nwork = n;
+      // pass 0: C-hat <= 32 points, every row; pass 1: C-hat <= 64, the rest
+      for (int pass = 0; pass < 2 && nwork > 0; pass++) {
+         const int mc = pass == 0 ? XF_MC : 2 * XF_MC;
+         int *raw_cnt = dalloc_n<int>((size_t) nwork + 1);
+         f.rowlist = list;
+         f.nwork = nwork;
+         f.raw_j = dalloc_n<int>((size_t) nwork * mc);
+         f.raw_a = dalloc_n<double>((size_t) nwork * mc);
+         f.cnt = raw_cnt;
+         f.defer_list = lists[pass];
+         f.ndefer = device_flags() + 60;
+         dzero(f.ndefer, sizeof(int));
+         const int nbt = (int) std::max<long long>(1, std::min<long long>((nwork + TR_ROWS - 1) / TR_ROWS, 16384));
+         if (pass == 0) {
+            k_extpi_fast<XF_MC, XF_QC><<<launch_blocks(nwork), WAVE, 0, stream()>>>(f);
+            k_trunc_rows<XF_MC><<<nbt, TR_ROWS, 0, stream()>>>(nwork, list, raw_cnt, f.raw_j, f.raw_a,
+                                                               trunc_factor, max_elmts, stride, cnt,
+                                                               tmp_j, tmp_a);
+         } else {
+            k_extpi_fast<2 * XF_MC, 512><<<launch_blocks(nwork), WAVE, 0, stream()>>>(f);
+            k_trunc_rows<2 * XF_MC><<<nbt, TR_ROWS, 0, stream()>>>(nwork, list, raw_cnt, f.raw_j,
+                                                                   f.raw_a, trunc_factor, max_elmts,
+                                                                   stride, cnt, tmp_j, tmp_a);
+         }
+         HX_LAUNCH_CHECK();
+         HX_HIP(hipMemcpyAsync(h, f.ndefer, sizeof(int), hipMemcpyDeviceToHost, stream()));
+         sync();
+         if (setup_verbose()) {
+            std::fprintf(stderr, "[hypre_mi355x] ext+i %d rows, pass %d (C-hat <= %d): %d left\n", n,
+                         pass, mc, h[0]);
+         }
+         dfree(f.raw_j);
+         dfree(f.raw_a);
+         dfree(raw_cnt);
+         list = lists[pass];
+         nwork = h[0];
+      }
       SC.free_all();
       OC.free_all();
-      if (nd > 0) {
+      if (nwork > 0) {
          a.base = nullptr;
          a.out_j = tmp_j;
          a.out_a = tmp_a;
          a.cnt = cnt;
-         extpi_pass<MODE_NUM>(a, dlist, nd);
+         extpi_pass<MODE_NUM>(a, list, nwork);
       }
-      dfree(dlist);
+      dfree(lists[0]);
+      dfree(lists[1]);
    } else {
       if (max_elmts > 0) {
          stride = max_elmts;
@@ -1891,10 +1969,9 @@ static void rap_pass(RapArgs a, const int *init_list = nullptr, int init_n = 0)
 //     of every column (atomicMin on the slot) applies its product, so each
 //     column receives its products in traversal order, and columns created in
 //     a step get positions in the order of their first product.
-// Output rows go to a stride-RF_ACL buffer; rows whose RA row exceeds `rcl`
-// entries or whose Ac row exceeds RF_ACL entries are deferred.
-constexpr int RF_AC = 256;
-constexpr int RF_ACL = 128;
+// Output rows go to a buffer with a stride of `acl` entries; rows whose RA row
+// exceeds `rcl` entries or whose Ac row exceeds `acl` entries are deferred to
+// the next stage (larger tables) and finally to k_rap.
 constexpr int RF_U = 3;      // chunks of 64 products per batch of loads
 
 struct RapFastArgs
@@ -1908,18 +1985,22 @@ struct RapFastArgs
    const double *P_a;
    const int    *rowlist;
    int           nwork;
+   int           stage;
    int           rc;          // RA hash slots (power of two)
    int           rcl;         // RA entries (<= rc - 64)
-   int          *tmp_j;       // row ic at ic * RF_ACL
+   int           ac;          // Ac hash slots (power of two)
+   int           acl;         // Ac entries (<= ac - 64), the output stride
+   int          *tmp_j;       // row at (stage 0: ic, later: worklist index) * acl
    double       *tmp_a;
-   int          *cnt;         // -1 = deferred
+   int          *cnt;         // entries of a finished row
+   int          *src;         // stage 0: offset >= 0; later stages: -2 - offset; -1 deferred
    int          *defer_list;
    int          *ndefer;
 };
 
-static size_t rap_fast_lds(int rc, int rcl)
+static size_t rap_fast_lds(int rc, int rcl, int ac, int acl)
 {
-   return (size_t) rc * 12 + (size_t) rcl * 4 + (size_t) RF_AC * 20 + RF_ACL * 4;
+   return (size_t) rc * 12 + (size_t) rcl * 4 + (size_t) ac * 20 + (size_t) acl * 4;
 }
 
 __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
@@ -1929,22 +2010,22 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
    const int nb = gridDim.x;
    const int blk = xcd_blk(blockIdx.x, nb);
    double *ra_val = dyn_lds;                    // rc
-   double *ac_val = ra_val + p.rc;              // RF_AC
-   int *ra_keys = reinterpret_cast<int *>(ac_val + RF_AC);
+   double *ac_val = ra_val + p.rc;              // ac
+   int *ra_keys = reinterpret_cast<int *>(ac_val + p.ac);
    int *ra_order = ra_keys + p.rc;              // rcl
-   int *ac_keys = ra_order + p.rcl;             // RF_AC
-   int *ac_pos = ac_keys + RF_AC;
-   int *ac_mins = ac_pos + RF_AC;
-   int *ac_order = ac_mins + RF_AC;             // RF_ACL
+   int *ac_keys = ra_order + p.rcl;             // ac
+   int *ac_pos = ac_keys + p.ac;
+   int *ac_mins = ac_pos + p.ac;
+   int *ac_order = ac_mins + p.ac;              // acl
    RowHash HR{ra_keys, nullptr, p.rc, ilog2(p.rc)};
-   RowHash HC{ac_keys, nullptr, RF_AC, ilog2(RF_AC)};
+   RowHash HC{ac_keys, nullptr, p.ac, ilog2(p.ac)};
 
    // rows strided over the grid: co-resident waves work on one window of
    // neighbouring rows, whose neighbour rows stay in the XCD's L2
    for (int wi = blk; wi < p.nwork; wi += nb) {
       const int ic = p.rowlist ? p.rowlist[wi] : wi;
       for (int s = lane; s < p.rc; s += WAVE) { ra_keys[s] = -1; }
-      for (int s = lane; s < RF_AC; s += WAVE) {
+      for (int s = lane; s < p.ac; s += WAVE) {
          ac_keys[s] = -1;
          ac_pos[s] = -1;
          ac_mins[s] = 0x7fffffff;
@@ -2067,7 +2148,7 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
                   bool isn = false;
                   int sl = -1;
                   if (valid) { sl = HC.insert(key[u], isn); }
-                  if (nac + __popcll(__ballot(isn)) > RF_ACL) { defer = true; break; }
+                  if (nac + __popcll(__ballot(isn)) > p.acl) { defer = true; break; }
                   bool pending = valid && sl >= 0;
                   while (__ballot(pending)) {
                      if (pending) { atomicMin(&ac_mins[sl], lane); }
@@ -2095,14 +2176,18 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
          }
       }
       if (!defer) {
+         const size_t o = (size_t) (p.rowlist ? wi : ic) * p.acl;
          for (int k = lane; k < nac; k += WAVE) {
             const int sl = ac_order[k];
-            p.tmp_j[(size_t) ic * RF_ACL + k] = ac_keys[sl];
-            p.tmp_a[(size_t) ic * RF_ACL + k] = ac_val[sl];
+            p.tmp_j[o + k] = ac_keys[sl];
+            p.tmp_a[o + k] = ac_val[sl];
          }
-         if (lane == 0) { p.cnt[ic] = nac; }
+         if (lane == 0) {
+            p.cnt[ic] = nac;
+            p.src[ic] = (p.stage == 0) ? (int) o : -2 - (int) o;
+         }
       } else if (lane == 0) {
-         p.cnt[ic] = -1;
+         p.src[ic] = -1;
          const int d = atomicAdd(p.ndefer, 1);
          p.defer_list[d] = ic;
       }
@@ -2110,28 +2195,27 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
    }
 }
 
-__global__ void k_rap_compact(int nc, const int *__restrict__ cnt, const int *__restrict__ out_ia,
-                              const int *__restrict__ tmp_j, const double *__restrict__ tmp_a,
-                              int *__restrict__ oj, double *__restrict__ oa)
+// copy the rows finished by the fast stages into Ac (16 lanes per row)
+__global__ void k_rap_compact(int nc, const int *__restrict__ cnt, const int *__restrict__ src,
+                              const int *__restrict__ out_ia, const int *__restrict__ tj0,
+                              const double *__restrict__ ta0, const int *__restrict__ tj1,
+                              const double *__restrict__ ta1, int *__restrict__ oj,
+                              double *__restrict__ oa)
 {
-   // 16 lanes per row
    const int g = threadIdx.x & 15;
    for (long long r = ((long long) blockIdx.x * blockDim.x + threadIdx.x) >> 4; r < nc;
         r += ((long long) gridDim.x * blockDim.x) >> 4) {
+      const int s = src[r];
+      if (s == -1) { continue; }
       const int c = cnt[r];
-      if (c < 0) { continue; }
       const int o = out_ia[r];
+      const int *tj = s >= 0 ? tj0 : tj1;
+      const double *ta = s >= 0 ? ta0 : ta1;
+      const size_t b = s >= 0 ? (size_t) s : (size_t) (-2 - s);
       for (int k = g; k < c; k += 16) {
-         oj[o + k] = tmp_j[(size_t) r * RF_ACL + k];
-         oa[o + k] = tmp_a[(size_t) r * RF_ACL + k];
+         oj[o + k] = tj[b + k];
+         oa[o + k] = ta[b + k];
       }
-   }
-}
-
-__global__ void k_clamp0(int n, const int *__restrict__ in, int *__restrict__ out)
-{
-   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-      out[i] = max(in[i], 0);
    }
 }
 
@@ -2149,47 +2233,57 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
    Ac.a32 = nullptr;
    Ac.ia = dalloc_n<int>((size_t) nc + 1);
 
-   // fast path: RA tables of 512, 1024, 2048 slots
+   // fast path, stage 0: RA table 512 slots, Ac table 256 (rows of <= 128
+   // entries, output indexed by row); stage 1: RA 2048, Ac 1024 (<= 640
+   // entries, output indexed by position in the stage's row list)
    RapFastArgs f{};
    f.nc = nc;
    f.R_ia = R.ia; f.R_ja = R.ja; f.R_a = R.a;
    f.A_ia = A.ia; f.A_ja = A.ja; f.A_a = A.a;
    f.P_ia = P.ia; f.P_ja = P.ja; f.P_a = P.a;
-   int *fcnt = dalloc_n<int>((size_t) nc + 1);
-   int *tmp_j = dalloc_n<int>((size_t) std::max(nc, 1) * RF_ACL);
-   double *tmp_a = dalloc_n<double>((size_t) std::max(nc, 1) * RF_ACL);
+   static const int rcs[2] = {512, 2048}, acs[2] = {256, 1024}, acls[2] = {128, 640};
+   int *cnt = dalloc_n<int>((size_t) nc + 1);
+   int *src = dalloc_n<int>((size_t) nc + 1);
+   dzero(cnt, sizeof(int) * ((size_t) nc + 1));
+   int *tj[2] = {nullptr, nullptr};
+   double *ta[2] = {nullptr, nullptr};
    int *lists[2] = {dalloc_n<int>((size_t) nc + 1), dalloc_n<int>((size_t) nc + 1)};
    int *ndefer = device_flags() + 60;
    int *h = static_cast<int *>(pinned_scratch(64));
-   f.tmp_j = tmp_j;
-   f.tmp_a = tmp_a;
-   f.cnt = fcnt;
+   f.cnt = cnt;
+   f.src = src;
    f.ndefer = ndefer;
    const int *list = nullptr;
    int nwork = nc;
    int cur = 0;
-   static const int rcs[3] = {512, 1024, 2048};
-   for (int stage = 0; stage < 3 && nwork > 0; stage++) {
+   for (int stage = 0; stage < 2 && nwork > 0; stage++) {
+      tj[stage] = dalloc_n<int>((size_t) nwork * acls[stage]);
+      ta[stage] = dalloc_n<double>((size_t) nwork * acls[stage]);
       f.rowlist = list;
       f.nwork = nwork;
+      f.stage = stage;
       f.rc = rcs[stage];
       f.rcl = rcs[stage] * 5 / 8;
+      f.ac = acs[stage];
+      f.acl = acls[stage];
+      f.tmp_j = tj[stage];
+      f.tmp_a = ta[stage];
       f.defer_list = lists[cur];
       dzero(ndefer, sizeof(int));
-      const size_t lds = rap_fast_lds(f.rc, f.rcl);
+      const size_t lds = rap_fast_lds(f.rc, f.rcl, f.ac, f.acl);
       k_rap_fast<<<launch_blocks(nwork), WAVE, lds, stream()>>>(f);
       HX_LAUNCH_CHECK();
       HX_HIP(hipMemcpyAsync(h, ndefer, sizeof(int), hipMemcpyDeviceToHost, stream()));
       sync();
       list = lists[cur];
+      if (setup_verbose()) {
+         std::fprintf(stderr, "[hypre_mi355x] RAP %d coarse rows: stage %d (RA %d, Ac %d) left %d\n",
+                      nc, stage, f.rc, f.ac, h[0]);
+      }
       nwork = h[0];
       cur ^= 1;
    }
    // rows beyond the fast path: counts by the general kernel
-   int *cnt = dalloc_n<int>((size_t) nc + 1);
-   int nb = std::max(1, std::min((nc + 255) / 256, 8192));
-   k_clamp0<<<nb, 256, 0, stream()>>>(nc, fcnt, cnt);
-   HX_LAUNCH_CHECK();
    if (nwork > 0) {
       a.cnt = cnt;
       rap_pass<MODE_SYM>(a, list, nwork);
@@ -2198,7 +2292,8 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
    Ac.ja = dalloc_n<int>((size_t) std::max<long long>(Ac.nnz, 1));
    Ac.a = dalloc_n<double>((size_t) std::max<long long>(Ac.nnz, 1));
    int nbc = (int) std::max<long long>(1, std::min<long long>(((long long) nc * 16 + 255) / 256, 16384));
-   k_rap_compact<<<nbc, 256, 0, stream()>>>(nc, fcnt, Ac.ia, tmp_j, tmp_a, Ac.ja, Ac.a);
+   k_rap_compact<<<nbc, 256, 0, stream()>>>(nc, cnt, src, Ac.ia, tj[0], ta[0], tj[1], ta[1], Ac.ja,
+                                            Ac.a);
    HX_LAUNCH_CHECK();
    if (nwork > 0) {
       a.out_ia = Ac.ia;
@@ -2208,9 +2303,11 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
    }
    dfree(lists[0]);
    dfree(lists[1]);
-   dfree(tmp_j);
-   dfree(tmp_a);
-   dfree(fcnt);
+   for (int s = 0; s < 2; s++) {
+      dfree(tj[s]);
+      dfree(ta[s]);
+   }
+   dfree(src);
    dfree(cnt);
 }
 
