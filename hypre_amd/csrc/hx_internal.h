@@ -71,7 +71,7 @@ void       *dalloc(size_t bytes);         // caching device allocator
 void        dfree(void *p);
 void        release_cache();
 void       *pinned_scratch(size_t bytes); // host-pinned staging for scalars
-int        *device_flags();               // 256 ints of device status words
+int        *device_flags();               // 1024 ints of device status words (512..767: row claim counters)
 
 template <class T>
 inline T *dalloc_n(size_t n) { return static_cast<T *>(dalloc(n * sizeof(T) + 16)); }

@@ -35,8 +35,8 @@ void ensure_init()
    HX_HIP(hipGetDeviceCount(&ndev));
    if (ndev < 1) { fail(HYPRE_ERROR_GENERIC, "no HIP device"); }
    HX_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-   HX_HIP(hipMalloc(&g_flags, 256 * sizeof(int)));
-   HX_HIP(hipMemset(g_flags, 0, 256 * sizeof(int)));
+   HX_HIP(hipMalloc(&g_flags, 1024 * sizeof(int)));
+   HX_HIP(hipMemset(g_flags, 0, 1024 * sizeof(int)));
    g_inited = true;
 }
 

@@ -77,6 +77,56 @@ __device__ __forceinline__ int xcd_blk(int bid, int nb)
    return (nb & 7) ? bid : (bid & 7) * (nb >> 3) + (bid >> 3);
 }
 
+// XCD (accelerator complex die) the wave runs on, 0..7
+__device__ __forceinline__ int xcc_id()
+{
+   unsigned v;
+   asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+   return (int) (v & 7);
+}
+
+// Work items handed out in order from eight counters, one per XCD: XCD x
+// owns items [x*chunk, (x+1)*chunk) and its waves claim them one at a time,
+// so the rows in flight on an XCD form a narrow window of consecutive rows
+// (their neighbour rows stay in that XCD's L2) however unevenly rows cost.
+// A wave whose range is used up claims from the next XCD's range.  Every wave
+// ends after at most eight failed claims.  ctr: 8 counters 32 ints apart,
+// zero at launch.  Called by all lanes of the wave together.
+struct RowClaim
+{
+   int *ctr;
+   int  n, chunk, x, t, v;
+
+   __device__ RowClaim(int *c, int nwork) : ctr(c), n(nwork), chunk((nwork + 7) >> 3), x(xcc_id()), t(0), v(0) {}
+
+   // start claiming the next item; the result is read by take()
+   __device__ void issue(int lane)
+   {
+      v = 0;
+      if (lane == 0) { v = atomicAdd(ctr + ((x + t) & 7) * 32, 1); }
+   }
+
+   // the claimed item, or -1 once every range is used up
+   __device__ int take(int lane)
+   {
+      for (;;) {
+         const int cx = (x + t) & 7;
+         const int lo = cx * chunk;
+         const int got = readlane_i(v, 0);
+         if (got < min(chunk, n - lo)) { return lo + got; }
+         if (++t >= 8) { return -1; }
+         issue(lane);
+      }
+   }
+};
+
+static int *claim_counters()
+{
+   int *c = device_flags() + 512;
+   dzero(c, 256 * sizeof(int));
+   return c;
+}
+
 // inclusive prefix sum over the 64 lanes of a wave
 __device__ __forceinline__ int wave_incl_scan(int v, int lane)
 {
@@ -1092,7 +1142,8 @@ static void extpi_pass(InterpArgs a, const int *init_list = nullptr, int init_n 
 constexpr int XF_HC = 256;   // hash slots (<= 64 F markers + <= XF_MC + 64 C keys)
 constexpr int XF_MC = 32;    // C-hat points per row on the fast path
 constexpr int XF_U = 3;      // chunks of 64 entries per batch of loads
-constexpr int XF_QC = 192;   // C-hat terms of 64 entries of row i kept for pass B
+constexpr int XF_QC = 176;   // C-hat terms of 64 entries of row i kept for pass B
+// (LDS of the fast kernel stays under 5 KB: 32 waves per CU)
 
 struct ExtpiFastArgs
 {
@@ -1111,6 +1162,7 @@ struct ExtpiFastArgs
    int          *cnt;         // entries of the raw row of work item w, -1 = deferred
    int          *defer_list;
    int          *ndefer;
+   int          *claim;       // RowClaim counters
 };
 
 // last segment s in [0, nseg) with off_s <= q; segment s's offset is held by
@@ -1135,7 +1187,7 @@ __device__ __forceinline__ double shfl_d(double v, int src)
 }
 
 template <int MC, int QC>
-__global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void k_extpi_fast(ExtpiFastArgs p)
 {
    __shared__ int keys[XF_HC];
    __shared__ int vals[XF_HC];
@@ -1145,15 +1197,13 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
    __shared__ int mins[MC];
    __shared__ int seg_ki[WAVE];
    __shared__ double q_a[QC];
-   __shared__ int q_tgt[QC];
-   __shared__ int q_seg[QC];
+   __shared__ int q_ts[QC];   // weight index << 9 | 0x100 (C-hat entry) | segment lane
    const int lane = threadIdx.x;
-   const int nb = gridDim.x;
-   const int blk = xcd_blk(blockIdx.x, nb);
    RowHash H{keys, vals, XF_HC, ilog2(XF_HC)};
-   // rows strided over the grid: co-resident waves work on one window of
-   // neighbouring rows, whose neighbour rows stay in the XCD's L2
-   for (int wi = blk; wi < p.nwork; wi += nb) {
+   RowClaim rc(p.claim, p.nwork);
+   rc.issue(lane);
+   for (int wi = rc.take(lane); wi >= 0; wi = rc.take(lane)) {
+      rc.issue(lane);   // the next row's claim is in flight while this row runs
       // raw row and count are indexed by row, or by list position on a row list
       const int i = p.rowlist ? p.rowlist[wi] : wi;
       const size_t ro = (size_t) wi * MC;
@@ -1345,16 +1395,14 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                {
                   int pos = qincl - nq;
                   if (sk == 1) {
-                     q_tgt[pos] = v;
-                     q_seg[pos] = lane | 0x100;
+                     q_ts[pos] = (v << 9) | 0x100 | lane;
                      q_a[pos] = aij;
                   } else if (sk == 2) {
                      for (int e = ob; e < ob + olen; e++) {
                         const int s2 = H.find(p.OC_j[e]);
                         const int v2 = (s2 >= 0) ? vals[s2] : -1;
                         if (v2 >= 0) {
-                           q_tgt[pos] = v2;
-                           q_seg[pos] = lane;
+                           q_ts[pos] = (v2 << 9) | lane;
                            q_a[pos] = p.OC_a[e];
                            pos++;
                         }
@@ -1391,8 +1439,9 @@ __global__ __launch_bounds__(WAVE) void k_extpi_fast(ExtpiFastArgs p)
                for (int qb = 0; qb < qn; qb += WAVE) {
                   const int e = qb + lane;
                   const bool valid = e < qn;
-                  const int tg = valid ? q_tgt[e] : 0;
-                  const int sgw = valid ? q_seg[e] : 0;
+                  const int ts = valid ? q_ts[e] : 0;
+                  const int tg = ts >> 9;
+                  const int sgw = ts & 0x1ff;
                   const int sg = sgw & 0xff;
                   const bool dr = (sgw & 0x100) != 0;
                   const double a = valid ? q_a[e] : 0.0;
@@ -1632,6 +1681,7 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
          f.cnt = raw_cnt;
          f.defer_list = lists[pass];
          f.ndefer = device_flags() + 60;
+         f.claim = claim_counters();
          dzero(f.ndefer, sizeof(int));
          const int nbt = (int) std::max<long long>(1, std::min<long long>((nwork + TR_ROWS - 1) / TR_ROWS, 16384));
          if (pass == 0) {
@@ -1973,6 +2023,7 @@ static void rap_pass(RapArgs a, const int *init_list = nullptr, int init_n = 0)
 // exceeds `rcl` entries or whose Ac row exceeds `acl` entries are deferred to
 // the next stage (larger tables) and finally to k_rap.
 constexpr int RF_U = 3;      // chunks of 64 products per batch of loads
+constexpr int RAP_RAREG = 5; // stage 0: RA row held in 5 registers per lane
 
 struct RapFastArgs
 {
@@ -1996,39 +2047,49 @@ struct RapFastArgs
    int          *src;         // stage 0: offset >= 0; later stages: -2 - offset; -1 deferred
    int          *defer_list;
    int          *ndefer;
+   int          *claim;       // RowClaim counters
 };
 
-static size_t rap_fast_lds(int rc, int rcl, int ac, int acl)
+// LDS of one wave.  RAREG > 0: the finished RA row (<= 64 * RAREG entries) is
+// moved to registers before the Ac phase, whose tables then reuse the RA
+// tables' space.
+static size_t rap_fast_lds(int rc, int rcl, int ac, int acl, int rareg)
 {
-   return (size_t) rc * 12 + (size_t) rcl * 4 + (size_t) ac * 20 + (size_t) acl * 4;
+   const size_t ra = (size_t) rc * 12 + (size_t) rcl * 4;
+   const size_t acb = (size_t) ac * 20 + (size_t) acl * 4;
+   return rareg > 0 ? std::max(ra, acb) : ra + acb;
 }
 
+template <int RAREG>
 __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
 {
    extern __shared__ double dyn_lds[];
    const int lane = threadIdx.x;
-   const int nb = gridDim.x;
-   const int blk = xcd_blk(blockIdx.x, nb);
    double *ra_val = dyn_lds;                    // rc
-   double *ac_val = ra_val + p.rc;              // ac
-   int *ra_keys = reinterpret_cast<int *>(ac_val + p.ac);
+   int *ra_keys = reinterpret_cast<int *>(ra_val + p.rc);
    int *ra_order = ra_keys + p.rc;              // rcl
-   int *ac_keys = ra_order + p.rcl;             // ac
+   double *ac_val = RAREG > 0 ? dyn_lds : reinterpret_cast<double *>(ra_order + p.rcl);   // ac
+   int *ac_keys = reinterpret_cast<int *>(ac_val + p.ac);
    int *ac_pos = ac_keys + p.ac;
    int *ac_mins = ac_pos + p.ac;
    int *ac_order = ac_mins + p.ac;              // acl
    RowHash HR{ra_keys, nullptr, p.rc, ilog2(p.rc)};
    RowHash HC{ac_keys, nullptr, p.ac, ilog2(p.ac)};
+   int rk[RAREG > 0 ? RAREG : 1];
+   double rx[RAREG > 0 ? RAREG : 1];
 
-   // rows strided over the grid: co-resident waves work on one window of
-   // neighbouring rows, whose neighbour rows stay in the XCD's L2
-   for (int wi = blk; wi < p.nwork; wi += nb) {
+   RowClaim claim(p.claim, p.nwork);
+   claim.issue(lane);
+   for (int wi = claim.take(lane); wi >= 0; wi = claim.take(lane)) {
+      claim.issue(lane);   // the next row's claim is in flight while this row runs
       const int ic = p.rowlist ? p.rowlist[wi] : wi;
       for (int s = lane; s < p.rc; s += WAVE) { ra_keys[s] = -1; }
-      for (int s = lane; s < p.ac; s += WAVE) {
-         ac_keys[s] = -1;
-         ac_pos[s] = -1;
-         ac_mins[s] = 0x7fffffff;
+      if (RAREG == 0) {
+         for (int s = lane; s < p.ac; s += WAVE) {
+            ac_keys[s] = -1;
+            ac_pos[s] = -1;
+            ac_mins[s] = 0x7fffffff;
+         }
       }
       wave_sync();
       bool defer = false;
@@ -2098,6 +2159,28 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
       }
       // ---- (RA) x P, diagonal first (par_rap.c:1612-1620, 1860-1920)
       int nac = 0;
+      if (RAREG > 0 && !defer) {
+         // RA entry k to lane k % 64, register k / 64; then the Ac tables
+         // take over the space
+#pragma unroll
+         for (int t = 0; t < RAREG; t++) {
+            const int k = t * WAVE + lane;
+            rk[t] = 0;
+            rx[t] = 0.0;
+            if (k < nra) {
+               const int sl = ra_order[k];
+               rk[t] = ra_keys[sl];
+               rx[t] = ra_val[sl];
+            }
+         }
+         wave_sync();
+         for (int s = lane; s < p.ac; s += WAVE) {
+            ac_keys[s] = -1;
+            ac_pos[s] = -1;
+            ac_mins[s] = 0x7fffffff;
+         }
+         wave_sync();
+      }
       if (!defer) {
          if (lane == 0) {
             bool isn;
@@ -2113,9 +2196,20 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
             int len = 0, pb = 0;
             double rav = 0.0;
             if (k < nra) {
-               const int sl = ra_order[k];
-               const int i1 = ra_keys[sl];
-               rav = ra_val[sl];
+               int i1;
+               if (RAREG > 0) {
+                  // register kb / 64, selected without dynamic indexing
+                  i1 = rk[0];
+                  rav = rx[0];
+#pragma unroll
+                  for (int t = 1; t < RAREG; t++) {
+                     if (kb == t * WAVE) { i1 = rk[t]; rav = rx[t]; }
+                  }
+               } else {
+                  const int sl = ra_order[k];
+                  i1 = ra_keys[sl];
+                  rav = ra_val[sl];
+               }
                pb = p.P_ia[i1];
                len = p.P_ia[i1 + 1] - pb;
             }
@@ -2270,8 +2364,16 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
       f.tmp_a = ta[stage];
       f.defer_list = lists[cur];
       dzero(ndefer, sizeof(int));
-      const size_t lds = rap_fast_lds(f.rc, f.rcl, f.ac, f.acl);
-      k_rap_fast<<<launch_blocks(nwork), WAVE, lds, stream()>>>(f);
+      f.claim = claim_counters();
+      if (stage == 0) {
+         // RA rows of <= 320 entries: held in 5 registers per lane for the Ac phase
+         f.rcl = RAP_RAREG * WAVE;
+         const size_t lds = rap_fast_lds(f.rc, f.rcl, f.ac, f.acl, RAP_RAREG);
+         k_rap_fast<RAP_RAREG><<<launch_blocks(nwork), WAVE, lds, stream()>>>(f);
+      } else {
+         const size_t lds = rap_fast_lds(f.rc, f.rcl, f.ac, f.acl, 0);
+         k_rap_fast<0><<<launch_blocks(nwork), WAVE, lds, stream()>>>(f);
+      }
       HX_LAUNCH_CHECK();
       HX_HIP(hipMemcpyAsync(h, ndefer, sizeof(int), hipMemcpyDeviceToHost, stream()));
       sync();
