@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <climits>
 
 namespace hx {
 
@@ -1483,6 +1484,338 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
    }
 }
 
+// ------------------------------------------------------------------ ext+i, rows of <= 64 entries
+// The common case (<= 64 strong neighbours, <= 64 off-diagonal entries, the
+// weight terms in <= XR_OL positions) with the arithmetic of k_extpi_fast but
+// about seven dependent rounds of global loads per row instead of ~20:
+//   T1 row offsets and C/F state; T2 strong neighbours (lane s <-> S entry s)
+//   and off-diagonal entries (lane j <-> entry j); T3 everything indexed by
+//   those columns, read whether or not it is used; T4 one flattened sequence
+//   of the strong-C candidates (C-hat, first-seen order) followed by the rows
+//   of the strong F neighbours (the search for column i, "pass I"); T5 the
+//   terms: one per C-hat entry of row i, and the opposite-sign C entries of
+//   the row of every strong F neighbour, in entry order.
+// The denominators fold each row's terms left to right out of LDS, with the
+// entry of column i merged in by storage index; the weights receive the terms
+// straight from registers, 64 per step, lowest lane first on a shared weight.
+// Rows outside these limits are deferred to k_extpi_fast<64, ...>.
+constexpr int XR_U = 4;      // chunks of 64 per batch of the candidate / pass I sequence
+constexpr int XR_OL = 192;   // term positions of a row (3 chunks)
+
+template <int MC>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void k_extpi_row64(ExtpiFastArgs p)
+{
+   __shared__ int keys[XF_HC];
+   __shared__ int vals[XF_HC];   // C key: stamp, then C-hat position; F neighbour: -2 - S index
+   __shared__ double wgt[MC];
+   __shared__ int chat_key[MC];
+   __shared__ int mins[MC];
+   __shared__ int seg_ki[WAVE];     // S entry s: index of column i in row S_ja[s], or -1
+   __shared__ double st_a[XR_OL];   // term values
+   __shared__ int st_k[XR_OL];      // term storage index if its column is in C-hat, else -1
+   const int lane = threadIdx.x;
+   RowHash H{keys, vals, XF_HC, ilog2(XF_HC)};
+   RowClaim rc(p.claim, p.nwork);
+   rc.issue(lane);
+   for (int wi = rc.take(lane); wi >= 0; wi = rc.take(lane)) {
+      rc.issue(lane);   // the next row's claim is in flight while this row runs
+      const int i = p.rowlist ? p.rowlist[wi] : wi;
+      const size_t ro = (size_t) wi * MC;
+      // T1 (the offsets are read before the C/F test)
+      const int cfi = p.CF[i];
+      const int s0 = p.S_ia[i], s1 = p.S_ia[i + 1];
+      const int a0 = p.A_ia[i], a1 = p.A_ia[i + 1];
+      if (cfi >= 0 || cfi == -3) {
+         if (lane == 0) {
+            if (cfi >= 0) {
+               p.raw_j[ro] = p.f2c[i];
+               p.raw_a[ro] = 1.0;
+               p.cnt[wi] = 1;
+            } else {
+               p.cnt[wi] = 0;
+            }
+         }
+         continue;
+      }
+      const int ns = s1 - s0, na = a1 - a0 - 1;
+      bool defer = ns > WAVE || na > WAVE;
+      if (!defer) {
+         // T2
+         int i1 = -1, c1 = -1;
+         double aij = 0.0;
+         if (lane < ns) { i1 = p.S_ja[s0 + lane]; }
+         if (lane < na) {
+            c1 = p.A_ja[a0 + 1 + lane];
+            aij = p.A_a[a0 + 1 + lane];
+         }
+         double diag = p.A_a[a0];
+         for (int s = lane; s < XF_HC; s += WAVE) { keys[s] = -1; vals[s] = 0x7fffffff; }
+         seg_ki[lane] = -1;
+         // T3
+         int cf_i1 = 0, scb = 0, sce = 0, rb = 0, re = 0;
+         if (lane < ns) {
+            cf_i1 = p.CF[i1];
+            scb = p.SC_ia[i1];
+            sce = p.SC_ia[i1 + 1];
+            rb = p.A_ia[i1];
+            re = p.A_ia[i1 + 1];
+         }
+         int cf_c1 = 0, cb1 = 0, ce1 = 0, ob = 0, oe = 0;
+         if (lane < na) {
+            cf_c1 = p.CF[c1];
+            cb1 = p.A_ia[c1];
+            ce1 = p.A_ia[c1 + 1];
+            ob = p.OC_ia[c1];
+            oe = p.OC_ia[c1 + 1];
+         }
+         // strong neighbours: kind 1 = C, 2 = F (par_lr_interp.c:1505-1560); a
+         // strong F neighbour contributes its strong C neighbours as candidates
+         // and its row (without the diagonal) to pass I
+         int kind = 0, kb = 0, klen = 0, ib = 0, ilen = 0;
+         if (lane < ns) {
+            if (cf_i1 >= 0) {
+               kind = 1; klen = 1; kb = i1;
+            } else if (cf_i1 != -3) {
+               kind = 2; kb = scb; klen = sce - scb; ib = rb + 1; ilen = re - rb - 1;
+            }
+         }
+         wave_sync();
+         if (kind == 2) {
+            bool isn;
+            const int sl = H.insert(i1, isn);
+            if (sl >= 0) { vals[sl] = -2 - lane; }
+         }
+         const int incl = wave_incl_scan(klen, lane);
+         const int L = readlane_i(incl, WAVE - 1);
+         const int off = incl - klen;
+         const int iincl = wave_incl_scan(ilen, lane);
+         const int IL = readlane_i(iincl, WAVE - 1);
+         const int ioff = iincl - ilen;
+         const int nseg = max(ns, 1);
+         // the diagonal of each neighbour row (its sign), read with the first batch
+         double dg1 = 0.0;
+         if (lane < na && ce1 > cb1) { dg1 = p.A_a[cb1]; }
+         wave_sync();
+         // T4: candidates at [0, L), pass I at [L, L + IL)
+         int nch = 0;
+         for (int cb = 0; cb < L + IL && !defer; cb += WAVE * XR_U) {
+            int key[XR_U], hs[XR_U], hk[XR_U];
+#pragma unroll
+            for (int u = 0; u < XR_U; u++) {
+               const int c0 = cb + u * WAVE;   // uniform: the chunk's first position
+               const int q = c0 + lane;
+               const int qi = q - L;
+               key[u] = -1;
+               hs[u] = -1;
+               hk[u] = 0;
+               if (c0 < L) {
+                  const int sa = seg_find_lanes(off, nseg, q);
+                  const int kind_s = __shfl(kind, sa, WAVE);
+                  const int b_s = __shfl(kb, sa, WAVE);
+                  const int o_s = __shfl(off, sa, WAVE);
+                  if (q < L) {
+                     if (kind_s == 1) {
+                        key[u] = b_s;
+                     } else if (kind_s == 2) {
+                        key[u] = p.SC_ja[b_s + (q - o_s)];
+                     }
+                  }
+               }
+               if (c0 + WAVE > L && c0 < L + IL) {
+                  const int si = seg_find_lanes(ioff, nseg, qi);
+                  const int ib_s = __shfl(ib, si, WAVE);
+                  const int io_s = __shfl(ioff, si, WAVE);
+                  if (qi >= 0 && qi < IL) {
+                     hk[u] = ib_s + (qi - io_s);
+                     hs[u] = (p.A_ja[hk[u]] == i) ? si : -1;
+                  }
+               }
+            }
+#pragma unroll
+            for (int u = 0; u < XR_U; u++) {
+               if (hs[u] >= 0) { seg_ki[hs[u]] = hk[u]; }   // one hit per row at most
+            }
+#pragma unroll
+            for (int u = 0; u < XR_U; u++) {
+               if (cb + u * WAVE >= L || defer) { break; }
+               const int q = cb + u * WAVE + lane;
+               int sl = -1;
+               if (key[u] >= 0) {
+                  bool isn;
+                  sl = H.insert(key[u], isn);
+                  if (sl >= 0) { atomicMin(&vals[sl], q); }
+               }
+               wave_sync();
+               const bool first = (sl >= 0) && vals[sl] == q;
+               const unsigned long long fm = __ballot(first);
+               const int nnew = __popcll(fm);
+               if (nch + nnew > MC) {
+                  defer = true;
+               } else if (first) {
+                  const int pos = nch + __popcll(fm & lanemask_lt(lane));
+                  vals[sl] = pos;
+                  chat_key[pos] = key[u];
+               }
+               nch += nnew;
+               wave_sync();
+            }
+         }
+         if (!defer) {
+            // T5
+            int ccol = 0;
+            if (lane < nch) {
+               ccol = p.f2c[chat_key[lane]];
+               wgt[lane] = 0.0;
+            }
+            if (lane < MC) { mins[lane] = 0x7fffffff; }
+            // entry j: kind 1 C-hat (weight v), 2 strong F neighbour, 3 weak, 0 none
+            int v = -1, sk = 0;
+            if (lane < na) {
+               const int sl = H.find(c1);
+               v = (sl >= 0) ? vals[sl] : -1;
+               if (v >= 0) {
+                  sk = 1;
+               } else if (v <= -2) {
+                  sk = 2;
+               } else if (cf_c1 != -3) {
+                  sk = 3;
+               }
+            }
+            int ki = -1;
+            double sai = 0.0;
+            if (sk == 2) {
+               ki = seg_ki[-2 - v];
+               if (ki >= 0) { sai = p.A_a[ki]; }
+            }
+            const int olen = (sk == 1) ? 1 : ((sk == 2) ? oe - ob : 0);
+            const int oincl = wave_incl_scan(olen, lane);
+            const int OL = readlane_i(oincl, WAVE - 1);
+            const int ooff = oincl - olen;
+            const int nseg2 = max(na, 1);
+            if (OL > XR_OL) { defer = true; }
+            if (!defer) {
+               int tg[3], ts[3];
+               bool dr[3];
+               double ta[3];
+#pragma unroll
+               for (int u = 0; u < 3; u++) {
+                  const int q = u * WAVE + lane;
+                  ts[u] = 0;
+                  tg[u] = -1;
+                  dr[u] = false;
+                  ta[u] = 0.0;
+                  if (u * WAVE >= OL) { continue; }   // uniform
+                  const int s = seg_find_lanes(ooff, nseg2, q);
+                  const int sk_s = __shfl(sk, s, WAVE);
+                  const int ob_s = __shfl(ob, s, WAVE);
+                  const int oo_s = __shfl(ooff, s, WAVE);
+                  const int v_s = __shfl(v, s, WAVE);
+                  const double aij_s = shfl_d(aij, s);
+                  ts[u] = s;
+                  if (q < OL) {
+                     if (sk_s == 1) {
+                        dr[u] = true;
+                        tg[u] = v_s;
+                        ta[u] = aij_s;
+                     } else {
+                        const int e = ob_s + (q - oo_s);
+                        const int col = p.OC_j[e];
+                        const int kq = p.OC_k[e];
+                        ta[u] = p.OC_a[e];
+                        const int sl = H.find(col);
+                        const int v2 = (sl >= 0) ? vals[sl] : -1;
+                        tg[u] = (v2 >= 0) ? v2 : -1;
+                        st_a[q] = ta[u];
+                        st_k[q] = (v2 >= 0) ? kq : -1;
+                     }
+                  }
+               }
+               wave_sync();
+               // denominators (par_lr_interp.c:1700-1760): the terms of row c1
+               // whose column is in C-hat, in column order, with the entry of
+               // column i merged in by its storage index
+               double ssum = 0.0;
+               bool shas_i = false;
+               if (sk == 2) {
+                  shas_i = (ki >= 0) && ((dg1 < 0) ? (sai > 0) : (sai < 0));
+                  double acc = 0.0;
+                  bool pend = shas_i;
+                  for (int q = ooff; q < ooff + olen; q++) {
+                     const int kq = st_k[q];
+                     if (kq >= 0) {
+                        if (pend && ki < kq) {
+                           acc += sai;
+                           pend = false;
+                        }
+                        acc += st_a[q];
+                     }
+                  }
+                  if (pend) { acc += sai; }
+                  ssum = acc;
+               }
+               // per entry: distribution factor and diagonal term, in entry order
+               bool hasd = false, sdist = false;
+               double dval = 0.0, dist = 0.0;
+               if (sk == 2) {
+                  if (ssum != 0) {
+                     dist = aij / ssum;
+                     sdist = true;
+                     if (shas_i) { hasd = true; dval = dist * sai; }
+                  } else {
+                     hasd = true;
+                     dval = aij;
+                  }
+               } else if (sk == 3) {
+                  hasd = true;   // weak connection
+                  dval = aij;
+               }
+               unsigned long long dm = __ballot(hasd);
+               while (dm) {
+                  const int t = __ffsll((long long) dm) - 1;
+                  diag += readlane_d(dval, t);
+                  dm &= dm - 1;
+               }
+               // weights: a_ij of a C-hat entry, distribute * a of a strong F
+               // neighbour with a nonzero denominator; terms of one step that
+               // hit the same weight are applied lowest lane first
+#pragma unroll
+               for (int u = 0; u < 3; u++) {
+                  if (u * WAVE >= OL) { continue; }   // uniform
+                  const int d_s = __shfl((int) sdist, ts[u], WAVE);
+                  const double dist_s = shfl_d(dist, ts[u]);
+                  bool pending = (tg[u] >= 0) && (dr[u] || d_s);
+                  const double x = dr[u] ? ta[u] : dist_s * ta[u];
+                  while (__ballot(pending)) {
+                     if (pending) { atomicMin(&mins[tg[u]], lane); }
+                     wave_sync();
+                     if (pending && mins[tg[u]] == lane) {
+                        wgt[tg[u]] += x;
+                        mins[tg[u]] = 0x7fffffff;
+                        pending = false;
+                     }
+                     wave_sync();
+                  }
+               }
+               wave_sync();
+               if (lane < nch) {
+                  double w = wgt[lane];
+                  if (diag != 0.0) { w /= -diag; }
+                  p.raw_j[ro + lane] = ccol;
+                  p.raw_a[ro + lane] = w;
+               }
+               if (lane == 0) { p.cnt[wi] = nch; }
+            }
+         }
+      }
+      if (defer && lane == 0) {
+         p.cnt[wi] = -1;
+         const int d = atomicAdd(p.ndefer, 1);
+         p.defer_list[d] = i;
+      }
+      wave_sync();
+   }
+}
+
 // hypre_qsort2_abs (src/utilities/qsort.c:452) on a row stored with stride SS,
 // refined only as far as the first `keep` positions need; explicit stack.
 template <int SS>
@@ -1685,7 +2018,7 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
          dzero(f.ndefer, sizeof(int));
          const int nbt = (int) std::max<long long>(1, std::min<long long>((nwork + TR_ROWS - 1) / TR_ROWS, 16384));
          if (pass == 0) {
-            k_extpi_fast<XF_MC, XF_QC><<<launch_blocks(nwork), WAVE, 0, stream()>>>(f);
+            k_extpi_row64<XF_MC><<<launch_blocks(nwork), WAVE, 0, stream()>>>(f);
             k_trunc_rows<XF_MC><<<nbt, TR_ROWS, 0, stream()>>>(nwork, list, raw_cnt, f.raw_j, f.raw_a,
                                                                trunc_factor, max_elmts, stride, cnt,
                                                                tmp_j, tmp_a);
@@ -2278,7 +2611,8 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
          }
          if (lane == 0) {
             p.cnt[ic] = nac;
-            p.src[ic] = (p.stage == 0) ? (int) o : -2 - (int) o;
+            // stage 0: the offset; stage 1 or 2: -2 - (2 * offset + stage - 1)
+            p.src[ic] = (p.stage == 0) ? (int) o : -2 - (2 * (int) o + p.stage - 1);
          }
       } else if (lane == 0) {
          p.src[ic] = -1;
@@ -2293,7 +2627,8 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
 __global__ void k_rap_compact(int nc, const int *__restrict__ cnt, const int *__restrict__ src,
                               const int *__restrict__ out_ia, const int *__restrict__ tj0,
                               const double *__restrict__ ta0, const int *__restrict__ tj1,
-                              const double *__restrict__ ta1, int *__restrict__ oj,
+                              const double *__restrict__ ta1, const int *__restrict__ tj2,
+                              const double *__restrict__ ta2, int *__restrict__ oj,
                               double *__restrict__ oa)
 {
    const int g = threadIdx.x & 15;
@@ -2303,9 +2638,10 @@ __global__ void k_rap_compact(int nc, const int *__restrict__ cnt, const int *__
       if (s == -1) { continue; }
       const int c = cnt[r];
       const int o = out_ia[r];
-      const int *tj = s >= 0 ? tj0 : tj1;
-      const double *ta = s >= 0 ? ta0 : ta1;
-      const size_t b = s >= 0 ? (size_t) s : (size_t) (-2 - s);
+      const int later = s >= 0 ? 0 : 1 + ((-2 - s) & 1);
+      const int *tj = later == 0 ? tj0 : (later == 1 ? tj1 : tj2);
+      const double *ta = later == 0 ? ta0 : (later == 1 ? ta1 : ta2);
+      const size_t b = s >= 0 ? (size_t) s : (size_t) ((-2 - s) >> 1);
       for (int k = g; k < c; k += 16) {
          oj[o + k] = tj[b + k];
          oa[o + k] = ta[b + k];
@@ -2327,20 +2663,23 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
    Ac.a32 = nullptr;
    Ac.ia = dalloc_n<int>((size_t) nc + 1);
 
-   // fast path, stage 0: RA table 512 slots, Ac table 256 (rows of <= 128
-   // entries, output indexed by row); stage 1: RA 2048, Ac 1024 (<= 640
-   // entries, output indexed by position in the stage's row list)
+   // fast path in three stages of growing tables; stage 0 writes its rows
+   // indexed by coarse row, later stages by position in their row list
    RapFastArgs f{};
    f.nc = nc;
    f.R_ia = R.ia; f.R_ja = R.ja; f.R_a = R.a;
    f.A_ia = A.ia; f.A_ja = A.ja; f.A_a = A.a;
    f.P_ia = P.ia; f.P_ja = P.ja; f.P_a = P.a;
-   static const int rcs[2] = {512, 2048}, acs[2] = {256, 1024}, acls[2] = {128, 640};
+   // stage 0: RA <= 320 entries (in 5 registers per lane), Ac <= 128;
+   // stage 1: RA <= 640 (10 registers), Ac <= 256; stage 2: RA <= 1280, Ac <= 640
+   static const int rcs[3] = {512, 1024, 2048}, acs[3] = {256, 512, 1024},
+                    acls[3] = {128, 256, 640};
+   static const int raregs[3] = {RAP_RAREG, 2 * RAP_RAREG, 0};
    int *cnt = dalloc_n<int>((size_t) nc + 1);
    int *src = dalloc_n<int>((size_t) nc + 1);
    dzero(cnt, sizeof(int) * ((size_t) nc + 1));
-   int *tj[2] = {nullptr, nullptr};
-   double *ta[2] = {nullptr, nullptr};
+   int *tj[3] = {nullptr, nullptr, nullptr};
+   double *ta[3] = {nullptr, nullptr, nullptr};
    int *lists[2] = {dalloc_n<int>((size_t) nc + 1), dalloc_n<int>((size_t) nc + 1)};
    int *ndefer = device_flags() + 60;
    int *h = static_cast<int *>(pinned_scratch(64));
@@ -2350,7 +2689,9 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
    const int *list = nullptr;
    int nwork = nc;
    int cur = 0;
-   for (int stage = 0; stage < 2 && nwork > 0; stage++) {
+   for (int stage = 0; stage < 3 && nwork > 0; stage++) {
+      // offsets are kept as int (times 2 after stage 0)
+      if ((long long) nwork * acls[stage] * 2 >= INT_MAX) { break; }
       tj[stage] = dalloc_n<int>((size_t) nwork * acls[stage]);
       ta[stage] = dalloc_n<double>((size_t) nwork * acls[stage]);
       f.rowlist = list;
@@ -2365,13 +2706,13 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
       f.defer_list = lists[cur];
       dzero(ndefer, sizeof(int));
       f.claim = claim_counters();
+      if (raregs[stage] > 0) { f.rcl = raregs[stage] * WAVE; }
+      const size_t lds = rap_fast_lds(f.rc, f.rcl, f.ac, f.acl, raregs[stage]);
       if (stage == 0) {
-         // RA rows of <= 320 entries: held in 5 registers per lane for the Ac phase
-         f.rcl = RAP_RAREG * WAVE;
-         const size_t lds = rap_fast_lds(f.rc, f.rcl, f.ac, f.acl, RAP_RAREG);
          k_rap_fast<RAP_RAREG><<<launch_blocks(nwork), WAVE, lds, stream()>>>(f);
+      } else if (stage == 1) {
+         k_rap_fast<2 * RAP_RAREG><<<launch_blocks(nwork), WAVE, lds, stream()>>>(f);
       } else {
-         const size_t lds = rap_fast_lds(f.rc, f.rcl, f.ac, f.acl, 0);
          k_rap_fast<0><<<launch_blocks(nwork), WAVE, lds, stream()>>>(f);
       }
       HX_LAUNCH_CHECK();
@@ -2394,8 +2735,8 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
    Ac.ja = dalloc_n<int>((size_t) std::max<long long>(Ac.nnz, 1));
    Ac.a = dalloc_n<double>((size_t) std::max<long long>(Ac.nnz, 1));
    int nbc = (int) std::max<long long>(1, std::min<long long>(((long long) nc * 16 + 255) / 256, 16384));
-   k_rap_compact<<<nbc, 256, 0, stream()>>>(nc, cnt, src, Ac.ia, tj[0], ta[0], tj[1], ta[1], Ac.ja,
-                                            Ac.a);
+   k_rap_compact<<<nbc, 256, 0, stream()>>>(nc, cnt, src, Ac.ia, tj[0], ta[0], tj[1], ta[1], tj[2],
+                                            ta[2], Ac.ja, Ac.a);
    HX_LAUNCH_CHECK();
    if (nwork > 0) {
       a.out_ia = Ac.ia;
@@ -2405,7 +2746,7 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
    }
    dfree(lists[0]);
    dfree(lists[1]);
-   for (int s = 0; s < 2; s++) {
+   for (int s = 0; s < 3; s++) {
       dfree(tj[s]);
       dfree(ta[s]);
    }
