@@ -86,6 +86,9 @@ static void make_fp32_view(const Csr &src, Csr &dst)
 // src/parcsr_ls/par_amg_setup.c:1002-3185 (the level loop) for coarsen_type
 // 8/9, interp_type 6, relax 18/0 with Gaussian elimination on the coarsest
 // grid.
+// device word that the strength pass of `level` sets on a zero l1 norm
+static int *l1_zero_flag(int level) { return device_flags() + 768 + std::min(level, 255); }
+
 void AMGSolver::setup(const Csr &A0)
 {
    double t0 = wall_seconds();
@@ -103,10 +106,18 @@ void AMGSolver::setup(const Csr &A0)
       const Csr &A = *lv.A;
       const int fine_size = A.nrows;
       if (p.max_levels == 1) { break; }
-      create_s(A, theta(level), p.max_row_sum, lv.S);
+      // the strength pass also yields the column counts of S (PMIS measure)
+      // and, for l1-Jacobi smoothing, the level's l1 norms
+      int *stc = dalloc_n<int>((size_t) std::max(fine_size, 1));
+      if (p.relax_type[1] == 18 || p.relax_type[2] == 18) {
+         lv.l1 = dalloc_n<double>((size_t) std::max(fine_size, 1));
+         dzero(l1_zero_flag(level), sizeof(int));
+      }
+      create_s(A, theta(level), p.max_row_sum, lv.S, stc, lv.l1, l1_zero_flag(level));
       lv.CF = dalloc_n<int>((size_t) fine_size);
       if (p.keep_setup_data) { lv.measure = dalloc_n<double>((size_t) fine_size); }
-      lv.pmis_rounds = pmis(lv.S, nullptr, lv.CF, lv.measure);
+      lv.pmis_rounds = pmis(lv.S, nullptr, lv.CF, lv.measure, stc);
+      dfree(stc);
       int *f2c = dalloc_n<int>((size_t) fine_size + 1);
       const int coarse_size = coarse_map(fine_size, lv.CF, f2c);
       // par_amg_setup.c:1690-1725
@@ -160,9 +171,17 @@ void AMGSolver::setup(const Csr &A0)
       const bool need_l1 = (j < num_levels - 1) ? (p.relax_type[1] == 18 || p.relax_type[2] == 18)
                                                   : (coarse_relax == 18);
       (void) rt;
-      if (need_l1) {
+      if (need_l1 && lv.l1) {
+         // computed with the level's strength matrix
+         int h = 0;
+         d2h(&h, l1_zero_flag(j), sizeof(int));
+         if (h) { g_error_arg = 1; set_error(HYPRE_ERROR_ARG, "zero l1 norm"); }
+      } else if (need_l1) {
          lv.l1 = dalloc_n<double>((size_t) n);
          l1_norms(*lv.A, lv.l1);
+      } else if (lv.l1) {
+         dfree(lv.l1);
+         lv.l1 = nullptr;
       }
       if (j > 0) {
          lv.f = dalloc_n<double>((size_t) n);

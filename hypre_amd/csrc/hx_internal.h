@@ -150,12 +150,17 @@ void   gselim_solve32(int n, const double *Amat_dev, const float *f, float *u);
 
 // ------------------------------------------------------------------ setup kernels
 // strength pattern (S.a unused), sequential row order kept
-void create_s(const Csr &A, double theta, double max_row_sum, Csr &S);
+// S of A; optionally also the column counts of S (st_count, n ints) and the
+// l1 norms of A's rows (l1, n doubles; *l1_zero set to 1 on a zero norm)
+void create_s(const Csr &A, double theta, double max_row_sum, Csr &S, int *st_count = nullptr,
+              double *l1 = nullptr, int *l1_zero = nullptr);
 // l1 norms option 1 of hypre_ParCSRComputeL1Norms
 void l1_norms(const Csr &A, double *l1);
 // PMIS: measure (optional input: given measure on device, else Park-Miller
 // stream seed 2747), CF out; returns number of rounds
-int  pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_keep_dev);
+// st_count: the column counts of S when already known (else counted here)
+int  pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_keep_dev,
+          const int *st_count = nullptr);
 // fine->coarse map; returns n_coarse
 int  coarse_map(int n, const int *CF, int *f2c);
 // ext+i interpolation + truncation (CF: -3 -> -1 on exit)

@@ -206,81 +206,263 @@ static int max_row_len(const Csr &A)
 }
 
 // ================================================================== strength
-// src/parcsr_ls/par_strength.c:75-525 (CreateSHost, num_functions 1, one rank).
-// K lanes per row; row_scale is a min/max (order-free), the row sum is folded
-// in column order, S keeps the column order of A.
-template <int K, bool FILL>
-__global__ void k_strength(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                           const double *__restrict__ a, double theta, double mrs,
-                           int *__restrict__ S_ia_or_cnt, int *__restrict__ S_ja)
+// src/parcsr_ls/par_strength.c:75-525 (CreateSHost, num_functions 1, one rank):
+// row_scale is a min/max (order-free), the row sum is folded in column order,
+// S keeps the column order of A.
+// ------------------------------------------------------------------ strength, fused passes
+// k_row_stats: one row per lane, 64 rows per wave, the values streamed in
+// coalesced windows of RS_SW entries through LDS (as the solve-phase SpMV);
+// each lane walks its row in storage order and produces what k_strength and
+// k_l1 derive per row: the strength threshold theta * row_scale with the
+// diagonal-sign and all-weak flags (par_strength.c:260-330), the l1 norm
+// (hypre_ParCSRComputeL1Norms option 1 with the sign of the diagonal), and,
+// when the row lies in one window, its number of strong entries (-1 when it
+// does not: k_strength_recount counts those rows).
+constexpr int RS_SW = 1024;
+constexpr int RS_WPB = 4;
+
+__global__ __launch_bounds__(RS_WPB * WAVE) void k_row_stats(int n, const int *__restrict__ ia,
+                                                               const double *__restrict__ a,
+                                                               double theta, double mrs,
+                                                               double *__restrict__ thr,
+                                                               int *__restrict__ flg,
+                                                               int *__restrict__ cnt,
+                                                               double *__restrict__ l1,
+                                                               int *__restrict__ l1_zero)
+{
+   constexpr int U = RS_SW / WAVE;
+   __shared__ double sb_all[RS_WPB * RS_SW];
+   const int nb = gridDim.x;
+   const int blk = xcd_blk(blockIdx.x, nb);
+   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   double *sb = sb_all + wv * RS_SW;
+   const int ng = (n + WAVE - 1) / WAVE;
+   const int gpb = (ng + nb - 1) / nb;
+   const int g0 = blk * gpb, g1 = min(ng, g0 + gpb);
+   for (int g = g0 + wv; g < g1; g += RS_WPB) {
+      const int r0 = g * WAVE;
+      const int nr = min(WAVE, n - r0);
+      int rs = 0, re = 0;
+      if (lane < nr) {
+         rs = ia[r0 + lane];
+         re = ia[r0 + lane + 1];
+      }
+      const int base = __shfl(rs, 0, WAVE);
+      const int end = __shfl(re, nr - 1, WAVE);
+      double diag = 0.0, scale = 0.0, rsum = 0.0, l1s = 0.0;
+      bool neg = false, allweak = false;
+      double t = 0.0;
+      int c = 0;
+      double an[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+         const int j = base + u * WAVE + lane;
+         an[u] = (j < end) ? __builtin_nontemporal_load(a + j) : 0.0;
+      }
+      for (int wb = base; wb < end; wb += RS_SW) {
+         const int we = min(end, wb + RS_SW);
+#pragma unroll
+         for (int u = 0; u < U; u++) { sb[u * WAVE + lane] = an[u]; }
+         const int nxt = wb + RS_SW;
+#pragma unroll
+         for (int u = 0; u < U; u++) {
+            const int j = nxt + u * WAVE + lane;
+            an[u] = (j < end) ? __builtin_nontemporal_load(a + j) : 0.0;
+         }
+         __builtin_amdgcn_wave_barrier();
+         if (lane < nr) {
+            const int s = max(rs, wb), e = min(re, we);
+            for (int k = s; k < e; k++) {
+               const double v = sb[k - wb];
+               if (k == rs) {
+                  diag = v;
+                  neg = v < 0;
+                  rsum = v;
+                  l1s = 0.0 + fabs(v);
+               } else {
+                  scale = neg ? ((scale < v) ? v : scale) : ((scale < v) ? scale : v);
+                  rsum += v;
+                  l1s += fabs(v);
+               }
+            }
+            if (re > wb && re <= we) {
+               // the row ends in this window
+               allweak = (fabs(rsum) > fabs(diag) * mrs) && (mrs < 1.0);
+               t = theta * scale;
+               if (rs >= wb) {
+                  if (!allweak) {
+                     for (int k = rs + 1; k < re; k++) {
+                        const double v = sb[k - wb];
+                        c += (neg ? !(v <= t) : !(v >= t)) ? 1 : 0;
+                     }
+                  }
+               } else {
+                  c = -1;
+               }
+            }
+         }
+         __builtin_amdgcn_wave_barrier();
+      }
+      if (lane < nr) {
+         const int row = r0 + lane;
+         thr[row] = t;
+         flg[row] = (neg ? 1 : 0) | (allweak ? 2 : 0);
+         cnt[row] = c;
+         if (l1) {
+            double s = l1s;
+            if (re > rs && diag < 0.0) { s = -s; }
+            if (fabs(s) == 0.0) { *l1_zero = 1; }
+            l1[row] = s;
+         }
+      }
+   }
+}
+
+// strong entries of the rows that k_row_stats left uncounted (cnt < 0)
+template <int K>
+__global__ void k_strength_recount(int n, const int *__restrict__ ia, const double *__restrict__ a,
+                                   const double *__restrict__ thr, const int *__restrict__ flg,
+                                   int *__restrict__ cnt)
 {
    const int gl = threadIdx.x & (K - 1);
    const int rows_per_block = blockDim.x / K;
    for (long long rb = (long long) blockIdx.x * rows_per_block; rb < n;
         rb += (long long) gridDim.x * rows_per_block) {
       const long long ir = rb + threadIdx.x / K;
-      const bool act = ir < n;
+      const bool act = ir < n && cnt[ir] < 0;
+      if (!__any(act)) { continue; }
       const int i = act ? (int) ir : 0;
       const int s = act ? ia[i] : 0, e = act ? ia[i + 1] : 0;
-      const double diag = act ? a[s] : 0.0;
-      const bool neg = diag < 0;
-      // pass 1: row_scale and the ordered row sum
-      double scale = 0.0, rsum = diag;
-      for (int kb = s + 1; __any(kb < e); kb += K) {
-         const int k = kb + gl;
-         const bool valid = k < e;
-         const double v = valid ? a[k] : 0.0;
-         if (valid) { scale = neg ? ((scale < v) ? v : scale) : ((scale < v) ? scale : v); }
-         rsum = group_fold<K>(rsum, v, valid);
-      }
-      for (int d = K / 2; d > 0; d >>= 1) {
-         const double o = __shfl_xor(scale, d, K);
-         scale = neg ? ((scale < o) ? o : scale) : ((scale < o) ? scale : o);
-      }
-      const bool allweak = (fabs(rsum) > fabs(diag) * mrs) && (mrs < 1.0);
-      const double thr = theta * scale;
-      // pass 2: strong entries, positions by group ballot prefix
-      int o = FILL && act ? S_ia_or_cnt[i] : 0;
+      const double t = act ? thr[i] : 0.0;
+      const int fl = act ? flg[i] : 2;
+      const bool neg = fl & 1;
       int c = 0;
-      const int gshift = (threadIdx.x & 63) & ~(K - 1);
-      const unsigned long long gmask = (K == 64) ? ~0ULL : ((1ULL << K) - 1);
-      for (int kb = s + 1; __any(kb < e); kb += K) {
-         const int k = kb + gl;
-         bool strong = false;
-         if (k < e && !allweak) {
+      if (!(fl & 2)) {
+         for (int k = s + 1 + gl; k < e; k += K) {
             const double v = a[k];
-            strong = neg ? !(v <= thr) : !(v >= thr);
+            c += (neg ? !(v <= t) : !(v >= t)) ? 1 : 0;
          }
-         const unsigned long long m = (__ballot(strong) >> gshift) & gmask;
-         if (FILL && strong) { S_ja[o + c + __popcll(m & ((1ULL << gl) - 1))] = ja[k]; }
-         c += __popcll(m);
       }
-      if (!FILL && act && gl == 0) { S_ia_or_cnt[i] = c; }
+      for (int d = K / 2; d > 0; d >>= 1) { c += __shfl_xor(c, d, K); }
+      if (act && gl == 0) { cnt[i] = c; }
    }
 }
 
-void create_s(const Csr &A, double theta, double max_row_sum, Csr &S)
+// S rows (strong columns in storage order) at the offsets of S_ia; with stc,
+// also the number of rows in which each column is strong (the columns of
+// S^T), aggregated per block in an LDS table over SF_ROWS consecutive rows.
+constexpr int SF_ROWS = 256;
+constexpr int SF_SLOTS = 4096;
+
+template <int K>
+__global__ __launch_bounds__(256) void k_strength_fill(int n, const int *__restrict__ ia,
+                                                       const int *__restrict__ ja,
+                                                       const double *__restrict__ a,
+                                                       const double *__restrict__ thr,
+                                                       const int *__restrict__ flg,
+                                                       const int *__restrict__ S_ia,
+                                                       int *__restrict__ S_ja,
+                                                       int *__restrict__ stc)
+{
+   __shared__ int hkey[SF_SLOTS];
+   __shared__ int hcnt[SF_SLOTS];
+   const int gl = threadIdx.x & (K - 1);
+   const int rows_per_iter = 256 / K;
+   const int gshift = (threadIdx.x & 63) & ~(K - 1);
+   const unsigned long long gmask = (K == 64) ? ~0ULL : ((1ULL << K) - 1);
+   if (stc) {
+      for (int s = threadIdx.x; s < SF_SLOTS; s += 256) { hkey[s] = -1; hcnt[s] = 0; }
+      __syncthreads();
+   }
+   const long long r_begin = (long long) blockIdx.x * SF_ROWS;
+   const long long r_end = min((long long) n, r_begin + SF_ROWS);
+   for (long long rb = r_begin; rb < r_end; rb += rows_per_iter) {
+      const long long ir = rb + threadIdx.x / K;
+      const bool act = ir < r_end;
+      const int i = act ? (int) ir : 0;
+      const int s = act ? ia[i] : 0, e = act ? ia[i + 1] : 0;
+      const double t = act ? thr[i] : 0.0;
+      const int fl = act ? flg[i] : 2;
+      const bool neg = fl & 1, allweak = fl & 2;
+      const int o = act ? S_ia[i] : 0;
+      int c = 0;
+      for (int kb = s + 1; __any(kb < e); kb += K) {
+         const int k = kb + gl;
+         bool strong = false;
+         int j = -1;
+         if (k < e && !allweak) {
+            const double v = a[k];
+            strong = neg ? !(v <= t) : !(v >= t);
+            if (strong) { j = ja[k]; }
+         }
+         const unsigned long long m = (__ballot(strong) >> gshift) & gmask;
+         if (strong) {
+            S_ja[o + c + __popcll(m & ((1ULL << gl) - 1))] = j;
+            if (stc) {
+               unsigned h = ((unsigned) j * 2654435761u) >> (32 - ilog2(SF_SLOTS));
+               bool done = false;
+               for (int probe = 0; probe < 32; probe++) {
+                  const int old = atomicCAS(&hkey[h], -1, j);
+                  if (old == -1 || old == j) {
+                     atomicAdd(&hcnt[h], 1);
+                     done = true;
+                     break;
+                  }
+                  h = (h + 1) & (SF_SLOTS - 1);
+               }
+               if (!done) { atomicAdd(&stc[j], 1); }
+            }
+         }
+         c += __popcll(m);
+      }
+   }
+   if (stc) {
+      __syncthreads();
+      for (int s = threadIdx.x; s < SF_SLOTS; s += 256) {
+         if (hkey[s] >= 0) { atomicAdd(&stc[hkey[s]], hcnt[s]); }
+      }
+   }
+}
+
+void create_s(const Csr &A, double theta, double max_row_sum, Csr &S, int *st_count, double *l1,
+              int *l1_zero)
 {
    int n = A.nrows;
    TimedLaunch tl("setup_strength", 0.0);
    S.nrows = n;
    S.ncols = A.ncols;
    S.ia = dalloc_n<int>((size_t) n + 1);
+   S.a = nullptr;
    const int K = lanes_per_row(A.nnz, n);
-   const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
-   dispatch_k(K, [&](auto kc) {
-      k_strength<decltype(kc)::value, false><<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, theta,
-                                                                        max_row_sum, S.ia, nullptr);
-   });
-   HX_LAUNCH_CHECK();
+   double *thr = dalloc_n<double>((size_t) std::max(n, 1));
+   int *flg = dalloc_n<int>((size_t) std::max(n, 1));
+   if (st_count) { dzero(st_count, sizeof(int) * (size_t) std::max(n, 1)); }
+   if (n > 0) {
+      const long long ng = ((long long) n + WAVE - 1) / WAVE;
+      long long nbs = std::min<long long>((ng + RS_WPB - 1) / RS_WPB, 8192);
+      if (nbs >= 64) { nbs = (nbs + 7) / 8 * 8; }
+      k_row_stats<<<(int) std::max<long long>(nbs, 1), RS_WPB * WAVE, 0, stream()>>>(
+          n, A.ia, A.a, theta, max_row_sum, thr, flg, S.ia, l1, l1_zero);
+      HX_LAUNCH_CHECK();
+      const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
+      dispatch_k(K, [&](auto kc) {
+         k_strength_recount<decltype(kc)::value><<<nb, 256, 0, stream()>>>(n, A.ia, A.a, thr, flg,
+                                                                           S.ia);
+      });
+      HX_LAUNCH_CHECK();
+   }
    S.nnz = exclusive_scan(S.ia, S.ia, n);
    S.ja = dalloc_n<int>((size_t) std::max<long long>(S.nnz, 1));
-   S.a = nullptr;
-   dispatch_k(K, [&](auto kc) {
-      k_strength<decltype(kc)::value, true><<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, theta,
-                                                                       max_row_sum, S.ia, S.ja);
-   });
-   HX_LAUNCH_CHECK();
+   if (n > 0) {
+      const int nbf = (int) (((long long) n + SF_ROWS - 1) / SF_ROWS);
+      dispatch_k(K, [&](auto kc) {
+         k_strength_fill<decltype(kc)::value><<<nbf, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, thr, flg,
+                                                                         S.ia, S.ja, st_count);
+      });
+      HX_LAUNCH_CHECK();
+   }
+   dfree(thr);
+   dfree(flg);
 }
 
 // The strong C neighbours of every row, in S order (count or fill pass).
@@ -631,7 +813,8 @@ __global__ void k_pmis_update(int n, double *__restrict__ m, int *__restrict__ C
    if (threadIdx.x == 0 && sh) { atomicAdd(remaining, sh); }
 }
 
-int pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_keep_dev)
+int pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_keep_dev,
+         const int *st_count)
 {
    int n = S.nrows;
    if (n == 0) { return 0; }
@@ -640,14 +823,16 @@ int pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_ke
    int nb = std::max(1, std::min((n + 255) / 256, 8192));
    double *m = dalloc_n<double>(n);
    int *cnt = nullptr;
-   if (!measure_in_dev) {
+   const int *stc = st_count;
+   if (!measure_in_dev && !stc) {
       cnt = dalloc_n<int>(n);
       dzero(cnt, sizeof(int) * n);
       int nbe = std::max<long long>(1, std::min<long long>((S.nnz + 255) / 256, 8192));
       if (S.nnz > 0) { k_col_count<<<nbe, 256, 0, st>>>(S.nnz, S.ja, cnt); }
       HX_LAUNCH_CHECK();
+      stc = cnt;
    }
-   k_pmis_init<<<nb, 256, 0, st>>>(n, S.ia, cnt, measure_in_dev, 2747, m, CF);
+   k_pmis_init<<<nb, 256, 0, st>>>(n, S.ia, stc, measure_in_dev, 2747, m, CF);
    HX_LAUNCH_CHECK();
    if (measure_keep_dev) { d2d(measure_keep_dev, m, sizeof(double) * n); }
    int *remaining = device_flags() + 44;
