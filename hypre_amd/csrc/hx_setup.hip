@@ -465,62 +465,55 @@ void create_s(const Csr &A, double theta, double max_row_sum, Csr &S, int *st_co
    dfree(flg);
 }
 
-// The strong C neighbours of every row, in S order (count or fill pass).
-template <int K, bool FILL>
-__global__ void k_strong_c(int n, const int *__restrict__ S_ia, const int *__restrict__ S_ja,
-                           const int *__restrict__ CF, int *__restrict__ SC_ia_or_cnt,
-                           int *__restrict__ SC_ja)
+// An off-diagonal entry of A whose sign is opposite to its row's diagonal and
+// whose column is a C point: absolute entry index, column, value (16 bytes,
+// one load).
+struct __align__(16) OCEntry
+{
+   int    k;
+   int    j;
+   double a;
+};
+
+// One pass, K lanes per row: the strong C neighbours of the row in S order
+// (SC, stored from the row's offset in S) and its OC entries in storage order
+// (stored from the row's offset in A); list lengths to SC_cnt / OC_cnt.
+template <int K>
+__global__ void k_c_lists(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                          const double *__restrict__ a, const int *__restrict__ S_ia,
+                          const int *__restrict__ S_ja, const int *__restrict__ CF,
+                          int *__restrict__ SC_cnt, int *__restrict__ SC_ja,
+                          int *__restrict__ OC_cnt, OCEntry *__restrict__ OC)
 {
    const int gl = threadIdx.x & (K - 1);
    const int rows_per_block = blockDim.x / K;
    const int gshift = (threadIdx.x & 63) & ~(K - 1);
    const unsigned long long gmask = (K == 64) ? ~0ULL : ((1ULL << K) - 1);
+   const unsigned long long below = (1ULL << gl) - 1;
    for (long long rb = (long long) blockIdx.x * rows_per_block; rb < n;
         rb += (long long) gridDim.x * rows_per_block) {
       const long long ir = rb + threadIdx.x / K;
       const bool act = ir < n;
       const int i = act ? (int) ir : 0;
-      const int s = act ? S_ia[i] : 0, e = act ? S_ia[i + 1] : 0;
-      int o = FILL && act ? SC_ia_or_cnt[i] : 0;
-      int c = 0;
-      for (int kb = s; __any(kb < e); kb += K) {
+      // strong C neighbours
+      const int ss = act ? S_ia[i] : 0, se = act ? S_ia[i + 1] : 0;
+      int csc = 0;
+      for (int kb = ss; __any(kb < se); kb += K) {
          const int k = kb + gl;
          int j = -1;
          bool isc = false;
-         if (k < e) {
+         if (k < se) {
             j = S_ja[k];
             isc = CF[j] >= 0;
          }
          const unsigned long long m = (__ballot(isc) >> gshift) & gmask;
-         if (FILL && isc) { SC_ja[o + c + __popcll(m & ((1ULL << gl) - 1))] = j; }
-         c += __popcll(m);
+         if (isc) { SC_ja[ss + csc + __popcll(m & below)] = j; }
+         csc += __popcll(m);
       }
-      if (!FILL && act && gl == 0) { SC_ia_or_cnt[i] = c; }
-   }
-}
-
-// Off-diagonal entries of every row of A whose sign is opposite to the row's
-// diagonal and whose column is a C point, in storage order: absolute entry
-// index, column and value (count or fill pass).
-template <int K, bool FILL>
-__global__ void k_opp_c(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                        const double *__restrict__ a, const int *__restrict__ CF,
-                        int *__restrict__ OC_ia_or_cnt, int *__restrict__ OC_k,
-                        int *__restrict__ OC_j, double *__restrict__ OC_a)
-{
-   const int gl = threadIdx.x & (K - 1);
-   const int rows_per_block = blockDim.x / K;
-   const int gshift = (threadIdx.x & 63) & ~(K - 1);
-   const unsigned long long gmask = (K == 64) ? ~0ULL : ((1ULL << K) - 1);
-   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < n;
-        rb += (long long) gridDim.x * rows_per_block) {
-      const long long ir = rb + threadIdx.x / K;
-      const bool act = ir < n;
-      const int i = act ? (int) ir : 0;
+      // opposite-sign C entries
       const int s = act ? ia[i] : 0, e = act ? ia[i + 1] : 0;
-      const bool neg = act && a[s] < 0;
-      int o = FILL && act ? OC_ia_or_cnt[i] : 0;
-      int c = 0;
+      const bool neg = act && e > s && a[s] < 0;
+      int coc = 0;
       for (int kb = s + 1; __any(kb < e); kb += K) {
          const int k = kb + gl;
          bool take = false;
@@ -534,69 +527,42 @@ __global__ void k_opp_c(int n, const int *__restrict__ ia, const int *__restrict
             }
          }
          const unsigned long long m = (__ballot(take) >> gshift) & gmask;
-         if (FILL && take) {
-            const int pos = o + c + __popcll(m & ((1ULL << gl) - 1));
-            OC_k[pos] = k;
-            OC_j[pos] = j;
-            OC_a[pos] = v;
-         }
-         c += __popcll(m);
+         if (take) { OC[s + coc + __popcll(m & below)] = OCEntry{k, j, v}; }
+         coc += __popcll(m);
       }
-      if (!FILL && act && gl == 0) { OC_ia_or_cnt[i] = c; }
+      if (act && gl == 0) {
+         SC_cnt[i] = csc;
+         OC_cnt[i] = coc;
+      }
    }
 }
 
-struct OppCLists
+// SC and OC lists of one level (storage as in k_c_lists)
+struct CLists
 {
-   int    *ia = nullptr, *k = nullptr, *j = nullptr;
-   double *a = nullptr;
-   void    free_all()
+   int     *sc_cnt = nullptr, *sc_ja = nullptr, *oc_cnt = nullptr;
+   OCEntry *oc = nullptr;
+   void     free_all()
    {
-      dfree(ia); dfree(k); dfree(j); dfree(a);
-      ia = k = j = nullptr; a = nullptr;
+      dfree(sc_cnt); dfree(sc_ja); dfree(oc_cnt); dfree(oc);
+      sc_cnt = sc_ja = oc_cnt = nullptr;
+      oc = nullptr;
    }
 };
 
-static void opp_c_lists(const Csr &A, const int *CF, OppCLists &OC)
+static void c_lists(const Csr &A, const Csr &S, const int *CF, CLists &L)
 {
    const int n = A.nrows;
-   OC.ia = dalloc_n<int>((size_t) n + 1);
+   L.sc_cnt = dalloc_n<int>((size_t) std::max(n, 1));
+   L.oc_cnt = dalloc_n<int>((size_t) std::max(n, 1));
+   L.sc_ja = dalloc_n<int>((size_t) std::max<long long>(S.nnz, 1));
+   L.oc = dalloc_n<OCEntry>((size_t) std::max<long long>(A.nnz, 1));
+   if (n == 0) { return; }
    const int K = lanes_per_row(A.nnz, n);
    const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
    dispatch_k(K, [&](auto kc) {
-      k_opp_c<decltype(kc)::value, false><<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, CF, OC.ia,
-                                                                     nullptr, nullptr, nullptr);
-   });
-   HX_LAUNCH_CHECK();
-   const long long tot = exclusive_scan(OC.ia, OC.ia, n);
-   OC.k = dalloc_n<int>((size_t) std::max<long long>(tot, 1));
-   OC.j = dalloc_n<int>((size_t) std::max<long long>(tot, 1));
-   OC.a = dalloc_n<double>((size_t) std::max<long long>(tot, 1));
-   dispatch_k(K, [&](auto kc) {
-      k_opp_c<decltype(kc)::value, true><<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, CF, OC.ia,
-                                                                    OC.k, OC.j, OC.a);
-   });
-   HX_LAUNCH_CHECK();
-}
-
-static void strong_c_lists(const Csr &S, const int *CF, Csr &SC)
-{
-   const int n = S.nrows;
-   SC.nrows = n;
-   SC.ncols = S.ncols;
-   SC.ia = dalloc_n<int>((size_t) n + 1);
-   SC.a = nullptr;
-   SC.a32 = nullptr;
-   const int K = lanes_per_row(S.nnz, n);
-   const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
-   dispatch_k(K, [&](auto kc) {
-      k_strong_c<decltype(kc)::value, false><<<nb, 256, 0, stream()>>>(n, S.ia, S.ja, CF, SC.ia, nullptr);
-   });
-   HX_LAUNCH_CHECK();
-   SC.nnz = exclusive_scan(SC.ia, SC.ia, n);
-   SC.ja = dalloc_n<int>((size_t) std::max<long long>(SC.nnz, 1));
-   dispatch_k(K, [&](auto kc) {
-      k_strong_c<decltype(kc)::value, true><<<nb, 256, 0, stream()>>>(n, S.ia, S.ja, CF, SC.ia, SC.ja);
+      k_c_lists<decltype(kc)::value><<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, S.ia, S.ja, CF,
+                                                               L.sc_cnt, L.sc_ja, L.oc_cnt, L.oc);
    });
    HX_LAUNCH_CHECK();
 }
@@ -1337,9 +1303,9 @@ struct ExtpiFastArgs
    const int    *A_ia, *A_ja;
    const double *A_a;
    const int    *S_ia, *S_ja;
-   const int    *SC_ia, *SC_ja;   // strong C neighbours of every row, S order
-   const int    *OC_ia, *OC_k, *OC_j;   // opposite-sign C entries: entry index, column
-   const double *OC_a;                  // and value
+   const int    *SC_cnt, *SC_ja;  // strong C neighbours of row r: SC_ja[S_ia[r] + t], t < SC_cnt[r]
+   const int    *OC_cnt;          // opposite-sign C entries of row r: OC[A_ia[r] + t], t < OC_cnt[r]
+   const OCEntry *OC;
    const int    *CF, *f2c;
    const int    *rowlist;     // rows to process (nullptr: 0..n-1)
    int           nwork;
@@ -1420,7 +1386,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                kind = 1; klen = 1; kb = i1;
             } else if (c1 != -3) {
                // the candidates of a strong F neighbour are its strong C neighbours
-               kind = 2; kb = p.SC_ia[i1]; klen = p.SC_ia[i1 + 1] - kb;
+               kind = 2; kb = p.S_ia[i1]; klen = p.SC_cnt[i1];
             }
          }
          wave_sync();
@@ -1512,8 +1478,8 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                      ib = b1 + 1;
                      ilen = p.A_ia[c1 + 1] - b1 - 1;
                      sneg = p.A_a[b1] < 0;
-                     ob = p.OC_ia[c1];
-                     olen = p.OC_ia[c1 + 1] - ob;
+                     ob = b1;
+                     olen = p.OC_cnt[c1];
                   } else if (p.CF[c1] != -3) {
                      sk = 3;
                   }
@@ -1562,13 +1528,14 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                   double acc = 0.0;
                   bool pend = shas_i;
                   for (int e = ob; e < ob + olen; e++) {
-                     const int s2 = H.find(p.OC_j[e]);
+                     const OCEntry en = p.OC[e];
+                     const int s2 = H.find(en.j);
                      if (s2 >= 0 && vals[s2] >= 0) {
-                        if (pend && ki < p.OC_k[e]) {
+                        if (pend && ki < en.k) {
                            acc += sai;
                            pend = false;
                         }
-                        acc += p.OC_a[e];
+                        acc += en.a;
                         nq++;
                      }
                   }
@@ -1585,11 +1552,12 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                      q_a[pos] = aij;
                   } else if (sk == 2) {
                      for (int e = ob; e < ob + olen; e++) {
-                        const int s2 = H.find(p.OC_j[e]);
+                        const OCEntry en = p.OC[e];
+                        const int s2 = H.find(en.j);
                         const int v2 = (s2 >= 0) ? vals[s2] : -1;
                         if (v2 >= 0) {
                            q_ts[pos] = (v2 << 9) | lane;
-                           q_a[pos] = p.OC_a[e];
+                           q_a[pos] = en.a;
                            pos++;
                         }
                      }
@@ -1740,8 +1708,8 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
          int cf_i1 = 0, scb = 0, sce = 0, rb = 0, re = 0;
          if (lane < ns) {
             cf_i1 = p.CF[i1];
-            scb = p.SC_ia[i1];
-            sce = p.SC_ia[i1 + 1];
+            scb = p.S_ia[i1];
+            sce = scb + p.SC_cnt[i1];
             rb = p.A_ia[i1];
             re = p.A_ia[i1 + 1];
          }
@@ -1750,8 +1718,8 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
             cf_c1 = p.CF[c1];
             cb1 = p.A_ia[c1];
             ce1 = p.A_ia[c1 + 1];
-            ob = p.OC_ia[c1];
-            oe = p.OC_ia[c1 + 1];
+            ob = cb1;
+            oe = cb1 + p.OC_cnt[c1];
          }
          // strong neighbours: kind 1 = C, 2 = F (par_lr_interp.c:1505-1560); a
          // strong F neighbour contributes its strong C neighbours as candidates
@@ -1904,9 +1872,10 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                         ta[u] = aij_s;
                      } else {
                         const int e = ob_s + (q - oo_s);
-                        const int col = p.OC_j[e];
-                        const int kq = p.OC_k[e];
-                        ta[u] = p.OC_a[e];
+                        const OCEntry en = p.OC[e];
+                        const int col = en.j;
+                        const int kq = en.k;
+                        ta[u] = en.a;
                         const int sl = H.find(col);
                         const int v2 = (sl >= 0) ? vals[sl] : -1;
                         tg[u] = (v2 >= 0) ? v2 : -1;
@@ -2171,16 +2140,14 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
    if (max_elmts > 0 && max_elmts <= XF_MC) {
       // fast path into raw rows, truncation per thread, leftovers by k_extpi
       stride = max_elmts;
-      Csr SC;
-      strong_c_lists(S, CF, SC);
-      OppCLists OC;
-      opp_c_lists(A, CF, OC);
+      CLists CL;
+      c_lists(A, S, CF, CL);
       ExtpiFastArgs f{};
       f.n = n;
       f.A_ia = A.ia; f.A_ja = A.ja; f.A_a = A.a;
       f.S_ia = S.ia; f.S_ja = S.ja;
-      f.SC_ia = SC.ia; f.SC_ja = SC.ja;
-      f.OC_ia = OC.ia; f.OC_k = OC.k; f.OC_j = OC.j; f.OC_a = OC.a;
+      f.SC_cnt = CL.sc_cnt; f.SC_ja = CL.sc_ja;
+      f.OC_cnt = CL.oc_cnt; f.OC = CL.oc;
       f.CF = CF; f.f2c = f2c;
       tmp_j = dalloc_n<int>((size_t) std::max(n, 1) * stride);
       tmp_a = dalloc_n<double>((size_t) std::max(n, 1) * stride);
@@ -2226,8 +2193,7 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
          list = lists[pass];
          nwork = h[0];
       }
-      SC.free_all();
-      OC.free_all();
+      CL.free_all();
       if (nwork > 0) {
          a.base = nullptr;
          a.out_j = tmp_j;
