@@ -688,18 +688,18 @@ __global__ void k_pmis_init(int n, const int *__restrict__ S_ia, const int *__re
 // par_coarsen.c:2506-2550: drop candidates with a larger-measure S-neighbour
 // (K lanes per row; every write is a 0, so their order does not matter)
 template <int K>
-__global__ void k_pmis_remove(int n, const int *__restrict__ S_ia, const int *__restrict__ S_ja,
-                              const double *__restrict__ m, int *CF)
+__global__ void k_pmis_remove(int nw, const int *__restrict__ list, const int *__restrict__ S_ia,
+                              const int *__restrict__ S_ja, const double *__restrict__ m, int *CF)
 {
    const int gl = threadIdx.x & (K - 1);
    const int rows_per_block = blockDim.x / K;
    const int gshift = (threadIdx.x & 63) & ~(K - 1);
    const unsigned long long gmask = (K == 64) ? ~0ULL : ((1ULL << K) - 1);
-   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < n;
+   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < nw;
         rb += (long long) gridDim.x * rows_per_block) {
       const long long ir = rb + threadIdx.x / K;
-      const bool act = ir < n;
-      const int i = act ? (int) ir : 0;
+      const bool act = ir < nw;
+      const int i = act ? (list ? list[ir] : (int) ir) : 0;
       const double mi = act ? m[i] : 0.0;
       const bool cand = mi > 1;
       const int s = (act && cand) ? S_ia[i] : 0, e = (act && cand) ? S_ia[i + 1] : 0;
@@ -724,18 +724,18 @@ __global__ void k_pmis_remove(int n, const int *__restrict__ S_ia, const int *__
 // point.  A neighbour's "CF > 0" does not change within this pass (survivors
 // stay 1, everything else is 0 or -1), so the concurrent updates are safe.
 template <int K>
-__global__ void k_pmis_cf(int n, const int *__restrict__ S_ia, const int *__restrict__ S_ja,
-                          const double *__restrict__ m, int *CF)
+__global__ void k_pmis_cf(int nw, const int *__restrict__ list, const int *__restrict__ S_ia,
+                          const int *__restrict__ S_ja, const double *__restrict__ m, int *CF)
 {
    const int gl = threadIdx.x & (K - 1);
    const int rows_per_block = blockDim.x / K;
    const int gshift = (threadIdx.x & 63) & ~(K - 1);
    const unsigned long long gmask = (K == 64) ? ~0ULL : ((1ULL << K) - 1);
-   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < n;
+   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < nw;
         rb += (long long) gridDim.x * rows_per_block) {
       const long long ir = rb + threadIdx.x / K;
-      const bool act = ir < n;
-      const int i = act ? (int) ir : 0;
+      const bool act = ir < nw;
+      const int i = act ? (list ? list[ir] : (int) ir) : 0;
       const double mi = act ? m[i] : 0.0;
       const bool in_graph = act && mi > 0;
       int c = in_graph ? CF[i] : 0;
@@ -756,27 +756,60 @@ __global__ void k_pmis_cf(int n, const int *__restrict__ S_ia, const int *__rest
 }
 
 // par_coarsen.c:2690-2750 graph update, fused with the next round's
-// candidate flags (par_coarsen.c:2485-2495)
-__global__ void k_pmis_update(int n, double *__restrict__ m, int *__restrict__ CF,
-                              int *__restrict__ remaining)
+// candidate flags (par_coarsen.c:2485-2495).  The rows still in the graph
+// (m > 0) are appended to `next` (in no particular order: a round's outcome
+// does not depend on the order of its rows); *remaining counts them.
+__global__ __launch_bounds__(256) void k_pmis_update(int nw, const int *__restrict__ list,
+                                                     double *__restrict__ m, int *__restrict__ CF,
+                                                     int *__restrict__ next,
+                                                     int *__restrict__ remaining)
 {
-   __shared__ int sh;
-   if (threadIdx.x == 0) { sh = 0; }
-   __syncthreads();
-   int local = 0;
-   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-      double mi = m[i];
-      if (!(mi > 0)) { continue; }
-      if (CF[i] != 0) {
-         m[i] = 0.0;
-      } else {
-         local++;
-         if (mi > 1) { CF[i] = 1; }
+   __shared__ int wsum[4];
+   __shared__ int bbase;
+   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+   // each block owns a contiguous chunk of the work list: update and count,
+   // one atomic for the chunk's place in `next`, then append in chunk order
+   const int chunk = (nw + gridDim.x - 1) / gridDim.x;
+   const int w0 = blockIdx.x * chunk, w1 = min(nw, w0 + chunk);
+   int kept = 0;
+   for (int w = w0 + threadIdx.x; w < w1; w += blockDim.x) {
+      const int i = list ? list[w] : w;
+      const double mi = m[i];
+      if (mi > 0) {
+         if (CF[i] != 0) {
+            m[i] = 0.0;
+         } else {
+            kept++;
+            if (mi > 1) { CF[i] = 1; }
+         }
       }
    }
-   if (local) { atomicAdd(&sh, local); }
+   for (int d = 32; d > 0; d >>= 1) { kept += __shfl_xor(kept, d, 64); }
+   if (lane == 0) { wsum[wv] = kept; }
    __syncthreads();
-   if (threadIdx.x == 0 && sh) { atomicAdd(remaining, sh); }
+   if (threadIdx.x == 0) {
+      const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+      bbase = tot ? atomicAdd(remaining, tot) : 0;
+   }
+   __syncthreads();
+   int run = bbase;
+   for (int wb = w0; wb < w1; wb += blockDim.x) {
+      const int w = wb + threadIdx.x;
+      bool keep = false;
+      int i = -1;
+      if (w < w1) {
+         i = list ? list[w] : w;
+         keep = m[i] > 0;
+      }
+      const unsigned long long km = __ballot(keep);
+      __syncthreads();
+      if (lane == 0) { wsum[wv] = __popcll(km); }
+      __syncthreads();
+      int off = run;
+      for (int v = 0; v < wv; v++) { off += wsum[v]; }
+      if (keep) { next[off + __popcll(km & ((1ULL << lane) - 1))] = i; }
+      run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+   }
 }
 
 int pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_keep_dev,
@@ -805,21 +838,31 @@ int pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_ke
    int *h = static_cast<int *>(pinned_scratch(64));
    int rounds = 0;
    const int K = lanes_per_row(S.nnz, n);
-   const int nbk = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
+   // rows in the graph: all rows in the first round, then those left by the update
+   int *lists[2] = {dalloc_n<int>((size_t) n), dalloc_n<int>((size_t) n)};
+   const int *list = nullptr;
+   int nw = n;
    while (true) {
       dzero(remaining, sizeof(int));
+      const int nbk = (int) std::max<long long>(1, std::min<long long>(((long long) nw * K + 255) / 256, 16384));
+      const int nbu = std::max(1, std::min((nw + 255) / 256, 8192));
       dispatch_k(K, [&](auto kc) {
-         k_pmis_remove<decltype(kc)::value><<<nbk, 256, 0, st>>>(n, S.ia, S.ja, m, CF);
-         k_pmis_cf<decltype(kc)::value><<<nbk, 256, 0, st>>>(n, S.ia, S.ja, m, CF);
+         k_pmis_remove<decltype(kc)::value><<<nbk, 256, 0, st>>>(nw, list, S.ia, S.ja, m, CF);
+         k_pmis_cf<decltype(kc)::value><<<nbk, 256, 0, st>>>(nw, list, S.ia, S.ja, m, CF);
       });
-      k_pmis_update<<<nb, 256, 0, st>>>(n, m, CF, remaining);
+      int *next = lists[rounds & 1];
+      k_pmis_update<<<nbu, 256, 0, st>>>(nw, list, m, CF, next, remaining);
       HX_LAUNCH_CHECK();
       rounds++;
       HX_HIP(hipMemcpyAsync(h, remaining, sizeof(int), hipMemcpyDeviceToHost, st));
       sync();
       if (h[0] == 0) { break; }
       if (rounds > 100000) { fail(HYPRE_ERROR_GENERIC, "PMIS did not terminate"); }
+      list = next;
+      nw = h[0];
    }
+   dfree(lists[0]);
+   dfree(lists[1]);
    dfree(m);
    dfree(cnt);
    return rounds;
