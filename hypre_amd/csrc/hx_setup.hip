@@ -2549,7 +2549,7 @@ static void rap_pass(RapArgs a, const int *init_list = nullptr, int init_n = 0)
 // Output rows go to a buffer with a stride of `acl` entries; rows whose RA row
 // exceeds `rcl` entries or whose Ac row exceeds `acl` entries are deferred to
 // the next stage (larger tables) and finally to k_rap.
-constexpr int RF_U = 3;      // chunks of 64 products per batch of loads
+constexpr int RF_U = 4;      // chunks of 64 products per batch of loads
 constexpr int RAP_RAREG = 5; // stage 0: RA row held in 5 registers per lane
 
 struct RapFastArgs
@@ -2604,6 +2604,7 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
    RowHash HC{ac_keys, nullptr, p.ac, ilog2(p.ac)};
    int rk[RAREG > 0 ? RAREG : 1];
    double rx[RAREG > 0 ? RAREG : 1];
+   int rpb[RAREG > 0 ? RAREG : 1], rpl[RAREG > 0 ? RAREG : 1];   // P rows of the RA entries
 
    RowClaim claim(p.claim, p.nwork);
    claim.issue(lane);
@@ -2700,6 +2701,17 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
                rx[t] = ra_val[sl];
             }
          }
+         // the P row offsets of all RA entries in one round of loads
+#pragma unroll
+         for (int t = 0; t < RAREG; t++) {
+            const int k = t * WAVE + lane;
+            rpb[t] = 0;
+            rpl[t] = 0;
+            if (k < nra) {
+               rpb[t] = p.P_ia[rk[t]];
+               rpl[t] = p.P_ia[rk[t] + 1] - rpb[t];
+            }
+         }
          wave_sync();
          for (int s = lane; s < p.ac; s += WAVE) {
             ac_keys[s] = -1;
@@ -2723,22 +2735,22 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
             int len = 0, pb = 0;
             double rav = 0.0;
             if (k < nra) {
-               int i1;
                if (RAREG > 0) {
                   // register kb / 64, selected without dynamic indexing
-                  i1 = rk[0];
                   rav = rx[0];
+                  pb = rpb[0];
+                  len = rpl[0];
 #pragma unroll
                   for (int t = 1; t < RAREG; t++) {
-                     if (kb == t * WAVE) { i1 = rk[t]; rav = rx[t]; }
+                     if (kb == t * WAVE) { rav = rx[t]; pb = rpb[t]; len = rpl[t]; }
                   }
                } else {
                   const int sl = ra_order[k];
-                  i1 = ra_keys[sl];
+                  const int i1 = ra_keys[sl];
                   rav = ra_val[sl];
+                  pb = p.P_ia[i1];
+                  len = p.P_ia[i1 + 1] - pb;
                }
-               pb = p.P_ia[i1];
-               len = p.P_ia[i1 + 1] - pb;
             }
             const int incl = wave_incl_scan(len, lane);
             const int L = readlane_i(incl, WAVE - 1);
