@@ -106,28 +106,37 @@ __device__ __forceinline__ X rows_dot_window(int r0, int nr, int lane, const int
    const int base = __shfl(rs, 0, 64);
    const int end = __shfl(re, nr - 1, 64);
    X t = 0;
+   if (base >= end) { return t; }
+   // Loads are unconditional (positions past the range read its last entry
+   // and are masked when used) so that the compiler issues each batch of
+   // matrix loads and of x gathers back to back and waits once per batch.
    int jn[U];
    V an[U];
 #pragma unroll
    for (int u = 0; u < U; u++) {
-      const int j = base + u * 64 + lane;
-      jn[u] = (j < end) ? __builtin_nontemporal_load(ja + j) : 0;
-      an[u] = (j < end) ? __builtin_nontemporal_load(a + j) : (V) 0;
+      const int j = min(base + u * 64 + lane, end - 1);
+      jn[u] = __builtin_nontemporal_load(ja + j);
+      an[u] = __builtin_nontemporal_load(a + j);
    }
    for (int wb = base; wb < end; wb += SW) {
       const int we = min(end, wb + SW);
+      X xv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) { xv[u] = x[jn[u]]; }
       X p[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
          const int j = wb + u * 64 + lane;
-         p[u] = (j < we) ? (X) an[u] * x[jn[u]] : (X) 0;
+         p[u] = (j < we) ? (X) an[u] * xv[u] : (X) 0;
       }
       const int nxt = wb + SW;
+      if (nxt < end) {
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-         const int j = nxt + u * 64 + lane;
-         jn[u] = (j < end) ? __builtin_nontemporal_load(ja + j) : 0;
-         an[u] = (j < end) ? __builtin_nontemporal_load(a + j) : (V) 0;
+         for (int u = 0; u < U; u++) {
+            const int j = min(nxt + u * 64 + lane, end - 1);
+            jn[u] = __builtin_nontemporal_load(ja + j);
+            an[u] = __builtin_nontemporal_load(a + j);
+         }
       }
 #pragma unroll
       for (int u = 0; u < U; u++) { sbuf[u * 64 + lane] = p[u]; }
