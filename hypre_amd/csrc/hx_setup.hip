@@ -1535,8 +1535,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                seg_ki[lane] = -1;
                wave_sync();
                for (int cb = 0; cb < IL; cb += WAVE * XF_U) {
-                  int kk[XF_U], sq[XF_U];
-                  bool hit[XF_U];
+                  int kk[XF_U], sq[XF_U], cv[XF_U];
 #pragma unroll
                   for (int u = 0; u < XF_U; u++) {
                      const int q = cb + u * WAVE + lane;
@@ -1545,11 +1544,11 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                      const int o_s = __shfl(ioff, s, WAVE);
                      sq[u] = s;
                      kk[u] = b_s + (q - o_s);
-                     hit[u] = (q < IL) && p.A_ja[kk[u]] == i;
+                     cv[u] = (q < IL) ? p.A_ja[kk[u]] : -1;   // compared after all loads
                   }
 #pragma unroll
                   for (int u = 0; u < XF_U; u++) {
-                     if (hit[u]) { seg_ki[sq[u]] = kk[u]; }   // one hit per row at most
+                     if (cv[u] == i) { seg_ki[sq[u]] = kk[u]; }   // one hit per row at most
                   }
                }
                wave_sync();
@@ -1795,7 +1794,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
          // T4: candidates at [0, L), pass I at [L, L + IL)
          int nch = 0;
          for (int cb = 0; cb < L + IL && !defer; cb += WAVE * XR_U) {
-            int key[XR_U], hs[XR_U], hk[XR_U];
+            int key[XR_U], hs[XR_U], hk[XR_U], hv[XR_U];
 #pragma unroll
             for (int u = 0; u < XR_U; u++) {
                const int c0 = cb + u * WAVE;   // uniform: the chunk's first position
@@ -1804,6 +1803,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                key[u] = -1;
                hs[u] = -1;
                hk[u] = 0;
+               hv[u] = -1;
                if (c0 < L) {
                   const int sa = seg_find_lanes(off, nseg, q);
                   const int kind_s = __shfl(kind, sa, WAVE);
@@ -1823,13 +1823,14 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                   const int io_s = __shfl(ioff, si, WAVE);
                   if (qi >= 0 && qi < IL) {
                      hk[u] = ib_s + (qi - io_s);
-                     hs[u] = (p.A_ja[hk[u]] == i) ? si : -1;
+                     hs[u] = si;
+                     hv[u] = p.A_ja[hk[u]];   // compared below, after all loads are issued
                   }
                }
             }
 #pragma unroll
             for (int u = 0; u < XR_U; u++) {
-               if (hs[u] >= 0) { seg_ki[hs[u]] = hk[u]; }   // one hit per row at most
+               if (hs[u] >= 0 && hv[u] == i) { seg_ki[hs[u]] = hk[u]; }   // one hit per row at most
             }
 #pragma unroll
             for (int u = 0; u < XR_U; u++) {
@@ -1890,14 +1891,17 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
             const int nseg2 = max(na, 1);
             if (OL > XR_OL) { defer = true; }
             if (!defer) {
-               int tg[3], ts[3];
+               int tg[3], ts[3], tcol[3], tk[3];
                bool dr[3];
                double ta[3];
+               // the loads of all three chunks first, the lookups after
 #pragma unroll
                for (int u = 0; u < 3; u++) {
                   const int q = u * WAVE + lane;
                   ts[u] = 0;
                   tg[u] = -1;
+                  tcol[u] = -1;
+                  tk[u] = -1;
                   dr[u] = false;
                   ta[u] = 0.0;
                   if (u * WAVE >= OL) { continue; }   // uniform
@@ -1914,17 +1918,22 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                         tg[u] = v_s;
                         ta[u] = aij_s;
                      } else {
-                        const int e = ob_s + (q - oo_s);
-                        const OCEntry en = p.OC[e];
-                        const int col = en.j;
-                        const int kq = en.k;
+                        const OCEntry en = p.OC[ob_s + (q - oo_s)];
+                        tcol[u] = en.j;
+                        tk[u] = en.k;
                         ta[u] = en.a;
-                        const int sl = H.find(col);
-                        const int v2 = (sl >= 0) ? vals[sl] : -1;
-                        tg[u] = (v2 >= 0) ? v2 : -1;
-                        st_a[q] = ta[u];
-                        st_k[q] = (v2 >= 0) ? kq : -1;
                      }
+                  }
+               }
+#pragma unroll
+               for (int u = 0; u < 3; u++) {
+                  const int q = u * WAVE + lane;
+                  if (tcol[u] >= 0) {
+                     const int sl = H.find(tcol[u]);
+                     const int v2 = (sl >= 0) ? vals[sl] : -1;
+                     tg[u] = (v2 >= 0) ? v2 : -1;
+                     st_a[q] = ta[u];
+                     st_k[q] = (v2 >= 0) ? tk[u] : -1;
                   }
                }
                wave_sync();
@@ -2640,22 +2649,25 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
          const int nseg = min(WAVE, rb1 - rb);
          for (int cb = 0; cb < L && !defer; cb += WAVE * RF_U) {
             int key[RF_U], sq[RF_U];
-            double x[RF_U];
+            double x[RF_U], av[RF_U], rsv[RF_U];
+            // loads first (positions past L read entry L - 1), products after,
+            // so that the batch's loads are in flight together
 #pragma unroll
             for (int u = 0; u < RF_U; u++) {
-               const int q = cb + u * WAVE + lane;
+               const int q = min(cb + u * WAVE + lane, L - 1);
                const int s = seg_find_lanes(off, nseg, q);
                const int b_s = __shfl(ab, s, WAVE);
                const int o_s = __shfl(off, s, WAVE);
-               const double r_s = shfl_d(rv, s);
+               rsv[u] = shfl_d(rv, s);
                sq[u] = s;
-               key[u] = -1;
-               x[u] = 0.0;
-               if (q < L) {
-                  const int k = b_s + (q - o_s);
-                  key[u] = p.A_ja[k];
-                  x[u] = r_s * p.A_a[k];
-               }
+               const int k = b_s + (q - o_s);
+               key[u] = p.A_ja[k];
+               av[u] = p.A_a[k];
+            }
+#pragma unroll
+            for (int u = 0; u < RF_U; u++) {
+               if (cb + u * WAVE + lane >= L) { key[u] = -1; }
+               x[u] = rsv[u] * av[u];
             }
 #pragma unroll
             for (int u = 0; u < RF_U; u++) {
@@ -2758,21 +2770,22 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
             const int nseg = min(WAVE, nra - kb);
             for (int cb = 0; cb < L && !defer; cb += WAVE * RF_U) {
                int key[RF_U];
-               double x[RF_U];
+               double x[RF_U], pv[RF_U], rsv[RF_U];
 #pragma unroll
                for (int u = 0; u < RF_U; u++) {
-                  const int q = cb + u * WAVE + lane;
+                  const int q = min(cb + u * WAVE + lane, L - 1);
                   const int s = seg_find_lanes(off, nseg, q);
                   const int b_s = __shfl(pb, s, WAVE);
                   const int o_s = __shfl(off, s, WAVE);
-                  const double r_s = shfl_d(rav, s);
-                  key[u] = -1;
-                  x[u] = 0.0;
-                  if (q < L) {
-                     const int e = b_s + (q - o_s);
-                     key[u] = p.P_ja[e];
-                     x[u] = r_s * p.P_a[e];
-                  }
+                  rsv[u] = shfl_d(rav, s);
+                  const int e = b_s + (q - o_s);
+                  key[u] = p.P_ja[e];
+                  pv[u] = p.P_a[e];
+               }
+#pragma unroll
+               for (int u = 0; u < RF_U; u++) {
+                  if (cb + u * WAVE + lane >= L) { key[u] = -1; }
+                  x[u] = rsv[u] * pv[u];
                }
 #pragma unroll
                for (int u = 0; u < RF_U; u++) {
