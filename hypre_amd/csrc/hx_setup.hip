@@ -1682,19 +1682,21 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
 // ------------------------------------------------------------------ ext+i, rows of <= 64 entries
 // The common case (<= 64 strong neighbours, <= 64 off-diagonal entries, the
 // weight terms in <= XR_OL positions) with the arithmetic of k_extpi_fast but
-// about seven dependent rounds of global loads per row instead of ~20:
+// about five dependent rounds of global loads per row instead of ~20:
 //   T1 row offsets and C/F state; T2 strong neighbours (lane s <-> S entry s)
 //   and off-diagonal entries (lane j <-> entry j); T3 everything indexed by
-//   those columns, read whether or not it is used; T4 one flattened sequence
-//   of the strong-C candidates (C-hat, first-seen order) followed by the rows
-//   of the strong F neighbours (the search for column i, "pass I"); T5 the
-//   terms: one per C-hat entry of row i, and the opposite-sign C entries of
-//   the row of every strong F neighbour, in entry order.
+//   those columns, read whether or not it is used; T4 the strong-C
+//   candidates (C-hat, first-seen order), and for each neighbour row c1 the
+//   entry at the mirror of c1's position in row i, which holds column i when
+//   a symmetric stencil is stored in a symmetric order ("pass I"; rows where
+//   it does not are searched afterwards); T5 the terms: one per C-hat entry
+//   of row i, and the opposite-sign C entries of the row of every strong F
+//   neighbour, in entry order.
 // The denominators fold each row's terms left to right out of LDS, with the
 // entry of column i merged in by storage index; the weights receive the terms
 // straight from registers, 64 per step, lowest lane first on a shared weight.
 // Rows outside these limits are deferred to k_extpi_fast<64, ...>.
-constexpr int XR_U = 4;      // chunks of 64 per batch of the candidate / pass I sequence
+constexpr int XR_U = 4;      // chunks of 64 per batch of the candidate and pass I searches
 constexpr int XR_OL = 192;   // term positions of a row (3 chunks)
 
 template <int MC>
@@ -1705,7 +1707,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
    __shared__ double wgt[MC];
    __shared__ int chat_key[MC];
    __shared__ int mins[MC];
-   __shared__ int seg_ki[WAVE];     // S entry s: index of column i in row S_ja[s], or -1
+   __shared__ int seg_ki[WAVE];     // entry j: index of column i in row A_ja[j], or -1
    __shared__ double st_a[XR_OL];   // term values
    __shared__ int st_k[XR_OL];      // term storage index if its column is in C-hat, else -1
    const int lane = threadIdx.x;
@@ -1745,15 +1747,12 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
          }
          double diag = p.A_a[a0];
          for (int s = lane; s < XF_HC; s += WAVE) { keys[s] = -1; vals[s] = 0x7fffffff; }
-         seg_ki[lane] = -1;
          // T3
-         int cf_i1 = 0, scb = 0, sce = 0, rb = 0, re = 0;
+         int cf_i1 = 0, scb = 0, sce = 0;
          if (lane < ns) {
             cf_i1 = p.CF[i1];
             scb = p.S_ia[i1];
             sce = scb + p.SC_cnt[i1];
-            rb = p.A_ia[i1];
-            re = p.A_ia[i1 + 1];
          }
          int cf_c1 = 0, cb1 = 0, ce1 = 0, ob = 0, oe = 0;
          if (lane < na) {
@@ -1765,13 +1764,12 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
          }
          // strong neighbours: kind 1 = C, 2 = F (par_lr_interp.c:1505-1560); a
          // strong F neighbour contributes its strong C neighbours as candidates
-         // and its row (without the diagonal) to pass I
-         int kind = 0, kb = 0, klen = 0, ib = 0, ilen = 0;
+         int kind = 0, kb = 0, klen = 0;
          if (lane < ns) {
             if (cf_i1 >= 0) {
                kind = 1; klen = 1; kb = i1;
             } else if (cf_i1 != -3) {
-               kind = 2; kb = scb; klen = sce - scb; ib = rb + 1; ilen = re - rb - 1;
+               kind = 2; kb = scb; klen = sce - scb;
             }
          }
          wave_sync();
@@ -1783,27 +1781,30 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
          const int incl = wave_incl_scan(klen, lane);
          const int L = readlane_i(incl, WAVE - 1);
          const int off = incl - klen;
-         const int iincl = wave_incl_scan(ilen, lane);
-         const int IL = readlane_i(iincl, WAVE - 1);
-         const int ioff = iincl - ilen;
          const int nseg = max(ns, 1);
-         // the diagonal of each neighbour row (its sign), read with the first batch
+         // with the first batch: the diagonal of each neighbour row c1 (its
+         // sign) and the entry at the mirror of c1's position in row i, where
+         // a symmetric stencil stored in a symmetric order keeps column i
          double dg1 = 0.0;
-         if (lane < na && ce1 > cb1) { dg1 = p.A_a[cb1]; }
+         int gk = -1, gv = -1;
+         if (lane < na && ce1 > cb1) {
+            dg1 = p.A_a[cb1];
+            const int g = ce1 - 1 - lane;
+            if (g > cb1) {
+               gk = g;
+               gv = p.A_ja[g];
+            }
+         }
          wave_sync();
-         // T4: candidates at [0, L), pass I at [L, L + IL)
+         // T4: the candidates
          int nch = 0;
-         for (int cb = 0; cb < L + IL && !defer; cb += WAVE * XR_U) {
-            int key[XR_U], hs[XR_U], hk[XR_U], hv[XR_U];
+         for (int cb = 0; cb < L && !defer; cb += WAVE * XR_U) {
+            int key[XR_U];
 #pragma unroll
             for (int u = 0; u < XR_U; u++) {
                const int c0 = cb + u * WAVE;   // uniform: the chunk's first position
                const int q = c0 + lane;
-               const int qi = q - L;
                key[u] = -1;
-               hs[u] = -1;
-               hk[u] = 0;
-               hv[u] = -1;
                if (c0 < L) {
                   const int sa = seg_find_lanes(off, nseg, q);
                   const int kind_s = __shfl(kind, sa, WAVE);
@@ -1817,20 +1818,6 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                      }
                   }
                }
-               if (c0 + WAVE > L && c0 < L + IL) {
-                  const int si = seg_find_lanes(ioff, nseg, qi);
-                  const int ib_s = __shfl(ib, si, WAVE);
-                  const int io_s = __shfl(ioff, si, WAVE);
-                  if (qi >= 0 && qi < IL) {
-                     hk[u] = ib_s + (qi - io_s);
-                     hs[u] = si;
-                     hv[u] = p.A_ja[hk[u]];   // compared below, after all loads are issued
-                  }
-               }
-            }
-#pragma unroll
-            for (int u = 0; u < XR_U; u++) {
-               if (hs[u] >= 0 && hv[u] == i) { seg_ki[hs[u]] = hk[u]; }   // one hit per row at most
             }
 #pragma unroll
             for (int u = 0; u < XR_U; u++) {
@@ -1878,12 +1865,49 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                   sk = 3;
                }
             }
+            // pass I: the entry of column i in the row of each strong F
+            // neighbour; rows whose mirror guess missed are searched
             int ki = -1;
-            double sai = 0.0;
-            if (sk == 2) {
-               ki = seg_ki[-2 - v];
-               if (ki >= 0) { sai = p.A_a[ki]; }
+            const bool miss = (sk == 2) && (gv != i);
+            if (sk == 2 && !miss) { ki = gk; }
+            const int ilen = miss ? ce1 - cb1 - 1 : 0;
+            const int iincl = wave_incl_scan(ilen, lane);
+            const int IL = readlane_i(iincl, WAVE - 1);
+            if (IL > 0) {
+               const int ioff = iincl - ilen;
+               const int ib = cb1 + 1;
+               const int nsegi = max(na, 1);
+               seg_ki[lane] = -1;
+               wave_sync();
+               for (int cb = 0; cb < IL; cb += WAVE * XR_U) {
+                  int hs[XR_U], hk[XR_U], hv[XR_U];
+#pragma unroll
+                  for (int u = 0; u < XR_U; u++) {
+                     const int q = cb + u * WAVE + lane;
+                     hs[u] = -1;
+                     hk[u] = 0;
+                     hv[u] = -1;
+                     if (cb + u * WAVE < IL) {   // uniform
+                        const int si = seg_find_lanes(ioff, nsegi, q);
+                        const int ib_s = __shfl(ib, si, WAVE);
+                        const int io_s = __shfl(ioff, si, WAVE);
+                        if (q < IL) {
+                           hk[u] = ib_s + (q - io_s);
+                           hs[u] = si;
+                           hv[u] = p.A_ja[hk[u]];   // compared after all loads are issued
+                        }
+                     }
+                  }
+#pragma unroll
+                  for (int u = 0; u < XR_U; u++) {
+                     if (hs[u] >= 0 && hv[u] == i) { seg_ki[hs[u]] = hk[u]; }   // one hit per row
+                  }
+               }
+               wave_sync();
+               if (miss) { ki = seg_ki[lane]; }
             }
+            double sai = 0.0;
+            if (sk == 2 && ki >= 0) { sai = p.A_a[ki]; }
             const int olen = (sk == 1) ? 1 : ((sk == 2) ? oe - ob : 0);
             const int oincl = wave_incl_scan(olen, lane);
             const int OL = readlane_i(oincl, WAVE - 1);
