@@ -78,14 +78,17 @@ static int pick_rw(long long nnz, int n)
 __device__ __forceinline__ void wave_fence() { __builtin_amdgcn_wave_barrier(); }
 
 // window size of the SpMV / l1-Jacobi kernels: 1024 entries (measured faster
-// on the 27-point 200^3 fine level: 0.69 vs 0.73 ms), or 512 when the
-// environment sets HYPRE_MI355X_SPMV_WINDOW=512 (read once)
-static int spmv_window()
+// on the 27-point 200^3 fine level than 512), or 512 when the environment sets
+// HYPRE_MI355X_SPMV_WINDOW=512 (read once); 256 for matrices whose 64-row
+// groups hold at most 256 entries on average (interpolation: ~4 per row),
+// where a wide window would mostly issue masked loads
+static int spmv_window(long long nnz, int n)
 {
    static int w = [] {
       const char *e = std::getenv("HYPRE_MI355X_SPMV_WINDOW");
       return (e && std::atoi(e) == SW) ? SW : 1024;
    }();
+   if (n > 0 && (double) nnz / n * 64.0 <= 256.0) { return 256; }
    return w;
 }
 
@@ -123,11 +126,11 @@ __device__ __forceinline__ X rows_dot_window(int r0, int nr, int lane, const int
       X xv[U];
 #pragma unroll
       for (int u = 0; u < U; u++) { xv[u] = x[jn[u]]; }
-      X p[U];
+      // products straight to LDS (no register copy), then the next window's loads
 #pragma unroll
       for (int u = 0; u < U; u++) {
          const int j = wb + u * 64 + lane;
-         p[u] = (j < we) ? (X) an[u] * xv[u] : (X) 0;
+         sbuf[u * 64 + lane] = (j < we) ? (X) an[u] * xv[u] : (X) 0;
       }
       const int nxt = wb + SW;
       if (nxt < end) {
@@ -138,8 +141,6 @@ __device__ __forceinline__ X rows_dot_window(int r0, int nr, int lane, const int
             an[u] = __builtin_nontemporal_load(a + j);
          }
       }
-#pragma unroll
-      for (int u = 0; u < U; u++) { sbuf[u * 64 + lane] = p[u]; }
       wave_fence();
       if (lane < nr) {
          const int s = max(rs, wb), e = min(re, we);
@@ -245,10 +246,14 @@ static int launch_spmv(const Csr &A, const V *vals, X alpha, const X *x, X beta,
                   (double) sizeof(X) * (A.ncols + A.nrows) + (beta != (X) 0 ? sizeof(X) * (double) A.nrows : 0.0);
    TimedLaunch tl(cls, bytes);
    hipStream_t s = stream();
+   const int sw = spmv_window(A.nnz, A.nrows);
    dispatch_rw(rw, [&](auto c) {
-      if (spmv_window() == 1024) {
+      if (sw == 1024) {
          k_spmv_w<decltype(c)::value, 1024, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals,
                                                                         x, alpha, beta, b, y, part);
+      } else if (sw == 256) {
+         k_spmv_w<decltype(c)::value, 256, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals,
+                                                                       x, alpha, beta, b, y, part);
       } else {
          k_spmv_w<decltype(c)::value, SW, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, x,
                                                                       alpha, beta, b, y, part);
@@ -367,10 +372,14 @@ static void launch_relax_l1(const Csr &A, const V *vals, const X *f, const X *u,
    } else {
       const int rw = pick_rw(A.nnz, A.nrows);
       const int nb = grid_for_groups(A.nrows, rw);
+      const int sw = spmv_window(A.nnz, A.nrows);
       dispatch_rw(rw, [&](auto c) {
-         if (spmv_window() == 1024) {
+         if (sw == 1024) {
             k_relax_l1_w<decltype(c)::value, 1024, V, X><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals,
                                                                           f, u, l1, w, uo);
+         } else if (sw == 256) {
+            k_relax_l1_w<decltype(c)::value, 256, V, X><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals,
+                                                                         f, u, l1, w, uo);
          } else {
             k_relax_l1_w<decltype(c)::value, SW, V, X><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, f,
                                                                         u, l1, w, uo);
