@@ -1712,16 +1712,36 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
    __shared__ int st_k[XR_OL];      // term storage index if its column is in C-hat, else -1
    const int lane = threadIdx.x;
    RowHash H{keys, vals, XF_HC, ilog2(XF_HC)};
+   // Rows are pipelined two deep: while row wi runs, the claim of the row
+   // after the next is in flight and the next row's T1 values are loading.
    RowClaim rc(p.claim, p.nwork);
    rc.issue(lane);
-   for (int wi = rc.take(lane); wi >= 0; wi = rc.take(lane)) {
-      rc.issue(lane);   // the next row's claim is in flight while this row runs
-      const int i = p.rowlist ? p.rowlist[wi] : wi;
+   int wn = rc.take(lane);
+   int n_i = 0, n_cf = 0, n_s0 = 0, n_s1 = 0, n_a0 = 0, n_a1 = 0;
+   auto load_t1 = [&](int w) {
+      n_i = p.rowlist ? p.rowlist[w] : w;
+      n_cf = p.CF[n_i];
+      n_s0 = p.S_ia[n_i];
+      n_s1 = p.S_ia[n_i + 1];
+      n_a0 = p.A_ia[n_i];
+      n_a1 = p.A_ia[n_i + 1];
+   };
+   if (wn >= 0) {
+      rc.issue(lane);
+      load_t1(wn);
+   }
+   for (int wi = wn; wi >= 0; wi = wn) {
+      // T1 of this row (loaded during the previous row)
+      const int i = n_i;
+      const int cfi = n_cf;
+      const int s0 = n_s0, s1 = n_s1;
+      const int a0 = n_a0, a1 = n_a1;
+      wn = rc.take(lane);
+      if (wn >= 0) {
+         rc.issue(lane);
+         load_t1(wn);
+      }
       const size_t ro = (size_t) wi * MC;
-      // T1 (the offsets are read before the C/F test)
-      const int cfi = p.CF[i];
-      const int s0 = p.S_ia[i], s1 = p.S_ia[i + 1];
-      const int a0 = p.A_ia[i], a1 = p.A_ia[i + 1];
       if (cfi >= 0 || cfi == -3) {
          if (lane == 0) {
             if (cfi >= 0) {
