@@ -90,6 +90,10 @@ def main():
                          "lap7: 7-point Laplacian (256^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp32-cycle", action="store_true", help="FP32 V-cycle inside FP64 PCG")
+    ap.add_argument("--learned", action="store_true",
+                    help="per-level theta/omega/sweeps from the trained parameter model, kept "
+                         "only if they need less work per digit than the defaults on this "
+                         "problem (decided once, before the timed steps)")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: one setup, 3 PCG iterations, "
                          "then 3 copies of a 1 GiB vector (the bytes reference of the counters)")
@@ -114,6 +118,17 @@ def main():
                        P_max_elmts=4, relax_type=18, num_sweeps=1, max_coarse_size=9,
                        fp32_cycle=1 if args.fp32_cycle else 0)
     pcg = hx.PCG(amg, tol=1e-8, max_iter=3 if args.probe else 1000, two_norm=1)
+    learned_info = None
+    if args.learned:
+        from hypre_amd import learned as hl
+        model = hl.ParamModel.load()
+        which, lst, dst = hl.apply_with_fallback(amg, model, A, b)
+        learned_info = {"choice": which,
+                        "work_per_digit_learned": round(lst.work_per_digit, 4),
+                        "work_per_digit_default": round(dst.work_per_digit, 4),
+                        "iterations_learned_fp64": lst.iterations,
+                        "iterations_default_fp64": dst.iterations,
+                        "decided": "once before the timed steps, from two untimed FP64 solves"}
 
     if args.probe:
         x.fill(0.0)
@@ -186,6 +201,8 @@ def main():
             "rows": n,
             "nnz": nnz,
             "parallelism": "single GPU",
+            "cycle": "fp32 V-cycle in fp64 PCG" if args.fp32_cycle else "fp64",
+            "learned_parameters": learned_info,
         },
         "setup_s": round(float(np.median(setup_t)), 5),
         "solve_s": round(float(np.median(solve_t)), 5),
