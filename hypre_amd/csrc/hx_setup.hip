@@ -2108,7 +2108,7 @@ __device__ void qsort2_abs_strided(int *v, double *w, int cnt, int keep, int *st
 // stride of 65 so that a wave's accesses fall into distinct banks.
 constexpr int TR_ROWS = 64;
 constexpr int TR_SS = 65;
-constexpr int TR_STACK = 16;
+constexpr int TR_STACK = 8;    // >= log2(64) + 1: the smaller side is always refined first
 
 template <int MC>
 __global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__restrict__ rowlist,
