@@ -2066,6 +2066,430 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
    }
 }
 
+// ------------------------------------------------------------------ ext+i, two rows per wave
+// k_extpi_row64 for rows of <= 32 strong neighbours and <= 32 entries (the
+// fine levels of 3-D 7- and 27-point problems): each half of the wave (a
+// group of 32 lanes) runs one row with the same steps, so a wave keeps two
+// rows' dependent loads in flight.  Every scan, segment search and shuffle
+// stays inside its group; loops run while either group has work, the other
+// group's lanes idling through them.  Rows outside these limits (or with
+// more than 32 C-hat points or 96 terms) are deferred to k_extpi_row64.
+template <int G>
+__device__ __forceinline__ int group_incl_scan(int v, int gl)
+{
+#pragma unroll
+   for (int d = 1; d < G; d <<= 1) {
+      const int t = __shfl_up(v, d, G);
+      if (gl >= d) { v += t; }
+   }
+   return v;
+}
+
+// last segment s in [0, nseg) with off_s <= q inside a group of G lanes
+template <int G>
+__device__ __forceinline__ int group_seg_find(int off_reg, int nseg, int q)
+{
+   int lo = 0, hi = nseg - 1;
+#pragma unroll
+   for (int it = 0; it < ilog2(G); it++) {
+      const int mid = (lo + hi + 1) >> 1;
+      const int om = __shfl(off_reg, mid, G);
+      if (lo < hi) {
+         if (om <= q) { lo = mid; } else { hi = mid - 1; }
+      }
+   }
+   return lo;
+}
+
+constexpr int X2_G = 32;     // lanes per row
+constexpr int X2_HC = 128;   // hash slots per row
+constexpr int X2_OL = 96;    // term positions per row (3 chunks of 32)
+
+template <int MC>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(7))) void k_extpi_row32(ExtpiFastArgs p)
+{
+   constexpr int G = X2_G;
+   __shared__ int keys_all[2 * X2_HC];
+   __shared__ int vals_all[2 * X2_HC];
+   __shared__ double wgt_all[2 * MC];
+   __shared__ int chat_key_all[2 * MC];
+   __shared__ int mins_all[2 * MC];
+   __shared__ int seg_ki_all[WAVE];
+   __shared__ double st_a_all[2 * X2_OL];
+   __shared__ int st_k_all[2 * X2_OL];
+   const int lane = threadIdx.x;
+   const int grp = lane / G, gl = lane % G;
+   const unsigned long long gbits = grp ? 0xffffffff00000000ull : 0x00000000ffffffffull;
+   int *keys = keys_all + grp * X2_HC;
+   int *vals = vals_all + grp * X2_HC;
+   double *wgt = wgt_all + grp * MC;
+   int *chat_key = chat_key_all + grp * MC;
+   int *mins = mins_all + grp * MC;
+   int *seg_ki = seg_ki_all + grp * G;
+   double *st_a = st_a_all + grp * X2_OL;
+   int *st_k = st_k_all + grp * X2_OL;
+   RowHash H{keys, vals, X2_HC, ilog2(X2_HC)};
+   // work item w is the pair of rows 2w, 2w + 1; pairs are pipelined as in
+   // k_extpi_row64 (next pair's claim and T1 values in flight)
+   const int npairs = (p.nwork + 1) >> 1;
+   RowClaim rc(p.claim, npairs);
+   rc.issue(lane);
+   int wn = rc.take(lane);
+   int n_i = 0, n_cf = -1, n_s0 = 0, n_s1 = 0, n_a0 = 0, n_a1 = 0;
+   auto load_t1 = [&](int w) {
+      const int r = 2 * w + grp;
+      if (r < p.nwork) {
+         n_i = p.rowlist ? p.rowlist[r] : r;
+         n_cf = p.CF[n_i];
+         n_s0 = p.S_ia[n_i];
+         n_s1 = p.S_ia[n_i + 1];
+         n_a0 = p.A_ia[n_i];
+         n_a1 = p.A_ia[n_i + 1];
+      }
+   };
+   if (wn >= 0) {
+      rc.issue(lane);
+      load_t1(wn);
+   }
+   for (int wp = wn; wp >= 0; wp = wn) {
+      const int wi = 2 * wp + grp;
+      const bool has_row = wi < p.nwork;
+      const int i = n_i;
+      const int cfi = n_cf;
+      const int s0 = n_s0, s1 = n_s1;
+      const int a0 = n_a0, a1 = n_a1;
+      wn = rc.take(lane);
+      if (wn >= 0) {
+         rc.issue(lane);
+         load_t1(wn);
+      }
+      const size_t ro = (size_t) wi * MC;
+      bool live = has_row;
+      if (has_row && (cfi >= 0 || cfi == -3)) {
+         if (gl == 0) {
+            if (cfi >= 0) {
+               p.raw_j[ro] = p.f2c[i];
+               p.raw_a[ro] = 1.0;
+               p.cnt[wi] = 1;
+            } else {
+               p.cnt[wi] = 0;
+            }
+         }
+         live = false;
+      }
+      const int ns = live ? s1 - s0 : 0, na = live ? a1 - a0 - 1 : 0;
+      bool defer = live && (ns > G || na > G);
+      live = live && !defer;
+      // T2
+      int i1 = -1, c1 = -1;
+      double aij = 0.0, diag = 0.0;
+      if (live && gl < ns) { i1 = p.S_ja[s0 + gl]; }
+      if (live && gl < na) {
+         c1 = p.A_ja[a0 + 1 + gl];
+         aij = p.A_a[a0 + 1 + gl];
+      }
+      if (live) { diag = p.A_a[a0]; }
+      for (int s = gl; s < X2_HC; s += G) { keys[s] = -1; vals[s] = 0x7fffffff; }
+      // T3
+      int cf_i1 = 0, scb = 0, sce = 0;
+      if (live && gl < ns) {
+         cf_i1 = p.CF[i1];
+         scb = p.S_ia[i1];
+         sce = scb + p.SC_cnt[i1];
+      }
+      int cf_c1 = 0, cb1 = 0, ce1 = 0, ob = 0, oe = 0;
+      if (live && gl < na) {
+         cf_c1 = p.CF[c1];
+         cb1 = p.A_ia[c1];
+         ce1 = p.A_ia[c1 + 1];
+         ob = cb1;
+         oe = cb1 + p.OC_cnt[c1];
+      }
+      // strong neighbours: kind 1 = C, 2 = F (par_lr_interp.c:1505-1560)
+      int kind = 0, kb = 0, klen = 0;
+      if (live && gl < ns) {
+         if (cf_i1 >= 0) {
+            kind = 1; klen = 1; kb = i1;
+         } else if (cf_i1 != -3) {
+            kind = 2; kb = scb; klen = sce - scb;
+         }
+      }
+      wave_sync();
+      if (kind == 2) {
+         bool isn;
+         const int sl = H.insert(i1, isn);
+         if (sl >= 0) { vals[sl] = -2 - gl; }
+      }
+      const int incl = group_incl_scan<G>(klen, gl);
+      const int L = __shfl(incl, G - 1, G);
+      const int off = incl - klen;
+      const int nseg = max(ns, 1);
+      // the diagonal of each neighbour row and the mirror guess of column i
+      double dg1 = 0.0;
+      int gk = -1, gv = -1;
+      if (live && gl < na && ce1 > cb1) {
+         dg1 = p.A_a[cb1];
+         const int g = ce1 - 1 - gl;
+         if (g > cb1) {
+            gk = g;
+            gv = p.A_ja[g];
+         }
+      }
+      wave_sync();
+      // T4: the candidates, both rows' chunks in the same steps
+      int nch = 0;
+      for (int cb = 0; __any(live && cb < L); cb += G * XR_U) {
+         int key[XR_U];
+#pragma unroll
+         for (int u = 0; u < XR_U; u++) {
+            const int c0 = cb + u * G;   // uniform within the group
+            const int q = c0 + gl;
+            key[u] = -1;
+            if (live && c0 < L) {
+               const int sa = group_seg_find<G>(off, nseg, q);
+               const int kind_s = __shfl(kind, sa, G);
+               const int b_s = __shfl(kb, sa, G);
+               const int o_s = __shfl(off, sa, G);
+               if (q < L) {
+                  if (kind_s == 1) {
+                     key[u] = b_s;
+                  } else if (kind_s == 2) {
+                     key[u] = p.SC_ja[b_s + (q - o_s)];
+                  }
+               }
+            }
+         }
+#pragma unroll
+         for (int u = 0; u < XR_U; u++) {
+            const bool cact = live && (cb + u * G < L);   // uniform within the group
+            const int q = cb + u * G + gl;
+            int sl = -1;
+            if (cact && key[u] >= 0) {
+               bool isn;
+               sl = H.insert(key[u], isn);
+               if (sl >= 0) { atomicMin(&vals[sl], q); }
+            }
+            wave_sync();
+            const bool first = cact && (sl >= 0) && vals[sl] == q;
+            const unsigned long long fm = __ballot(first) & gbits;
+            const int nnew = __popcll(fm);
+            if (cact) {
+               if (nch + nnew > MC) {
+                  defer = true;
+                  live = false;
+               } else if (first) {
+                  const int pos = nch + __popcll(fm & lanemask_lt(lane));
+                  vals[sl] = pos;
+                  chat_key[pos] = key[u];
+               }
+               nch += nnew;
+            }
+            wave_sync();
+         }
+      }
+      // T5
+      int ccol = 0;
+      if (live && gl < nch) {
+         ccol = p.f2c[chat_key[gl]];
+         wgt[gl] = 0.0;
+      }
+      if (gl < MC) { mins[gl] = 0x7fffffff; }
+      // entry j: kind 1 C-hat (weight v), 2 strong F neighbour, 3 weak, 0 none
+      int v = -1, sk = 0;
+      if (live && gl < na) {
+         const int sl = H.find(c1);
+         v = (sl >= 0) ? vals[sl] : -1;
+         if (v >= 0) {
+            sk = 1;
+         } else if (v <= -2) {
+            sk = 2;
+         } else if (cf_c1 != -3) {
+            sk = 3;
+         }
+      }
+      // pass I: the mirror guess, searched where it missed
+      int ki = -1;
+      const bool miss = (sk == 2) && (gv != i);
+      if (sk == 2 && !miss) { ki = gk; }
+      const int ilen = miss ? ce1 - cb1 - 1 : 0;
+      const int iincl = group_incl_scan<G>(ilen, gl);
+      const int IL = __shfl(iincl, G - 1, G);
+      if (__any(IL > 0)) {
+         const int ioff = iincl - ilen;
+         const int ib = cb1 + 1;
+         const int nsegi = max(na, 1);
+         seg_ki[gl] = -1;
+         wave_sync();
+         for (int cb = 0; __any(cb < IL); cb += G * XR_U) {
+            int hs[XR_U], hk[XR_U], hv[XR_U];
+#pragma unroll
+            for (int u = 0; u < XR_U; u++) {
+               const int q = cb + u * G + gl;
+               hs[u] = -1;
+               hk[u] = 0;
+               hv[u] = -1;
+               if (cb + u * G < IL) {   // uniform within the group
+                  const int si = group_seg_find<G>(ioff, nsegi, q);
+                  const int ib_s = __shfl(ib, si, G);
+                  const int io_s = __shfl(ioff, si, G);
+                  if (q < IL) {
+                     hk[u] = ib_s + (q - io_s);
+                     hs[u] = si;
+                     hv[u] = p.A_ja[hk[u]];
+                  }
+               }
+            }
+#pragma unroll
+            for (int u = 0; u < XR_U; u++) {
+               if (hs[u] >= 0 && hv[u] == i) { seg_ki[hs[u]] = hk[u]; }
+            }
+         }
+         wave_sync();
+         if (miss) { ki = seg_ki[gl]; }
+      }
+      double sai = 0.0;
+      if (sk == 2 && ki >= 0) { sai = p.A_a[ki]; }
+      const int olen = (sk == 1) ? 1 : ((sk == 2) ? oe - ob : 0);
+      const int oincl = group_incl_scan<G>(olen, gl);
+      const int OL = __shfl(oincl, G - 1, G);
+      const int ooff = oincl - olen;
+      const int nseg2 = max(na, 1);
+      if (live && OL > X2_OL) {
+         defer = true;
+         live = false;
+      }
+      int tg[3], ts[3], tcol[3], tk[3];
+      bool dr[3];
+      double ta[3];
+#pragma unroll
+      for (int u = 0; u < 3; u++) {
+         const int q = u * G + gl;
+         ts[u] = 0;
+         tg[u] = -1;
+         tcol[u] = -1;
+         tk[u] = -1;
+         dr[u] = false;
+         ta[u] = 0.0;
+         if (!live || u * G >= OL) { continue; }   // uniform within the group
+         const int s = group_seg_find<G>(ooff, nseg2, q);
+         const int sk_s = __shfl(sk, s, G);
+         const int ob_s = __shfl(ob, s, G);
+         const int oo_s = __shfl(ooff, s, G);
+         const int v_s = __shfl(v, s, G);
+         const double aij_s = __shfl(aij, s, G);
+         ts[u] = s;
+         if (q < OL) {
+            if (sk_s == 1) {
+               dr[u] = true;
+               tg[u] = v_s;
+               ta[u] = aij_s;
+            } else {
+               const OCEntry en = p.OC[ob_s + (q - oo_s)];
+               tcol[u] = en.j;
+               tk[u] = en.k;
+               ta[u] = en.a;
+            }
+         }
+      }
+#pragma unroll
+      for (int u = 0; u < 3; u++) {
+         const int q = u * G + gl;
+         if (tcol[u] >= 0) {
+            const int sl = H.find(tcol[u]);
+            const int v2 = (sl >= 0) ? vals[sl] : -1;
+            tg[u] = (v2 >= 0) ? v2 : -1;
+            st_a[q] = ta[u];
+            st_k[q] = (v2 >= 0) ? tk[u] : -1;
+         }
+      }
+      wave_sync();
+      // denominators (par_lr_interp.c:1700-1760)
+      double ssum = 0.0;
+      bool shas_i = false;
+      if (live && sk == 2) {
+         shas_i = (ki >= 0) && ((dg1 < 0) ? (sai > 0) : (sai < 0));
+         double acc = 0.0;
+         bool pend = shas_i;
+         for (int q = ooff; q < ooff + olen; q++) {
+            const int kq = st_k[q];
+            if (kq >= 0) {
+               if (pend && ki < kq) {
+                  acc += sai;
+                  pend = false;
+               }
+               acc += st_a[q];
+            }
+         }
+         if (pend) { acc += sai; }
+         ssum = acc;
+      }
+      bool hasd = false, sdist = false;
+      double dval = 0.0, dist = 0.0;
+      if (live) {
+         if (sk == 2) {
+            if (ssum != 0) {
+               dist = aij / ssum;
+               sdist = true;
+               if (shas_i) { hasd = true; dval = dist * sai; }
+            } else {
+               hasd = true;
+               dval = aij;
+            }
+         } else if (sk == 3) {
+            hasd = true;   // weak connection
+            dval = aij;
+         }
+      }
+      // both rows' diagonal terms, each row's in entry order
+      unsigned long long dm = __ballot(hasd);
+      while (dm) {
+         const int t = __ffsll((long long) dm) - 1;
+         const double dt = readlane_d(dval, t);
+         if ((t / G) == grp) { diag += dt; }
+         dm &= dm - 1;
+      }
+      // weights, terms of one step applied lowest lane first per weight
+#pragma unroll
+      for (int u = 0; u < 3; u++) {
+         const bool chunk = live && (u * G < OL);   // uniform within the group
+         int d_s = 0;
+         double dist_s = 0.0;
+         if (chunk) {
+            d_s = __shfl((int) sdist, ts[u], G);
+            dist_s = __shfl(dist, ts[u], G);
+         }
+         bool pending = chunk && (tg[u] >= 0) && (dr[u] || d_s);
+         const double x = dr[u] ? ta[u] : dist_s * ta[u];
+         while (__ballot(pending)) {
+            if (pending) { atomicMin(&mins[tg[u]], gl); }
+            wave_sync();
+            if (pending && mins[tg[u]] == gl) {
+               wgt[tg[u]] += x;
+               mins[tg[u]] = 0x7fffffff;
+               pending = false;
+            }
+            wave_sync();
+         }
+      }
+      wave_sync();
+      if (live) {
+         if (gl < nch) {
+            double w = wgt[gl];
+            if (diag != 0.0) { w /= -diag; }
+            p.raw_j[ro + gl] = ccol;
+            p.raw_a[ro + gl] = w;
+         }
+         if (gl == 0) { p.cnt[wi] = nch; }
+      }
+      if (has_row && defer && gl == 0) {
+         p.cnt[wi] = -1;
+         const int d = atomicAdd(p.ndefer, 1);
+         p.defer_list[d] = i;
+      }
+      wave_sync();
+   }
+}
+
 // hypre_qsort2_abs (src/utilities/qsort.c:452) on a row stored with stride SS,
 // refined only as far as the first `keep` positions need; explicit stack.
 template <int SS>
@@ -2251,21 +2675,34 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
       int *lists[2] = {dalloc_n<int>((size_t) n + 1), dalloc_n<int>((size_t) n + 1)};
       const int *list = nullptr;
       int nwork = n;
-      // pass 0: C-hat <= 32 points, every row; pass 1: C-hat <= 64, the rest
-      for (int pass = 0; pass < 2 && nwork > 0; pass++) {
-         const int mc = pass == 0 ? XF_MC : 2 * XF_MC;
+      // passes over the rows left by the previous one: two rows per wave (rows
+      // of <= 32 entries; tried where the mean row is that short), one row
+      // per wave (C-hat <= 32 points), then C-hat <= 64 points
+      int kinds[3];
+      int nk = 0;
+      if ((double) A.nnz / std::max(n, 1) <= 28.0) { kinds[nk++] = 0; }
+      kinds[nk++] = 1;
+      kinds[nk++] = 2;
+      for (int pass = 0; pass < nk && nwork > 0; pass++) {
+         const int kind = kinds[pass];
+         const int mc = kind == 2 ? 2 * XF_MC : XF_MC;
          int *raw_cnt = dalloc_n<int>((size_t) nwork + 1);
          f.rowlist = list;
          f.nwork = nwork;
          f.raw_j = dalloc_n<int>((size_t) nwork * mc);
          f.raw_a = dalloc_n<double>((size_t) nwork * mc);
          f.cnt = raw_cnt;
-         f.defer_list = lists[pass];
+         f.defer_list = lists[pass & 1];
          f.ndefer = device_flags() + 60;
          f.claim = claim_counters();
          dzero(f.ndefer, sizeof(int));
          const int nbt = (int) std::max<long long>(1, std::min<long long>((nwork + TR_ROWS - 1) / TR_ROWS, 16384));
-         if (pass == 0) {
+         if (kind == 0) {
+            k_extpi_row32<XF_MC><<<launch_blocks((nwork + 1) / 2), WAVE, 0, stream()>>>(f);
+            k_trunc_rows<XF_MC><<<nbt, TR_ROWS, 0, stream()>>>(nwork, list, raw_cnt, f.raw_j, f.raw_a,
+                                                               trunc_factor, max_elmts, stride, cnt,
+                                                               tmp_j, tmp_a);
+         } else if (kind == 1) {
             k_extpi_row64<XF_MC><<<launch_blocks(nwork), WAVE, 0, stream()>>>(f);
             k_trunc_rows<XF_MC><<<nbt, TR_ROWS, 0, stream()>>>(nwork, list, raw_cnt, f.raw_j, f.raw_a,
                                                                trunc_factor, max_elmts, stride, cnt,
@@ -2286,7 +2723,7 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
          dfree(f.raw_j);
          dfree(f.raw_a);
          dfree(raw_cnt);
-         list = lists[pass];
+         list = lists[pass & 1];
          nwork = h[0];
       }
       CL.free_all();
