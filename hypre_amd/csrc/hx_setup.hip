@@ -2106,7 +2106,7 @@ constexpr int X2_HC = 128;   // hash slots per row
 constexpr int X2_OL = 96;    // term positions per row (3 chunks of 32)
 
 template <int MC>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(6))) void k_extpi_row32(ExtpiFastArgs p)
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void k_extpi_row32(ExtpiFastArgs p)
 {
    constexpr int G = X2_G;
    __shared__ int keys_all[2 * X2_HC];
