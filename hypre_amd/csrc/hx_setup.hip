@@ -154,8 +154,8 @@ static int lanes_per_row(long long nnz, int n)
 {
    double m = n > 0 ? (double) nnz / n : 0.0;
    if (m <= 6) { return 4; }
-   if (m <= 20) { return 8; }
-   if (m <= 48) { return 16; }
+   if (m <= 32) { return 8; }
+   if (m <= 64) { return 16; }
    return 32;
 }
 
