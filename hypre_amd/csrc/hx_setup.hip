@@ -159,6 +159,15 @@ static int lanes_per_row(long long nnz, int n)
    return 32;
 }
 
+// the PMIS rounds and the strength recount do little per entry: more rows
+// per wave pay off (measured on the 27-point fine level)
+static int lanes_per_row_light(long long nnz, int n)
+{
+   double m = n > 0 ? (double) nnz / n : 0.0;
+   if (m <= 32) { return 4; }
+   return lanes_per_row(nnz, n);
+}
+
 template <class F>
 static void dispatch_k(int K, F f)
 {
@@ -444,8 +453,9 @@ void create_s(const Csr &A, double theta, double max_row_sum, Csr &S, int *st_co
       k_row_stats<<<(int) std::max<long long>(nbs, 1), RS_WPB * WAVE, 0, stream()>>>(
           n, A.ia, A.a, theta, max_row_sum, thr, flg, S.ia, l1, l1_zero);
       HX_LAUNCH_CHECK();
-      const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
-      dispatch_k(K, [&](auto kc) {
+      const int KR = lanes_per_row_light(A.nnz, n);
+      const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * KR + 255) / 256, 16384));
+      dispatch_k(KR, [&](auto kc) {
          k_strength_recount<decltype(kc)::value><<<nb, 256, 0, stream()>>>(n, A.ia, A.a, thr, flg,
                                                                            S.ia);
       });
@@ -837,7 +847,7 @@ int pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_ke
    int *remaining = device_flags() + 44;
    int *h = static_cast<int *>(pinned_scratch(64));
    int rounds = 0;
-   const int K = lanes_per_row(S.nnz, n);
+   const int K = lanes_per_row_light(S.nnz, n);
    // rows in the graph: all rows in the first round, then those left by the update
    int *lists[2] = {dalloc_n<int>((size_t) n), dalloc_n<int>((size_t) n)};
    const int *list = nullptr;
