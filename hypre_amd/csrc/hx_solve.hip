@@ -71,7 +71,7 @@ static int pick_rw(long long nnz, int n)
 {
    double avg = n > 0 ? (double) nnz / n : 1.0;
    int rw = 64;
-   while (rw > 1 && rw * avg > 8 * SW) { rw >>= 1; }
+   while (rw > 1 && rw * avg > 4 * SW) { rw >>= 1; }
    return rw;
 }
 
