@@ -79,16 +79,18 @@ __device__ __forceinline__ void wave_fence() { __builtin_amdgcn_wave_barrier(); 
 
 // window size of the SpMV / l1-Jacobi kernels: 1024 entries (measured faster
 // on the 27-point 200^3 fine level than 512), or 512 when the environment sets
-// HYPRE_MI355X_SPMV_WINDOW=512 (read once); 256 for matrices whose 64-row
-// groups hold at most 256 entries on average (interpolation: ~4 per row),
-// where a wide window would mostly issue masked loads
+// HYPRE_MI355X_SPMV_WINDOW=512 (read once); 256 or 512 for matrices whose
+// 64-row groups hold at most that many entries on average (interpolation,
+// 7-point operators), where a wide window would mostly issue masked loads
 static int spmv_window(long long nnz, int n)
 {
    static int w = [] {
       const char *e = std::getenv("HYPRE_MI355X_SPMV_WINDOW");
       return (e && std::atoi(e) == SW) ? SW : 1024;
    }();
-   if (n > 0 && (double) nnz / n * 64.0 <= 256.0) { return 256; }
+   const double group = n > 0 ? (double) nnz / n * 64.0 : 0.0;
+   if (group <= 256.0) { return 256; }
+   if (group <= 512.0) { return 512; }
    return w;
 }
 
