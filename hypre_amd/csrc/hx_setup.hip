@@ -2076,14 +2076,14 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
    }
 }
 
-// ------------------------------------------------------------------ ext+i, two rows per wave
-// k_extpi_row64 for rows of <= 32 strong neighbours and <= 32 entries (the
-// fine levels of 3-D 7- and 27-point problems): each half of the wave (a
-// group of 32 lanes) runs one row with the same steps, so a wave keeps two
-// rows' dependent loads in flight.  Every scan, segment search and shuffle
-// stays inside its group; loops run while either group has work, the other
-// group's lanes idling through them.  Rows outside these limits (or with
-// more than 32 C-hat points or 96 terms) are deferred to k_extpi_row64.
+// ------------------------------------------------------------------ ext+i, several rows per wave
+// k_extpi_row64 for rows of <= G strong neighbours and <= G entries (the fine
+// levels of 3-D 7- and 27-point problems): each group of G lanes runs one row
+// with the same steps, so a wave keeps 64/G rows' dependent loads in flight.
+// Every scan, segment search and shuffle stays inside its group; loops run
+// while any group has work, the other groups' lanes idling through them.
+// Rows outside these limits (or with more than MC <= G C-hat points or 3G
+// terms) are deferred to k_extpi_row64.
 template <int G>
 __device__ __forceinline__ int group_incl_scan(int v, int gl)
 {
@@ -2111,43 +2111,42 @@ __device__ __forceinline__ int group_seg_find(int off_reg, int nseg, int q)
    return lo;
 }
 
-constexpr int X2_G = 32;     // lanes per row
-constexpr int X2_HC = 128;   // hash slots per row
-constexpr int X2_OL = 96;    // term positions per row (3 chunks of 32)
-
-template <int MC>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void k_extpi_row32(ExtpiFastArgs p)
+template <int G, int MC>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void k_extpi_rowg(ExtpiFastArgs p)
 {
-   constexpr int G = X2_G;
-   __shared__ int keys_all[2 * X2_HC];
-   __shared__ int vals_all[2 * X2_HC];
-   __shared__ double wgt_all[2 * MC];
-   __shared__ int chat_key_all[2 * MC];
-   __shared__ int mins_all[2 * MC];
+   static_assert(MC <= G && G <= 32, "one C-hat point per lane of a group");
+   constexpr int R = WAVE / G;     // rows per wave
+   constexpr int HC = 4 * G;       // hash slots per row
+   constexpr int OLC = 3 * G;      // term positions per row (3 chunks)
+   __shared__ int keys_all[R * HC];
+   __shared__ int vals_all[R * HC];
+   __shared__ double wgt_all[R * MC];
+   __shared__ int chat_key_all[R * MC];
+   __shared__ int mins_all[R * MC];
    __shared__ int seg_ki_all[WAVE];
-   __shared__ double st_a_all[2 * X2_OL];
-   __shared__ int st_k_all[2 * X2_OL];
+   __shared__ double st_a_all[R * OLC];
+   __shared__ int st_k_all[R * OLC];
    const int lane = threadIdx.x;
    const int grp = lane / G, gl = lane % G;
-   const unsigned long long gbits = grp ? 0xffffffff00000000ull : 0x00000000ffffffffull;
-   int *keys = keys_all + grp * X2_HC;
-   int *vals = vals_all + grp * X2_HC;
+   const unsigned long long gbits = ((1ull << G) - 1) << (grp * G);
+   int *keys = keys_all + grp * HC;
+   int *vals = vals_all + grp * HC;
    double *wgt = wgt_all + grp * MC;
    int *chat_key = chat_key_all + grp * MC;
    int *mins = mins_all + grp * MC;
    int *seg_ki = seg_ki_all + grp * G;
-   double *st_a = st_a_all + grp * X2_OL;
-   int *st_k = st_k_all + grp * X2_OL;
-   RowHash H{keys, vals, X2_HC, ilog2(X2_HC)};
-   // work item w is the pair of rows 2w, 2w + 1; pairs are pipelined as in
-   // k_extpi_row64 (next pair's claim and T1 values in flight)
-   const int npairs = (p.nwork + 1) >> 1;
-   RowClaim rc(p.claim, npairs);
+   double *st_a = st_a_all + grp * OLC;
+   int *st_k = st_k_all + grp * OLC;
+   RowHash H{keys, vals, HC, ilog2(HC)};
+   // work item w is the set of rows R w .. R w + R - 1; work items are
+   // pipelined as in k_extpi_row64 (next item's claim and T1 values in flight)
+   const int nitems = (p.nwork + R - 1) / R;
+   RowClaim rc(p.claim, nitems);
    rc.issue(lane);
    int wn = rc.take(lane);
    int n_i = 0, n_cf = -1, n_s0 = 0, n_s1 = 0, n_a0 = 0, n_a1 = 0;
    auto load_t1 = [&](int w) {
-      const int r = 2 * w + grp;
+      const int r = R * w + grp;
       if (r < p.nwork) {
          n_i = p.rowlist ? p.rowlist[r] : r;
          n_cf = p.CF[n_i];
@@ -2162,7 +2161,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
       load_t1(wn);
    }
    for (int wp = wn; wp >= 0; wp = wn) {
-      const int wi = 2 * wp + grp;
+      const int wi = R * wp + grp;
       const bool has_row = wi < p.nwork;
       const int i = n_i;
       const int cfi = n_cf;
@@ -2199,7 +2198,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
          aij = p.A_a[a0 + 1 + gl];
       }
       if (live) { diag = p.A_a[a0]; }
-      for (int s = gl; s < X2_HC; s += G) { keys[s] = -1; vals[s] = 0x7fffffff; }
+      for (int s = gl; s < HC; s += G) { keys[s] = -1; vals[s] = 0x7fffffff; }
       // T3
       int cf_i1 = 0, scb = 0, sce = 0;
       if (live && gl < ns) {
@@ -2364,7 +2363,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
       const int OL = __shfl(oincl, G - 1, G);
       const int ooff = oincl - olen;
       const int nseg2 = max(na, 1);
-      if (live && OL > X2_OL) {
+      if (live && OL > OLC) {
          defer = true;
          live = false;
       }
@@ -2690,12 +2689,17 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
       // per wave (C-hat <= 32 points), then C-hat <= 64 points
       int kinds[3];
       int nk = 0;
-      if ((double) A.nnz / std::max(n, 1) <= 28.0) { kinds[nk++] = 0; }
+      const double mean_row = (double) A.nnz / std::max(n, 1);
+      if (mean_row <= 12.0) {
+         kinds[nk++] = 3;   // four rows per wave
+      } else if (mean_row <= 28.0) {
+         kinds[nk++] = 0;   // two rows per wave
+      }
       kinds[nk++] = 1;
       kinds[nk++] = 2;
       for (int pass = 0; pass < nk && nwork > 0; pass++) {
          const int kind = kinds[pass];
-         const int mc = kind == 2 ? 2 * XF_MC : XF_MC;
+         const int mc = kind == 2 ? 2 * XF_MC : (kind == 3 ? XF_MC / 2 : XF_MC);
          int *raw_cnt = dalloc_n<int>((size_t) nwork + 1);
          f.rowlist = list;
          f.nwork = nwork;
@@ -2707,8 +2711,13 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
          f.claim = claim_counters();
          dzero(f.ndefer, sizeof(int));
          const int nbt = (int) std::max<long long>(1, std::min<long long>((nwork + TR_ROWS - 1) / TR_ROWS, 16384));
-         if (kind == 0) {
-            k_extpi_row32<XF_MC><<<launch_blocks((nwork + 1) / 2), WAVE, 0, stream()>>>(f);
+         if (kind == 3) {
+            k_extpi_rowg<16, XF_MC / 2><<<launch_blocks((nwork + 3) / 4), WAVE, 0, stream()>>>(f);
+            k_trunc_rows<XF_MC / 2><<<nbt, TR_ROWS, 0, stream()>>>(nwork, list, raw_cnt, f.raw_j,
+                                                                   f.raw_a, trunc_factor, max_elmts,
+                                                                   stride, cnt, tmp_j, tmp_a);
+         } else if (kind == 0) {
+            k_extpi_rowg<32, XF_MC><<<launch_blocks((nwork + 1) / 2), WAVE, 0, stream()>>>(f);
             k_trunc_rows<XF_MC><<<nbt, TR_ROWS, 0, stream()>>>(nwork, list, raw_cnt, f.raw_j, f.raw_a,
                                                                trunc_factor, max_elmts, stride, cnt,
                                                                tmp_j, tmp_a);
