@@ -113,7 +113,9 @@ void AMGSolver::setup(const Csr &A0)
          lv.l1 = dalloc_n<double>((size_t) std::max(fine_size, 1));
          dzero(l1_zero_flag(level), sizeof(int));
       }
-      create_s(A, theta(level), p.max_row_sum, lv.S, stc, lv.l1, l1_zero_flag(level));
+      double *s_thr = dalloc_n<double>((size_t) std::max(fine_size, 1));
+      int *s_flg = dalloc_n<int>((size_t) std::max(fine_size, 1));
+      create_s(A, theta(level), p.max_row_sum, lv.S, stc, lv.l1, l1_zero_flag(level), s_thr, s_flg);
       lv.CF = dalloc_n<int>((size_t) fine_size);
       if (p.keep_setup_data) { lv.measure = dalloc_n<double>((size_t) fine_size); }
       lv.pmis_rounds = pmis(lv.S, nullptr, lv.CF, lv.measure, stc);
@@ -127,17 +129,24 @@ void AMGSolver::setup(const Csr &A0)
             coarse_sweeps = 1;
          }
          dfree(f2c);
+         dfree(s_thr);
+         dfree(s_flg);
          if (!p.keep_setup_data) { lv.S.free_all(); }
          if (level > 0) { dfree(lv.CF); lv.CF = nullptr; }
          break;
       }
       if (coarse_size < p.min_coarse_size) {
          dfree(f2c);
+         dfree(s_thr);
+         dfree(s_flg);
          if (!p.keep_setup_data) { lv.S.free_all(); }
          dfree(lv.CF); lv.CF = nullptr;
          break;
       }
-      extpi_interp(A, lv.S, lv.CF, f2c, coarse_size, p.trunc_factor, p.P_max_elmts, lv.P);
+      extpi_interp(A, lv.S, lv.CF, f2c, coarse_size, p.trunc_factor, p.P_max_elmts, lv.P, s_thr,
+                   s_flg);
+      dfree(s_thr);
+      dfree(s_flg);
       dfree(f2c);
       if (!p.keep_setup_data) { lv.S.free_all(); }
       transpose(lv.P, lv.R);

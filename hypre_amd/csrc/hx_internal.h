@@ -150,10 +150,13 @@ void   gselim_solve32(int n, const double *Amat_dev, const float *f, float *u);
 
 // ------------------------------------------------------------------ setup kernels
 // strength pattern (S.a unused), sequential row order kept
-// S of A; optionally also the column counts of S (st_count, n ints) and the
-// l1 norms of A's rows (l1, n doubles; *l1_zero set to 1 on a zero norm)
+// S of A; optionally also the column counts of S (st_count, n ints), the
+// l1 norms of A's rows (l1, n doubles; *l1_zero set to 1 on a zero norm) and
+// each row's strength threshold and flags (thr_out, flg_out: n each; bit 0
+// negative diagonal, bit 1 all entries weak) for extpi_interp
 void create_s(const Csr &A, double theta, double max_row_sum, Csr &S, int *st_count = nullptr,
-              double *l1 = nullptr, int *l1_zero = nullptr);
+              double *l1 = nullptr, int *l1_zero = nullptr, double *thr_out = nullptr,
+              int *flg_out = nullptr);
 // l1 norms option 1 of hypre_ParCSRComputeL1Norms
 void l1_norms(const Csr &A, double *l1);
 // PMIS: measure (optional input: given measure on device, else Park-Miller
@@ -164,8 +167,11 @@ int  pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_k
 // fine->coarse map; returns n_coarse
 int  coarse_map(int n, const int *CF, int *f2c);
 // ext+i interpolation + truncation (CF: -3 -> -1 on exit)
+// s_thr / s_flg: the strength thresholds and flags create_s produced for S
+// (optional; they let the C lists be built in one pass over A)
 void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
-                  double trunc_factor, int max_elmts, Csr &P);
+                  double trunc_factor, int max_elmts, Csr &P, const double *s_thr = nullptr,
+                  const int *s_flg = nullptr);
 void transpose(const Csr &P, Csr &R);
 void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac);
 // dense column-major copy of a small CSR matrix (n <= 64)

@@ -434,7 +434,7 @@ __global__ __launch_bounds__(256) void k_strength_fill(int n, const int *__restr
 }
 
 void create_s(const Csr &A, double theta, double max_row_sum, Csr &S, int *st_count, double *l1,
-              int *l1_zero)
+              int *l1_zero, double *thr_out, int *flg_out)
 {
    int n = A.nrows;
    TimedLaunch tl("setup_strength", 0.0);
@@ -443,8 +443,8 @@ void create_s(const Csr &A, double theta, double max_row_sum, Csr &S, int *st_co
    S.ia = dalloc_n<int>((size_t) n + 1);
    S.a = nullptr;
    const int K = lanes_per_row(A.nnz, n);
-   double *thr = dalloc_n<double>((size_t) std::max(n, 1));
-   int *flg = dalloc_n<int>((size_t) std::max(n, 1));
+   double *thr = thr_out ? thr_out : dalloc_n<double>((size_t) std::max(n, 1));
+   int *flg = flg_out ? flg_out : dalloc_n<int>((size_t) std::max(n, 1));
    if (st_count) { dzero(st_count, sizeof(int) * (size_t) std::max(n, 1)); }
    if (n > 0) {
       const long long ng = ((long long) n + WAVE - 1) / WAVE;
@@ -471,8 +471,8 @@ void create_s(const Csr &A, double theta, double max_row_sum, Csr &S, int *st_co
       });
       HX_LAUNCH_CHECK();
    }
-   dfree(thr);
-   dfree(flg);
+   if (!thr_out) { dfree(thr); }
+   if (!flg_out) { dfree(flg); }
 }
 
 // An off-diagonal entry of A whose sign is opposite to its row's diagonal and
@@ -547,6 +547,95 @@ __global__ void k_c_lists(int n, const int *__restrict__ ia, const int *__restri
    }
 }
 
+// The lists of k_c_lists from one windowed pass over A (one row per lane, 64
+// rows per wave, values, columns and the columns' C/F flags staged in LDS per
+// window of CW_SW entries, as in k_row_stats); strong entries are those the
+// strength threshold of the row (kept from k_row_stats) marks, in A order,
+// i.e. S order.
+constexpr int CW_SW = 512;
+constexpr int CW_WPB = 4;
+
+__global__ __launch_bounds__(CW_WPB * WAVE) void k_c_lists_w(int n, const int *__restrict__ ia,
+                                                               const int *__restrict__ ja,
+                                                               const double *__restrict__ a,
+                                                               const double *__restrict__ thr,
+                                                               const int *__restrict__ flg,
+                                                               const int *__restrict__ S_ia,
+                                                               const int *__restrict__ CF,
+                                                               int *__restrict__ SC_cnt,
+                                                               int *__restrict__ SC_ja,
+                                                               int *__restrict__ OC_cnt,
+                                                               OCEntry *__restrict__ OC)
+{
+   constexpr int U = CW_SW / WAVE;
+   __shared__ double sa_all[CW_WPB * CW_SW];
+   __shared__ int sj_all[CW_WPB * CW_SW];
+   __shared__ int sc_all[CW_WPB * CW_SW];
+   const int nb = gridDim.x;
+   const int blk = xcd_blk(blockIdx.x, nb);
+   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   double *sa = sa_all + wv * CW_SW;
+   int *sj = sj_all + wv * CW_SW;
+   int *sc = sc_all + wv * CW_SW;
+   const int ng = (n + WAVE - 1) / WAVE;
+   const int gpb = (ng + nb - 1) / nb;
+   const int g0 = blk * gpb, g1 = min(ng, g0 + gpb);
+   for (int g = g0 + wv; g < g1; g += CW_WPB) {
+      const int r0 = g * WAVE;
+      const int nr = min(WAVE, n - r0);
+      int rs = 0, re = 0, so = 0, fl = 2;
+      double t = 0.0;
+      if (lane < nr) {
+         rs = ia[r0 + lane];
+         re = ia[r0 + lane + 1];
+         so = S_ia[r0 + lane];
+         fl = flg[r0 + lane];
+         t = thr[r0 + lane];
+      }
+      const bool neg = fl & 1, allweak = fl & 2;
+      const int base = __shfl(rs, 0, WAVE);
+      const int end = __shfl(re, nr - 1, WAVE);
+      int coc = 0, csc = 0;
+      for (int wb = base; wb < end; wb += CW_SW) {
+         int jn[U];
+         double an[U];
+#pragma unroll
+         for (int u = 0; u < U; u++) {
+            const int k = min(wb + u * WAVE + lane, end - 1);
+            jn[u] = __builtin_nontemporal_load(ja + k);
+            an[u] = __builtin_nontemporal_load(a + k);
+         }
+         int cv[U];
+#pragma unroll
+         for (int u = 0; u < U; u++) { cv[u] = CF[jn[u]]; }
+#pragma unroll
+         for (int u = 0; u < U; u++) {
+            sa[u * WAVE + lane] = an[u];
+            sj[u * WAVE + lane] = jn[u];
+            sc[u * WAVE + lane] = cv[u];
+         }
+         __builtin_amdgcn_wave_barrier();
+         if (lane < nr) {
+            const int we = min(end, wb + CW_SW);
+            const int s = max(rs + 1, wb), e = min(re, we);
+            for (int k = s; k < e; k++) {
+               const double v = sa[k - wb];
+               if (sc[k - wb] >= 0) {
+                  const int j = sj[k - wb];
+                  if (neg ? (v > 0) : (v < 0)) { OC[rs + coc++] = OCEntry{k, j, v}; }
+                  if (!allweak && (neg ? !(v <= t) : !(v >= t))) { SC_ja[so + csc++] = j; }
+               }
+            }
+         }
+         __builtin_amdgcn_wave_barrier();
+      }
+      if (lane < nr) {
+         SC_cnt[r0 + lane] = csc;
+         OC_cnt[r0 + lane] = coc;
+      }
+   }
+}
+
 // SC and OC lists of one level (storage as in k_c_lists)
 struct CLists
 {
@@ -560,7 +649,8 @@ struct CLists
    }
 };
 
-static void c_lists(const Csr &A, const Csr &S, const int *CF, CLists &L)
+static void c_lists(const Csr &A, const Csr &S, const int *CF, CLists &L, const double *thr,
+                    const int *flg)
 {
    const int n = A.nrows;
    L.sc_cnt = dalloc_n<int>((size_t) std::max(n, 1));
@@ -568,6 +658,15 @@ static void c_lists(const Csr &A, const Csr &S, const int *CF, CLists &L)
    L.sc_ja = dalloc_n<int>((size_t) std::max<long long>(S.nnz, 1));
    L.oc = dalloc_n<OCEntry>((size_t) std::max<long long>(A.nnz, 1));
    if (n == 0) { return; }
+   if (thr && flg) {
+      const long long ng = ((long long) n + WAVE - 1) / WAVE;
+      long long nbs = std::min<long long>((ng + CW_WPB - 1) / CW_WPB, 8192);
+      if (nbs >= 64) { nbs = (nbs + 7) / 8 * 8; }
+      k_c_lists_w<<<(int) std::max<long long>(nbs, 1), CW_WPB * WAVE, 0, stream()>>>(
+          n, A.ia, A.ja, A.a, thr, flg, S.ia, CF, L.sc_cnt, L.sc_ja, L.oc_cnt, L.oc);
+      HX_LAUNCH_CHECK();
+      return;
+   }
    const int K = lanes_per_row(A.nnz, n);
    const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
    dispatch_k(K, [&](auto kc) {
@@ -2650,7 +2749,8 @@ __global__ void k_cf_fix(int n, int *CF)
 }
 
 void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
-                  double trunc_factor, int max_elmts, Csr &P)
+                  double trunc_factor, int max_elmts, Csr &P, const double *s_thr,
+                  const int *s_flg)
 {
    int n = A.nrows;
    TimedLaunch tl("setup_interp", 0.0);
@@ -2670,7 +2770,7 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
       // fast path into raw rows, truncation per thread, leftovers by k_extpi
       stride = max_elmts;
       CLists CL;
-      c_lists(A, S, CF, CL);
+      c_lists(A, S, CF, CL, s_thr, s_flg);
       ExtpiFastArgs f{};
       f.n = n;
       f.A_ia = A.ia; f.A_ja = A.ja; f.A_a = A.a;
