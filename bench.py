@@ -77,6 +77,11 @@ def cpu_baseline(sample_n: int = 100):
     }
 
 
+SMOOTHERS = {"18": "l1-Jacobi", "13/14": "hybrid l1-Gauss-Seidel (13 down, 14 up)",
+             "8": "symmetric hybrid l1-Gauss-Seidel", "3/4": "hybrid Gauss-Seidel (3 down, 4 up)",
+             "6": "symmetric hybrid Gauss-Seidel"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +99,15 @@ def main():
                     help="per-level theta/omega/sweeps from the trained parameter model, kept "
                          "only if they need less work per digit than the defaults on this "
                          "problem (decided once, before the timed steps)")
+    ap.add_argument("--relax", default="18", choices=["18", "13/14", "8", "3/4", "6"],
+                    help="smoother: 18 l1-Jacobi (default); 13/14 hybrid l1 Gauss-Seidel "
+                         "forward down / backward up (the reference's host default); 8 "
+                         "symmetric hybrid l1 Gauss-Seidel; 3/4 and 6 without l1 norms")
+    ap.add_argument("--hgs-rows", type=int, default=64,
+                    help="rows per hybrid Gauss-Seidel partition (automatic partitioning)")
+    ap.add_argument("--hgs-parts", type=int, default=0,
+                    help="fixed number of hybrid Gauss-Seidel partitions on every level "
+                         "(the reference's thread count); 0 = automatic")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: one setup, 3 PCG iterations, "
                          "then 3 copies of a 1 GiB vector (the bytes reference of the counters)")
@@ -114,9 +128,11 @@ def main():
     b = hx.ParVector(n)
     b.set_random(prob["rhs_seed"])        # 2*hypre_Rand()-1 from hypre_SeedRand(seed)
     x = hx.ParVector(n)
+    down, up = (int(args.relax.split("/")[0]), int(args.relax.split("/")[-1]))
     amg = hx.BoomerAMG(coarsen_type=8, interp_type=6, strong_threshold=0.25, max_row_sum=0.9,
-                       P_max_elmts=4, relax_type=18, num_sweeps=1, max_coarse_size=9,
+                       P_max_elmts=4, relax_type=down, num_sweeps=1, max_coarse_size=9,
                        fp32_cycle=1 if args.fp32_cycle else 0)
+    amg.set(relax_up=up, hybrid_partitions=(args.hgs_parts, args.hgs_rows))
     pcg = hx.PCG(amg, tol=1e-8, max_iter=3 if args.probe else 1000, two_norm=1)
     learned_info = None
     if args.learned:
@@ -162,6 +178,11 @@ def main():
         solve_t.append(s2)
     t1 = time.perf_counter()
     ms_spmv, launches, bytes_spmv = hx.get_kernel_timing("spmv_A0")
+    # fine-level smoother, timed in one more (untimed) step
+    hx.kernel_timing(True, only="relax_A0")
+    hx.reset_kernel_timing()
+    step()
+    ms_rlx, n_rlx, bytes_rlx = hx.get_kernel_timing("relax_A0")
     hx.kernel_timing(False)
     sec_per_step = (t1 - t0) / args.steps
 
@@ -194,7 +215,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{prob['desc']} {N}^3 ({n} rows, {nnz} nnz), Dirichlet; PMIS, "
-                        "ext+i (Pmx 4), theta 0.25, max_row_sum 0.9, l1-Jacobi V(1,1), GE coarsest, "
+                        f"ext+i (Pmx 4), theta 0.25, max_row_sum 0.9, {SMOOTHERS[args.relax]} V(1,1), "
+                        "GE coarsest, "
                         f"two-norm PCG to 1e-8, x0 = 0, RHS uniform[-1,1] seed {prob['rhs_seed']}",
             "problem": args.problem,
             "grid": N,
@@ -203,6 +225,9 @@ def main():
             "parallelism": "single GPU",
             "cycle": "fp32 V-cycle in fp64 PCG" if args.fp32_cycle else "fp64",
             "learned_parameters": learned_info,
+            "relax": args.relax,
+            "hybrid_gs_partitions": (args.hgs_parts if args.hgs_parts else
+                                     f"about {args.hgs_rows} rows each") if args.relax != "18" else None,
         },
         "setup_s": round(float(np.median(setup_t)), 5),
         "solve_s": round(float(np.median(solve_t)), 5),
@@ -214,6 +239,8 @@ def main():
         "grid_complexity": round(sum(rows) / rows[0], 6),
         "operator_complexity": round(sum(nnzs) / nnzs[0], 6),
         "spmv_gbs": round(achieved, 1) if achieved else None,
+        "relax_A0": {"avg_launch_ms": round(ms_rlx / max(n_rlx, 1), 5), "launches": n_rlx,
+                     "gbs": round(bytes_rlx / (ms_rlx * 1e-3) / 1e9, 1) if ms_rlx > 0 else None},
         "roofline": {
             "bound": "hbm",
             "kernel": "k_spmv_w<RW,double,double,DOT> (PCG s = A p on the fine level)",
@@ -227,7 +254,7 @@ def main():
             "launches": launches,
         },
     }
-    if not args.no_cpu_baseline and args.problem == "lap27" and N == 200:
+    if not args.no_cpu_baseline and args.problem == "lap27" and N == 200 and args.relax == "18":
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
 

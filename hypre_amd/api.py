@@ -339,6 +339,7 @@ class BoomerAMG:
         "max_levels": "HYPRE_BoomerAMGSetMaxLevels",
         "max_coarse_size": "HYPRE_BoomerAMGSetMaxCoarseSize",
         "relax_wt": "HYPRE_BoomerAMGSetRelaxWt",
+        "outer_wt": "HYPRE_BoomerAMGSetOuterWt",
         "print_level": "HYPRE_BoomerAMGSetPrintLevel",
         "cycle_type": "HYPRE_BoomerAMGSetCycleType",
         "fp32_cycle": "HYPRE_MI355X_BoomerAMGSetCyclePrecision",
@@ -347,7 +348,14 @@ class BoomerAMG:
 
     def set(self, **params):
         for k, v in params.items():
-            call(self._SETTERS[k], self.h, v)
+            if k in ("relax_down", "relax_up", "relax_coarse"):
+                which = {"relax_down": 1, "relax_up": 2, "relax_coarse": 3}[k]
+                call("HYPRE_BoomerAMGSetCycleRelaxType", self.h, v, which)
+            elif k == "hybrid_partitions":
+                parts, rows = v if isinstance(v, tuple) else (v, 0)
+                call("HYPRE_MI355X_BoomerAMGSetHybridPartitions", self.h, parts, rows)
+            else:
+                call(self._SETTERS[k], self.h, v)
         return self
 
     def set_level_params(self, theta=None, omega=None, sweeps=None):
@@ -547,6 +555,18 @@ def spmv(A: HostCsr, x, alpha=1.0, beta=0.0, y=None) -> np.ndarray:
     call("HYPRE_MI355X_SpMV", A.nrows, A.ncols, _ip(A.ia), _ip(A.ja), _dp(A.a), alpha, _dp(x), beta,
          _dp(y))
     return y
+
+
+def hybrid_gs(A: HostCsr, f, u, relax_type: int, parts: int, l1=None, relax_weight=1.0,
+              omega=1.0, zero_guess=False) -> np.ndarray:
+    """One hybrid Gauss-Seidel relaxation on the GPU; returns the new u."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.ascontiguousarray(u, dtype=np.float64).copy()
+    l1a = None if l1 is None else np.ascontiguousarray(l1, dtype=np.float64)
+    l1p = None if l1a is None else _dp(l1a)
+    call("HYPRE_MI355X_HybridGS", A.nrows, _ip(A.ia), _ip(A.ja), _dp(A.a), _dp(f), l1p, _dp(out),
+         relax_type, relax_weight, omega, parts, 1 if zero_guess else 0)
+    return out
 
 
 # ------------------------------------------------------------------ timing

@@ -173,21 +173,31 @@ void AMGSolver::setup(const Csr &A0)
       }
    }
    // l1 norms (par_amg_setup.c:3352-3365) and level vectors
+   // option 4 for the l1 hybrid Gauss-Seidel types (par_amg_setup.c:3297-3317),
+   // option 1 for l1-Jacobi, evaluated after it there and so taking precedence
+   auto l1_gs = [](int t) { return t == 8 || t == 13 || t == 14; };
    for (int j = 0; j < num_levels; j++) {
       Level &lv = L[j];
       const int n = lv.A->nrows;
-      const int rt = (j < num_levels - 1) ? p.relax_type[1] : coarse_relax;
-      const bool need_l1 = (j < num_levels - 1) ? (p.relax_type[1] == 18 || p.relax_type[2] == 18)
-                                                  : (coarse_relax == 18);
-      (void) rt;
-      if (need_l1 && lv.l1) {
+      int opt = 0;
+      if (j < num_levels - 1) {
+         if (p.relax_type[1] == 18 || p.relax_type[2] == 18) { opt = 1; }
+         else if (l1_gs(p.relax_type[1]) || l1_gs(p.relax_type[2])) { opt = 4; }
+      } else {
+         if (coarse_relax == 18) { opt = 1; }
+         else if (l1_gs(coarse_relax)) { opt = 4; }
+      }
+      if (opt == 1 && lv.l1) {
          // computed with the level's strength matrix
          int h = 0;
          d2h(&h, l1_zero_flag(j), sizeof(int));
          if (h) { g_error_arg = 1; set_error(HYPRE_ERROR_ARG, "zero l1 norm"); }
-      } else if (need_l1) {
+      } else if (opt == 1) {
          lv.l1 = dalloc_n<double>((size_t) n);
          l1_norms(*lv.A, lv.l1);
+      } else if (opt == 4) {
+         if (!lv.l1) { lv.l1 = dalloc_n<double>((size_t) n); }
+         l1_norms_diag(*lv.A, lv.l1);
       } else if (lv.l1) {
          dfree(lv.l1);
          lv.l1 = nullptr;
@@ -232,28 +242,37 @@ void AMGSolver::setup(const Csr &A0)
 
 // ================================================================== V-cycle
 // src/parcsr_ls/par_cycle.c:23-889 with cycle_type 1; relax 18 through
-// par_relax.c:1178 (Relax7Jacobi), relax 0 through par_relax.c:180.
-static void relax_any(int type, const Csr &A, const double *f, const double *u, const double *l1,
-                      double w, double *uo, bool zero, const char *cls)
+// par_relax.c:1178 (Relax7Jacobi), relax 0 through par_relax.c:180, relax
+// 3/4/6/8/13/14 through par_relax.c:692 (hybrid Gauss-Seidel).
+void AMGSolver::relax_any(int type, int lev, const Csr &A, const double *f, const double *u,
+                          const double *l1, double *uo, bool zero, const char *cls) const
 {
+   const double w = omega(lev);
    if (type == 18) {
       relax_l1(A, f, u, l1, w, uo, zero, cls);
    } else if (type == 0 || type == 7) {
       if (zero) { fill(A.nrows, 0.0, const_cast<double *>(u)); }
       relax_jacobi(A, f, u, w, uo);
+   } else if (is_hybrid_gs(type)) {
+      relax_hgs(type, A, f, u, l1, w, prm.outer_wt, hgs_parts(A.nrows, prm.hgs_parts, prm.hgs_rows),
+                uo, zero, cls);
    } else {
       fail(HYPRE_ERROR_ARG, "relaxation type not supported");
    }
 }
 
-static void relax_any32(int type, const Csr &A, const float *f, const float *u, const float *l1,
-                        float w, float *uo, bool zero, const char *cls)
+void AMGSolver::relax_any32(int type, int lev, const Csr &A, const float *f, const float *u,
+                            const float *l1, float *uo, bool zero, const char *cls) const
 {
+   const float w = (float) omega(lev);
    if (type == 18) {
       relax_l1_32(A, f, u, l1, w, uo, zero, cls);
    } else if (type == 0 || type == 7) {
       if (zero) { fill32(A.nrows, 0.f, const_cast<float *>(u)); }
       relax_jacobi_32(A, f, u, w, uo);
+   } else if (is_hybrid_gs(type)) {
+      relax_hgs_32(type, A, f, u, l1, w, (float) prm.outer_wt,
+                   hgs_parts(A.nrows, prm.hgs_parts, prm.hgs_rows), uo, zero, cls);
    } else {
       fail(HYPRE_ERROR_ARG, "relaxation type not supported");
    }
@@ -275,7 +294,7 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
       const Csr &A = *lv.A;
       const char *cls = (l == 0) ? "relax_A0" : "relax";
       for (int s = 0; s < sweeps(l, 1); s++) {
-         relax_any(prm.relax_type[1], A, F[l], cur[l], lv.l1, omega(l), spare[l], zero, cls);
+         relax_any(prm.relax_type[1], l, A, F[l], cur[l], lv.l1, spare[l], zero, cls);
          std::swap(cur[l], spare[l]);
          zero = false;
       }
@@ -291,7 +310,7 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
          gselim_solve(coarse_n, Amat, F[l], cur[l]);
       } else {
          for (int s = 0; s < coarse_sweeps; s++) {
-            relax_any(coarse_relax, *lv.A, F[l], cur[l], lv.l1, omega(l), spare[l], zero, "relax");
+            relax_any(coarse_relax, l, *lv.A, F[l], cur[l], lv.l1, spare[l], zero, "relax");
             std::swap(cur[l], spare[l]);
             zero = false;
          }
@@ -303,7 +322,7 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
       spmv(lv.P, 1.0, cur[l + 1], 1.0, cur[l], cur[l], "interp");
       const char *cls = (l == 0) ? "relax_A0" : "relax";
       for (int s = 0; s < sweeps(l, 2); s++) {
-         relax_any(prm.relax_type[2], *lv.A, F[l], cur[l], lv.l1, omega(l), spare[l], false, cls);
+         relax_any(prm.relax_type[2], l, *lv.A, F[l], cur[l], lv.l1, spare[l], false, cls);
          std::swap(cur[l], spare[l]);
       }
    }
@@ -327,7 +346,7 @@ void AMGSolver::cycle32(const double *f, double *u)
       Level &lv = L[l];
       const char *cls = (l == 0) ? "relax_A0" : "relax";
       for (int s = 0; s < sweeps(l, 1); s++) {
-         relax_any32(prm.relax_type[1], lv.A32, lv.f32, cur[l], lv.l1f, (float) omega(l), spare[l], zero, cls);
+         relax_any32(prm.relax_type[1], l, lv.A32, lv.f32, cur[l], lv.l1f, spare[l], zero, cls);
          std::swap(cur[l], spare[l]);
          zero = false;
       }
@@ -342,7 +361,7 @@ void AMGSolver::cycle32(const double *f, double *u)
          gselim_solve32(coarse_n, Amat, lv.f32, cur[l]);
       } else {
          for (int s = 0; s < coarse_sweeps; s++) {
-            relax_any32(coarse_relax, lv.A32, lv.f32, cur[l], lv.l1f, (float) omega(l), spare[l], zero, "relax");
+            relax_any32(coarse_relax, l, lv.A32, lv.f32, cur[l], lv.l1f, spare[l], zero, "relax");
             std::swap(cur[l], spare[l]);
             zero = false;
          }
@@ -353,7 +372,7 @@ void AMGSolver::cycle32(const double *f, double *u)
       spmv32(lv.P32, 1.f, cur[l + 1], 1.f, cur[l], cur[l], "interp");
       const char *cls = (l == 0) ? "relax_A0" : "relax";
       for (int s = 0; s < sweeps(l, 2); s++) {
-         relax_any32(prm.relax_type[2], lv.A32, lv.f32, cur[l], lv.l1f, (float) omega(l), spare[l], false, cls);
+         relax_any32(prm.relax_type[2], l, lv.A32, lv.f32, cur[l], lv.l1f, spare[l], false, cls);
          std::swap(cur[l], spare[l]);
       }
    }
@@ -633,12 +652,15 @@ HYPRE_Int HYPRE_BoomerAMGSetCycleNumSweeps(HYPRE_Solver solver, HYPRE_Int v, HYP
    });
 }
 
-static bool relax_ok(int t) { return t == 18 || t == 0 || t == 7 || t == 9; }
+static bool relax_ok(int t)
+{
+   return t == 18 || t == 0 || t == 7 || t == 9 || is_hybrid_gs(t);
+}
 
 HYPRE_Int HYPRE_BoomerAMGSetRelaxType(HYPRE_Solver solver, HYPRE_Int v)
 {
    return guard([&] {
-      if (v != 18 && v != 0 && v != 7) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "relaxation type not supported"); }
+      if (!relax_ok(v) || v == 9) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "relaxation type not supported"); }
       AMGSolver *s = as_amg(solver);
       for (int k = 0; k < 3; k++) { s->prm.relax_type[k] = v; }
       s->prm.relax_type[3] = 9;   // par_amg.c hypre_BoomerAMGSetRelaxType
@@ -1072,6 +1094,18 @@ HYPRE_Int HYPRE_MI355X_BoomerAMGSetLevelStrongThreshold(HYPRE_Solver solver, HYP
                                                         HYPRE_Real theta)
 {
    return guard([&] { set_level_vec(as_amg(solver)->prm.level_theta, level, theta); });
+}
+
+HYPRE_Int HYPRE_MI355X_BoomerAMGSetHybridPartitions(HYPRE_Solver solver, HYPRE_Int parts,
+                                                    HYPRE_Int rows)
+{
+   return guard([&] {
+      if (parts < 0) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "parts < 0"); }
+      if (parts == 0 && rows < 1) { g_error_arg = 3; fail(HYPRE_ERROR_ARG, "rows < 1"); }
+      AMGSolver *s = as_amg(solver);
+      s->prm.hgs_parts = parts;
+      if (rows > 0) { s->prm.hgs_rows = rows; }
+   });
 }
 
 HYPRE_Int HYPRE_MI355X_BoomerAMGSetLevelRelaxWt(HYPRE_Solver solver, HYPRE_Int level, HYPRE_Real w)

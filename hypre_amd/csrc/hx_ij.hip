@@ -1192,4 +1192,40 @@ HYPRE_Int HYPRE_MI355X_SpMV(HYPRE_Int n, HYPRE_Int ncols, const HYPRE_Int *ia, c
    });
 }
 
+HYPRE_Int HYPRE_MI355X_HybridGS(HYPRE_Int n, const HYPRE_Int *ia, const HYPRE_Int *ja,
+                                const HYPRE_Real *a, const HYPRE_Real *f, const HYPRE_Real *l1,
+                                HYPRE_Real *u, HYPRE_Int relax_type, HYPRE_Real relax_weight,
+                                HYPRE_Real omega, HYPRE_Int parts, HYPRE_Int zero_guess)
+{
+   return guard([&] {
+      ensure_init();
+      if (!is_hybrid_gs(relax_type)) {
+         g_error_arg = 8;
+         fail(HYPRE_ERROR_ARG, "not a hybrid Gauss-Seidel relaxation type");
+      }
+      const bool l1t = relax_type == 8 || relax_type == 13 || relax_type == 14;
+      if (l1t && !l1) { g_error_arg = 6; fail(HYPRE_ERROR_ARG, "l1 norms required"); }
+      if (parts < 1) { g_error_arg = 11; fail(HYPRE_ERROR_ARG, "parts < 1"); }
+      Csr A;
+      csr_from_host(A, n, n, ia, ja, a);
+      const size_t nn = (size_t) std::max(n, 1);
+      double *df = dalloc_n<double>(nn), *du = dalloc_n<double>(nn), *dout = dalloc_n<double>(nn);
+      double *dl1 = nullptr;
+      h2d(df, f, sizeof(double) * (size_t) n);
+      if (!zero_guess) { h2d(du, u, sizeof(double) * (size_t) n); }
+      if (l1t) {
+         dl1 = dalloc_n<double>(nn);
+         h2d(dl1, l1, sizeof(double) * (size_t) n);
+      }
+      relax_hgs(relax_type, A, df, du, dl1, relax_weight, omega, parts, dout, zero_guess != 0,
+                "relax");
+      d2h(u, dout, sizeof(double) * (size_t) n);
+      dfree(df);
+      dfree(du);
+      dfree(dout);
+      if (dl1) { dfree(dl1); }
+      A.free_all();
+   });
+}
+
 } // extern "C"

@@ -129,6 +129,16 @@ void relax_l1_32(const Csr &A, const float *f, const float *u, const float *l1, 
 // weighted Jacobi (relax type 0): u_out = (1-w) u + w (f - sum_{j!=i} a_ij u_j)/a_ii
 void relax_jacobi(const Csr &A, const double *f, const double *u, double w, double *u_out);
 void relax_jacobi_32(const Csr &A, const float *f, const float *u, float w, float *u_out);
+// hybrid Gauss-Seidel, relax types 3/4/6 (l1 == nullptr: divide by the
+// diagonal) and 8/13/14 (l1 = option 4 norms): `parts` row partitions
+// (hypre_partition1D), Gauss-Seidel inside one, the pre-sweep u across them.
+// Out of place: u is not modified; zero_guess reads u as zeros.
+bool is_hybrid_gs(int type);
+int  hgs_parts(int n, int parts, int rows);
+void relax_hgs(int type, const Csr &A, const double *f, const double *u, const double *l1,
+               double w, double omega, int parts, double *u_out, bool zero_guess, const char *cls);
+void relax_hgs_32(int type, const Csr &A, const float *f, const float *u, const float *l1,
+                  float w, float omega, int parts, float *u_out, bool zero_guess, const char *cls);
 // BLAS-1
 double dot(int n, const double *x, const double *y);
 void   dot_dev(int n, const double *x, const double *y, double *out_dev);
@@ -159,6 +169,8 @@ void create_s(const Csr &A, double theta, double max_row_sum, Csr &S, int *st_co
               int *flg_out = nullptr);
 // l1 norms option 1 of hypre_ParCSRComputeL1Norms
 void l1_norms(const Csr &A, double *l1);
+// option 4 on one rank (signed diagonal), for the l1 hybrid Gauss-Seidel types
+void l1_norms_diag(const Csr &A, double *l1);
 // PMIS: measure (optional input: given measure on device, else Park-Miller
 // stream seed 2747), CF out; returns number of rounds
 // st_count: the column counts of S when already known (else counted here)

@@ -75,6 +75,8 @@ struct AMGParams
    int    keep_transpose = 1;
    int    cycle_fp32 = 0;
    int    keep_setup_data = 0;
+   int    hgs_parts = 0;         // hybrid GS partitions on every level (the reference's thread count)
+   int    hgs_rows = 64;         // else partitions of about this many rows
    std::vector<double> level_theta, level_omega;
    std::vector<int>    level_sweeps;
 };
@@ -102,6 +104,10 @@ struct AMGSolver : public hypre_Solver_struct
    double theta(int lev) const;
    double omega(int lev) const;
    int    sweeps(int lev, int k) const;
+   void   relax_any(int type, int lev, const Csr &A, const double *f, const double *u,
+                    const double *l1, double *uo, bool zero, const char *cls) const;
+   void   relax_any32(int type, int lev, const Csr &A, const float *f, const float *u,
+                      const float *l1, float *uo, bool zero, const char *cls) const;
 };
 
 struct PCGSolver : public hypre_Solver_struct

@@ -705,6 +705,54 @@ __global__ void k_l1(int n, const int *__restrict__ ia, const double *__restrict
    }
 }
 
+// src/parcsr_ls/ams.c:649-676 option 4 on one rank (no offd block): |a_ii|
+// of the last entry with j == i (csr_matop.c:2091-2117), sign of a_ii
+// restored (ams.c:776-795); K lanes per row, the largest matching position
+// found by a group max.
+template <int K>
+__global__ void k_l1_diag(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                          const double *__restrict__ a, double *__restrict__ l1,
+                          int *__restrict__ zero_flag)
+{
+   const int gl = threadIdx.x & (K - 1);
+   const int rows_per_block = blockDim.x / K;
+   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < n;
+        rb += (long long) gridDim.x * rows_per_block) {
+      const long long ir = rb + threadIdx.x / K;
+      const bool act = ir < n;
+      const int i = act ? (int) ir : 0;
+      const int b = act ? ia[i] : 0, e = act ? ia[i + 1] : 0;
+      int pos = -1;
+      for (int k = b + gl; k < e; k += K) {
+         if (ja[k] == i) { pos = k; }
+      }
+      for (int o = K / 2; o > 0; o >>= 1) { pos = max(pos, __shfl_xor(pos, o, K)); }
+      if (act && gl == 0) {
+         const double d = pos >= 0 ? a[pos] : 0.0;
+         const double ad = fabs(d);
+         const double s = d < 0.0 ? -ad : ad;
+         if (fabs(s) == 0.0) { *zero_flag = 1; }
+         l1[i] = s;
+      }
+   }
+}
+
+void l1_norms_diag(const Csr &A, double *l1)
+{
+   int *flag = device_flags() + 40;
+   dzero(flag, sizeof(int));
+   const int n = A.nrows;
+   const int K = lanes_per_row(A.nnz, n);
+   const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
+   dispatch_k(K, [&](auto kc) {
+      k_l1_diag<decltype(kc)::value><<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, l1, flag);
+   });
+   HX_LAUNCH_CHECK();
+   int h = 0;
+   d2h(&h, flag, sizeof(int));
+   if (h) { g_error_arg = 1; set_error(HYPRE_ERROR_ARG, "zero l1 norm"); }
+}
+
 void l1_norms(const Csr &A, double *l1)
 {
    int *flag = device_flags() + 40;

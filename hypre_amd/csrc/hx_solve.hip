@@ -430,6 +430,172 @@ void relax_jacobi_32(const Csr &A, const float *f, const float *u, float w, floa
    launch_relax_jac<float, float>(A, A.a32, f, u, w, u_out);
 }
 
+// ---------------------------------------------------------------- hybrid GS
+// src/parcsr_ls/par_relax.c:692-953 (hypre_BoomerAMGRelaxHybridGaussSeidel_core)
+// with the row kernels of par_relax.h:13-455.  The reference's threads become
+// `parts` contiguous row partitions (utilities.c:99 hypre_partition1D), one
+// lane each; a lane sweeps its rows in order, taking values of its own
+// partition that this sweep already updated from u_out (the previous row
+// from a register) and every other value from u, the copy of the iterate
+// before the sweep (the reference's Vtemp).  Folds keep the storage order.
+bool is_hybrid_gs(int t) { return t == 3 || t == 4 || t == 6 || t == 8 || t == 13 || t == 14; }
+
+int hgs_parts(int n, int parts, int rows)
+{
+   if (parts > 0) { return parts; }
+   rows = std::max(rows, 1);
+   if (n <= rows) { return 1; }
+   return (int) (((long long) n + rows - 1) / rows);
+}
+
+constexpr int HGS_U = 8;
+
+// mode: 0 forward, 1 backward, 2 forward then backward
+template <typename V, typename X, bool NS, bool SKIP, bool ZERO>
+__global__ __launch_bounds__(BS) void k_hgs(int n, int parts, const int *__restrict__ ia,
+                                            const int *__restrict__ ja, const V *__restrict__ a,
+                                            const X *__restrict__ f, const X *__restrict__ l1,
+                                            const X *__restrict__ uo, X *un, int mode, X w,
+                                            X omega, X prod, X omw)
+{
+   const long long pl = (long long) blockIdx.x * BS + threadIdx.x;
+   if (pl >= parts) { return; }
+   const int p = (int) pl;
+   int ns, ne;
+   if (parts == 1) {
+      ns = 0;
+      ne = n;
+   } else {
+      const int size = n / parts, rest = n - size * parts;
+      if (p < rest) {
+         ns = p * (size + 1);
+         ne = ns + size + 1;
+      } else {
+         ns = p * size + rest;
+         ne = ns + size;
+      }
+   }
+   if (ns >= ne) { return; }
+   const int nsw = (mode == 2) ? 2 : 1;
+   for (int sw = 0; sw < nsw; sw++) {
+      const int ord = (mode == 0) ? 1 : (mode == 1) ? -1 : (sw == 0 ? 1 : -1);
+      const bool second = sw == 1;
+      const int iend = ord > 0 ? ne : ns - 1;
+      int prev = -1;
+      X prev_val = (X) 0;
+      for (int i = ord > 0 ? ns : ne - 1; i != iend; i += ord) {
+         const int rs = ia[i], re = ia[i + 1];
+         const X d = l1 ? l1[i] : (re > rs ? (X) a[rs] : (X) 0);
+         const X old_i = ZERO ? (X) 0 : uo[i];
+         const X cur_i = second ? un[i] : old_i;
+         X nv = cur_i;
+         if (d != (X) 0) {
+            X res = f[i], res0 = (X) 0, res2 = (X) 0;
+            for (int k = rs + (SKIP ? 1 : 0); k < re; k += HGS_U) {
+               int jv[HGS_U];
+               V av[HGS_U];
+#pragma unroll
+               for (int u = 0; u < HGS_U; u++) {
+                  const int q = min(k + u, re - 1);
+                  jv[u] = ja[q];
+                  av[u] = a[q];
+               }
+               X xo[HGS_U], xn[HGS_U];
+#pragma unroll
+               for (int u = 0; u < HGS_U; u++) {
+                  const int j = jv[u];
+                  const bool in = j >= ns && j < ne;
+                  const bool upd = in && (second || (ord > 0 ? j < i : j > i));
+                  xo[u] = ZERO ? (X) 0 : uo[j];
+                  xn[u] = un[upd ? j : i];
+               }
+#pragma unroll
+               for (int u = 0; u < HGS_U; u++) {
+                  if (k + u < re) {
+                     const int j = jv[u];
+                     const bool in = j >= ns && j < ne;
+                     const bool upd = in && (second || (ord > 0 ? j < i : j > i));
+                     const X xc = upd ? (j == prev ? prev_val : xn[u]) : xo[u];
+                     const X av_ = (X) av[u];
+                     if (NS) {
+                        res -= av_ * xc;
+                     } else if (in) {
+                        res0 -= av_ * xc;
+                        res2 += av_ * xo[u];
+                     } else {
+                        res -= av_ * xo[u];
+                     }
+                  }
+               }
+            }
+            if (NS) {
+               nv = SKIP ? res / d : cur_i + res / d;
+            } else {
+               const X base = SKIP ? cur_i * prod : cur_i;
+               nv = base + w * (omega * res + res0 + omw * res2) / d;
+            }
+         }
+         un[i] = nv;
+         prev = i;
+         prev_val = nv;
+      }
+   }
+}
+
+template <typename V, typename X>
+static void launch_hgs(int type, const Csr &A, const V *vals, const X *f, const X *u, const X *l1,
+                       X w, X omega, X prod, X omw, int parts, X *uo, bool zero, const char *cls)
+{
+   if (A.nrows <= 0) { return; }
+   if (!is_hybrid_gs(type)) { fail(HYPRE_ERROR_ARG, "not a hybrid Gauss-Seidel type"); }
+   const bool l1t = (type == 8 || type == 13 || type == 14);
+   const bool ns = (w == (X) 1 && omega == (X) 1);
+   // par_relax.c:946-989, 1160-1171 (3/4/6: Skip_diag 1, no l1 norms) and
+   // 1266-1383 (8/13/14: Skip_diag 0 only without weights)
+   const bool skip = l1t ? !ns : true;
+   const int mode = (type == 6 || type == 8) ? 2 : (type == 4 || type == 14) ? 1 : 0;
+   if (l1t && !l1) { fail(HYPRE_ERROR_GENERIC, "hybrid l1 Gauss-Seidel without l1 norms"); }
+   parts = std::max(parts, 1);
+   const double xs = sizeof(X);
+   const double bytes = (double) A.nnz * (4.0 + sizeof(V) + (zero ? 0.0 : xs)) +
+                        4.0 * (A.nrows + 1) + xs * (zero ? 3.0 : 4.0) * A.nrows;
+   TimedLaunch tl(cls, bytes);
+   hipStream_t s = stream();
+   const int nb = (int) (((long long) parts + BS - 1) / BS);
+   const X *l1p = l1t ? l1 : nullptr;
+   auto go = [&](auto nsc, auto skc, auto zc) {
+      k_hgs<V, X, decltype(nsc)::value, decltype(skc)::value, decltype(zc)::value>
+         <<<nb, BS, 0, s>>>(A.nrows, parts, A.ia, A.ja, vals, f, l1p, u, uo, mode, w, omega, prod,
+                            omw);
+   };
+   using T = std::true_type;
+   using F = std::false_type;
+   auto by_zero = [&](auto nsc, auto skc) {
+      if (zero) { go(nsc, skc, T{}); } else { go(nsc, skc, F{}); }
+   };
+   if (ns) {
+      if (skip) { by_zero(T{}, T{}); } else { by_zero(T{}, F{}); }
+   } else {
+      if (skip) { by_zero(F{}, T{}); } else { by_zero(F{}, F{}); }
+   }
+   HX_LAUNCH_CHECK();
+}
+
+void relax_hgs(int type, const Csr &A, const double *f, const double *u, const double *l1,
+               double w, double omega, int parts, double *u_out, bool zero_guess, const char *cls)
+{
+   // prod and 1 - omega as the reference forms them on the host
+   launch_hgs<double, double>(type, A, A.a, f, u, l1, w, omega, 1.0 - w * omega, 1.0 - omega,
+                              parts, u_out, zero_guess, cls);
+}
+
+void relax_hgs_32(int type, const Csr &A, const float *f, const float *u, const float *l1,
+                  float w, float omega, int parts, float *u_out, bool zero_guess, const char *cls)
+{
+   launch_hgs<float, float>(type, A, A.a32, f, u, l1, w, omega, 1.f - w * omega, 1.f - omega,
+                            parts, u_out, zero_guess, cls);
+}
+
 // ---------------------------------------------------------------- BLAS-1
 static int blas_grid(int n)
 {

@@ -74,6 +74,7 @@ SIGNATURES = {
     "HYPRE_BoomerAMGSetPMaxElmts": (I, [P, I]),
     "HYPRE_BoomerAMGSetRelaxType": (I, [P, I]),
     "HYPRE_BoomerAMGSetCycleRelaxType": (I, [P, I, I]),
+    "HYPRE_BoomerAMGSetOuterWt": (I, [P, D]),
     "HYPRE_BoomerAMGSetNumSweeps": (I, [P, I]),
     "HYPRE_BoomerAMGSetCycleNumSweeps": (I, [P, I, I]),
     "HYPRE_BoomerAMGSetMaxIter": (I, [P, I]),
@@ -102,6 +103,7 @@ SIGNATURES = {
     "HYPRE_MI355X_BoomerAMGSetCyclePrecision": (I, [P, I]),
     "HYPRE_MI355X_BoomerAMGSetLevelStrongThreshold": (I, [P, I, D]),
     "HYPRE_MI355X_BoomerAMGSetLevelRelaxWt": (I, [P, I, D]),
+    "HYPRE_MI355X_BoomerAMGSetHybridPartitions": (I, [P, I, I]),
     "HYPRE_MI355X_BoomerAMGSetLevelNumSweeps": (I, [P, I, I]),
     "HYPRE_MI355X_BoomerAMGClearLevelParams": (I, [P]),
     "HYPRE_MI355X_BoomerAMGSetKeepSetupData": (I, [P, I]),
@@ -118,6 +120,7 @@ SIGNATURES = {
     "HYPRE_MI355X_ExtPIInterp": (I, [I, IP, IP, DP, IP, IP, IP, D, I, IP, IP, IP, IP, DP]),
     "HYPRE_MI355X_RAP": (I, [I, I, IP, IP, DP, IP, IP, DP, IP, IP, IP, DP]),
     "HYPRE_MI355X_SpMV": (I, [I, I, IP, IP, DP, D, DP, D, DP]),
+    "HYPRE_MI355X_HybridGS": (I, [I, IP, IP, DP, DP, DP, DP, I, D, D, I, I]),
     "HYPRE_MI355X_ParCSRMatrixGetDevice": (I, [P, IP, IP, PP, PP, PP]),
     "HYPRE_MI355X_ParCSRMatrixCopyToHost": (I, [P, IP, IP, DP]),
     "HYPRE_MI355X_ParVectorCopyFromHost": (I, [P, DP]),

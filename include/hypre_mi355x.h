@@ -33,6 +33,14 @@ HYPRE_Int HYPRE_MI355X_BoomerAMGSetLevelRelaxWt(HYPRE_Solver solver, HYPRE_Int l
 HYPRE_Int HYPRE_MI355X_BoomerAMGSetLevelNumSweeps(HYPRE_Solver solver, HYPRE_Int level,
                                                   HYPRE_Int sweeps);
 HYPRE_Int HYPRE_MI355X_BoomerAMGClearLevelParams(HYPRE_Solver solver);
+/* row partitions of the hybrid Gauss-Seidel smoothers (relax 3/4/6/8/13/14),
+ * which relax each partition in Gauss-Seidel order and couple partitions
+ * through the iterate before the sweep, like the reference's OpenMP threads:
+ * parts > 0 uses that many partitions on every level (the reference run with
+ * `parts` threads); parts == 0 uses partitions of about `rows` rows (default
+ * 64) so that every level keeps many partitions in flight. */
+HYPRE_Int HYPRE_MI355X_BoomerAMGSetHybridPartitions(HYPRE_Solver solver, HYPRE_Int parts,
+                                                    HYPRE_Int rows);
 /* keep the strength pattern S and the PMIS measure of every level for
  * inspection (off by default; they are freed during setup otherwise) */
 HYPRE_Int HYPRE_MI355X_BoomerAMGSetKeepSetupData(HYPRE_Solver solver, HYPRE_Int keep);
@@ -87,6 +95,15 @@ HYPRE_Int HYPRE_MI355X_RAP(HYPRE_Int n, HYPRE_Int nc, const HYPRE_Int *ia, const
 HYPRE_Int HYPRE_MI355X_SpMV(HYPRE_Int n, HYPRE_Int ncols, const HYPRE_Int *ia,
                             const HYPRE_Int *ja, const HYPRE_Real *a, HYPRE_Real alpha,
                             const HYPRE_Real *x, HYPRE_Real beta, HYPRE_Real *y);
+
+/* one hybrid Gauss-Seidel relaxation (relax_type 3/4/6/8/13/14; l1 norms
+ * required for 8/13/14) with `parts` row partitions, u updated in place;
+ * zero_guess != 0 reads u as zeros.  src/parcsr_ls/par_relax.c:692 with the
+ * reference's thread count equal to `parts`. */
+HYPRE_Int HYPRE_MI355X_HybridGS(HYPRE_Int n, const HYPRE_Int *ia, const HYPRE_Int *ja,
+                                const HYPRE_Real *a, const HYPRE_Real *f, const HYPRE_Real *l1,
+                                HYPRE_Real *u, HYPRE_Int relax_type, HYPRE_Real relax_weight,
+                                HYPRE_Real omega, HYPRE_Int parts, HYPRE_Int zero_guess);
 
 /* ---- ParCSR access ------------------------------------------------------ */
 /* borrowed device pointers of the (single-rank) CSR block */

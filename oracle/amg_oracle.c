@@ -866,6 +866,122 @@ int orc_l1_norms(const orc_csr *A, double *l1)
    return bad ? 4 : 0;
 }
 
+/* hypre_ParCSRComputeL1Norms option 4 on one rank (src/parcsr_ls/ams.c:649-676
+ * and 776-795): |a_ii| (hypre_CSRMatrixExtractDiagonalHost type 1, last
+ * entry with j == i, src/seq_mv/csr_matop.c:2091-2117); the truncation of
+ * Remark 6.2 leaves it unchanged without an offd block; the sign of a_ii is
+ * restored. */
+int orc_l1_norms_diag(const orc_csr *A, double *l1)
+{
+   int i, j, bad = 0;
+   for (i = 0; i < A->nrows; i++)
+   {
+      double d = 0.0, ad = 0.0;
+      for (j = A->ia[i]; j < A->ia[i + 1]; j++)
+      {
+         if (A->ja[j] == i) { d = A->a[j]; ad = fabs(A->a[j]); }
+      }
+      l1[i] = (d < 0.0) ? -ad : ad;
+      if (fabs(l1[i]) == 0.0) { bad = 1; }
+   }
+   return bad ? 4 : 0;
+}
+
+/* hypre_partition1D, src/utilities/utilities.c:99-125 */
+void orc_partition1d(int n, int p, int j, int *s, int *e)
+{
+   int size, rest;
+   if (p == 1) { *s = 0; *e = n; return; }
+   size = n / p;
+   rest = n - size * p;
+   if (j < rest) { *s = j * (size + 1); *e = (j + 1) * (size + 1); }
+   else { *s = j * size + rest; *e = (j + 1) * size + rest; }
+}
+
+/* Number of row partitions of the hybrid Gauss-Seidel smoother on a level of
+ * n rows: `parts` when positive (the reference's thread count), otherwise
+ * the automatic rule of the library: partitions of about `rows` rows. */
+int orc_hgs_parts(int n, int parts, int rows)
+{
+   if (parts > 0) { return parts; }
+   if (rows < 1) { rows = 1; }
+   if (n <= rows) { return 1; }
+   return (n + rows - 1) / rows;
+}
+
+/* One hybrid Gauss-Seidel relaxation on a single rank whose rows are split
+ * into `parts` contiguous partitions (hypre_partition1D): Gauss-Seidel in
+ * row order inside a partition, values of other partitions taken from the
+ * copy of u made before the sweep.  src/parcsr_ls/par_relax.c:692-953
+ * (hypre_BoomerAMGRelaxHybridGaussSeidel_core; with one partition the
+ * unthreaded loops) and the row kernels of src/parcsr_ls/par_relax.h:13-106
+ * (NS), 109-222 (NSThreads), 225-331 (scaled), 334-455 (scaled threads).
+ * direction 1 forward, -1 backward; symm: forward then backward against the
+ * same copy.  l1 == NULL divides by the first entry of the row (the
+ * diagonal), as the reference's A_diag_data[A_diag_i[i]]. */
+void orc_hybrid_gs(const orc_csr *A, const double *f, const double *l1, double *u,
+                   int parts, int direction, int symm, int skip_diag,
+                   double relax_weight, double omega)
+{
+   const int n = A->nrows;
+   const int non_scale = (relax_weight == 1.0 && omega == 1.0);
+   const double prod = 1.0 - relax_weight * omega;
+   const double one_minus_omega = 1.0 - omega;
+   const int nsw = symm ? 2 : 1;
+   double *v = (double *) xmalloc(sizeof(double) * (size_t) (n > 0 ? n : 1));
+   int p, sw;
+   memcpy(v, u, sizeof(double) * (size_t) n);
+   if (parts < 1) { parts = 1; }
+   for (p = 0; p < parts; p++)
+   {
+      int ns, ne;
+      orc_partition1d(n, parts, p, &ns, &ne);
+      for (sw = 0; sw < nsw; sw++)
+      {
+         const int ord = (nsw == 1) ? (direction > 0 ? 1 : -1) : (sw == 0 ? 1 : -1);
+         const int ib = ord > 0 ? ns : ne - 1, ie = ord > 0 ? ne : ns - 1;
+         int i;
+         for (i = ib; i != ie; i += ord)
+         {
+            const double d = l1 ? l1[i] : A->a[A->ia[i]];
+            int jj;
+            if (d == 0.0) { continue; }
+            if (non_scale)
+            {
+               double res = f[i];
+               for (jj = A->ia[i] + skip_diag; jj < A->ia[i + 1]; jj++)
+               {
+                  const int c = A->ja[jj];
+                  res -= A->a[jj] * ((c >= ns && c < ne) ? u[c] : v[c]);
+               }
+               if (skip_diag) { u[i] = res / d; }
+               else { u[i] += res / d; }
+            }
+            else
+            {
+               double res = f[i], res0 = 0.0, res2 = 0.0;
+               for (jj = A->ia[i] + skip_diag; jj < A->ia[i + 1]; jj++)
+               {
+                  const int c = A->ja[jj];
+                  if (c >= ns && c < ne)
+                  {
+                     res0 -= A->a[jj] * u[c];
+                     res2 += A->a[jj] * v[c];
+                  }
+                  else
+                  {
+                     res -= A->a[jj] * v[c];
+                  }
+               }
+               if (skip_diag) { u[i] *= prod; }
+               u[i] += relax_weight * (omega * res + res0 + one_minus_omega * res2) / d;
+            }
+         }
+      }
+   }
+   free(v);
+}
+
 /* ======================================================================
  * Vector kernels
  * ====================================================================== */
@@ -930,6 +1046,10 @@ void orc_amg_default_params(orc_amg_params *p)
    p->coarse_relax_type = 9;
    p->relax_weight = 1.0;
    p->num_sweeps = 1;
+   p->relax_up_type = -1;
+   p->outer_weight = 1.0;
+   p->hgs_parts = 0;
+   p->hgs_rows = 64;
 }
 
 static void csr_copy(const orc_csr *A, orc_csr *B)
@@ -943,6 +1063,8 @@ static void csr_copy(const orc_csr *A, orc_csr *B)
    memcpy(B->ja, A->ja, sizeof(int) * (size_t) nnz);
    memcpy(B->a, A->a, sizeof(double) * (size_t) nnz);
 }
+
+static int orc_is_l1_gs(int t) { return t == 8 || t == 13 || t == 14; }
 
 orc_amg *orc_amg_setup(const orc_csr *A0, const orc_amg_params *prm)
 {
@@ -1003,17 +1125,30 @@ orc_amg *orc_amg_setup(const orc_csr *A0, const orc_amg_params *prm)
          H->coarse_relax = prm->relax_type;
       }
    }
-   /* l1 norms, par_amg_setup.c:3352-3365 */
+   /* l1 norms, par_amg_setup.c:3297-3365: option 4 for the l1 hybrid
+    * Gauss-Seidel types, option 1 for l1-Jacobi (evaluated last there, so it
+    * wins when both kinds are used) */
    for (j = 0; j < H->num_levels; j++)
    {
-      int need = (j < H->num_levels - 1) ? (prm->relax_type == 18)
-                                           : (H->coarse_relax == 18);
+      const int up = prm->relax_up_type >= 0 ? prm->relax_up_type : prm->relax_type;
+      int opt = 0;
+      if (j < H->num_levels - 1)
+      {
+         if (prm->relax_type == 18 || up == 18) { opt = 1; }
+         else if (orc_is_l1_gs(prm->relax_type) || orc_is_l1_gs(up)) { opt = 4; }
+      }
+      else
+      {
+         if (H->coarse_relax == 18) { opt = 1; }
+         else if (orc_is_l1_gs(H->coarse_relax)) { opt = 4; }
+      }
       H->F[j] = (double *) xcalloc((size_t) H->A[j].nrows, sizeof(double));
       H->U[j] = (double *) xcalloc((size_t) H->A[j].nrows, sizeof(double));
-      if (need)
+      if (opt)
       {
          H->l1[j] = (double *) xmalloc(sizeof(double) * (size_t) H->A[j].nrows);
-         orc_l1_norms(&H->A[j], H->l1[j]);
+         if (opt == 1) { orc_l1_norms(&H->A[j], H->l1[j]); }
+         else { orc_l1_norms_diag(&H->A[j], H->l1[j]); }
       }
    }
    H->Vtemp = (double *) xcalloc((size_t) H->A[0].nrows, sizeof(double));
@@ -1046,13 +1181,27 @@ int orc_amg_coarse_relax(const orc_amg *H) { return H->coarse_relax; }
 
 /* relax_type 18: src/parcsr_ls/par_relax.c:1178-1240 Relax7Jacobi
  *   Vtemp = w*(f - A u);  u += Vtemp ./ l1
- * relax_type 0: par_relax.c:180-320 weighted Jacobi (Skip_diag = 1). */
+ * relax_type 0: par_relax.c:180-320 weighted Jacobi (Skip_diag = 1).
+ * relax_type 3/4/6 (par_relax.c:946-989, 1160-1171) and 8/13/14
+ * (par_relax.c:1266-1383): hybrid Gauss-Seidel, l1 norms for 8/13/14 with
+ * Skip_diag = 0 when weight and omega are both 1. */
 static void relax(orc_amg *H, int lev, int type, const double *f, double *u)
 {
    const orc_csr *A = &H->A[lev];
    int n = A->nrows, i, j;
    double w = H->prm.relax_weight;
    double *v = H->Vtemp;
+   if (type == 3 || type == 4 || type == 6 || type == 8 || type == 13 || type == 14)
+   {
+      const double om = H->prm.outer_weight;
+      const int l1t = (type == 8 || type == 13 || type == 14);
+      const int skip = l1t ? ((w == 1.0 && om == 1.0) ? 0 : 1) : 1;
+      const int dir = (type == 4 || type == 14) ? -1 : 1;
+      const int symm = (type == 6 || type == 8);
+      orc_hybrid_gs(A, f, l1t ? H->l1[lev] : NULL, u,
+                    orc_hgs_parts(n, H->prm.hgs_parts, H->prm.hgs_rows), dir, symm, skip, w, om);
+      return;
+   }
    if (type == 18)
    {
       for (i = 0; i < n; i++)
@@ -1165,7 +1314,11 @@ int orc_amg_cycle(orc_amg *H, const double *f, double *u)
    {
       /* interpolation u += P e, par_cycle.c:820-840 */
       orc_matvec(1.0, &H->P[lev], H->U[lev + 1], 1.0, H->U[lev], H->U[lev]);
-      for (s = 0; s < nsw; s++) { relax(H, lev, H->prm.relax_type, H->F[lev], H->U[lev]); }
+      for (s = 0; s < nsw; s++)
+      {
+         relax(H, lev, H->prm.relax_up_type >= 0 ? H->prm.relax_up_type : H->prm.relax_type,
+               H->F[lev], H->U[lev]);
+      }
    }
    memcpy(u, H->U[0], sizeof(double) * (size_t) H->A[0].nrows);
    return 0;

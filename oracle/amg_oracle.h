@@ -87,6 +87,18 @@ int orc_transpose(const orc_csr *P, orc_csr *R);
 int orc_rap(const orc_csr *R, const orc_csr *A, const orc_csr *P, orc_csr *Ac);
 /* src/parcsr_ls/ams.c:517 hypre_ParCSRComputeL1Norms option 1 */
 int orc_l1_norms(const orc_csr *A, double *l1);
+/* src/parcsr_ls/ams.c:649 option 4 on one rank (signed diagonal) */
+int orc_l1_norms_diag(const orc_csr *A, double *l1);
+
+/* ---------------- hybrid Gauss-Seidel (relax 3/4/6/8/13/14) ------------- */
+/* src/utilities/utilities.c:99 hypre_partition1D */
+void orc_partition1d(int n, int p, int j, int *s, int *e);
+int  orc_hgs_parts(int n, int parts, int rows);
+/* src/parcsr_ls/par_relax.c:692 hypre_BoomerAMGRelaxHybridGaussSeidel_core
+ * with `parts` partitions (the reference's thread count) */
+void orc_hybrid_gs(const orc_csr *A, const double *f, const double *l1, double *u,
+                   int parts, int direction, int symm, int skip_diag,
+                   double relax_weight, double omega);
 
 /* ---------------- solve side -------------------------------------------- */
 /* y = alpha*A*x + beta*b (src/seq_mv/csr_matvec.c:22) */
@@ -110,6 +122,10 @@ typedef struct
    int    coarse_relax_type;  /* 9 (Gaussian elimination) */
    double relax_weight;       /* 1.0 */
    int    num_sweeps;         /* 1 */
+   int    relax_up_type;      /* -1: relax_type on the up cycle too */
+   double outer_weight;       /* 1.0 (hypre's omega) */
+   int    hgs_parts;          /* hybrid GS partitions; 0: about hgs_rows rows each */
+   int    hgs_rows;           /* 64 */
 } orc_amg_params;
 
 void orc_amg_default_params(orc_amg_params *p);

@@ -29,7 +29,9 @@ class _Params(C.Structure):
                 ("trunc_factor", C.c_double), ("P_max_elmts", C.c_int),
                 ("max_levels", C.c_int), ("max_coarse_size", C.c_int),
                 ("relax_type", C.c_int), ("coarse_relax_type", C.c_int),
-                ("relax_weight", C.c_double), ("num_sweeps", C.c_int)]
+                ("relax_weight", C.c_double), ("num_sweeps", C.c_int),
+                ("relax_up_type", C.c_int), ("outer_weight", C.c_double),
+                ("hgs_parts", C.c_int), ("hgs_rows", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -68,6 +70,11 @@ def lib():
         L.orc_transpose.argtypes = [P(_Csr), P(_Csr)]
         L.orc_rap.argtypes = [P(_Csr), P(_Csr), P(_Csr), P(_Csr)]
         L.orc_l1_norms.argtypes = [P(_Csr), P(C.c_double)]
+        L.orc_l1_norms_diag.argtypes = [P(_Csr), P(C.c_double)]
+        L.orc_partition1d.argtypes = [C.c_int, C.c_int, C.c_int, P(C.c_int), P(C.c_int)]
+        L.orc_hgs_parts.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.orc_hybrid_gs.argtypes = [P(_Csr), P(C.c_double), P(C.c_double), P(C.c_double),
+                                    C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
         L.orc_matvec.argtypes = [C.c_double, P(_Csr), P(C.c_double), C.c_double,
                                  P(C.c_double), P(C.c_double)]
         L.orc_dot.argtypes = [C.c_int, P(C.c_double), P(C.c_double)]
@@ -259,6 +266,39 @@ def l1_norms(A: Csr) -> np.ndarray:
     out = np.empty(A.nrows, dtype=np.float64)
     ca = A.to_c()
     lib().orc_l1_norms(C.byref(ca), _dp(out))
+    return out
+
+
+def l1_norms_diag(A: Csr) -> np.ndarray:
+    out = np.empty(A.nrows, dtype=np.float64)
+    ca = A.to_c()
+    lib().orc_l1_norms_diag(C.byref(ca), _dp(out))
+    return out
+
+
+def partition1d(n: int, p: int, j: int):
+    s, e = C.c_int(), C.c_int()
+    lib().orc_partition1d(n, p, j, C.byref(s), C.byref(e))
+    return s.value, e.value
+
+
+def hgs_parts(n: int, parts: int = 0, rows: int = 64) -> int:
+    return lib().orc_hgs_parts(n, parts, rows)
+
+
+def hybrid_gs(A: Csr, f, u, parts: int, direction: int = 1, symm: bool = False,
+              skip_diag: int = 0, l1=None, relax_weight: float = 1.0,
+              omega: float = 1.0) -> np.ndarray:
+    """One hybrid Gauss-Seidel relaxation; returns the new u (input kept)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.ascontiguousarray(u, dtype=np.float64).copy()
+    ca = A.to_c()
+    l1p = C.POINTER(C.c_double)()
+    if l1 is not None:
+        l1 = np.ascontiguousarray(l1, dtype=np.float64)
+        l1p = _dp(l1)
+    lib().orc_hybrid_gs(C.byref(ca), _dp(f), l1p, _dp(out), parts, direction, int(symm),
+                        skip_diag, relax_weight, omega)
     return out
 
 
