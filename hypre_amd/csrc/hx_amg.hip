@@ -45,6 +45,7 @@ void AMGSolver::free_hierarchy()
       l.P.free_all();
       l.R.free_all();
       l.S.free_all();
+      l.Arev.free_all();
       dfree(l.A32.a32); dfree(l.P32.a32); dfree(l.R32.a32);
       dfree(l.CF); dfree(l.measure); dfree(l.l1); dfree(l.l1f);
       dfree(l.f); dfree(l.u); dfree(l.ut); dfree(l.r);
@@ -223,6 +224,23 @@ void AMGSolver::setup(const Csr &A0)
          lv.ut32 = dalloc_n<float>((size_t) n);
          lv.r32 = dalloc_n<float>((size_t) n);
       }
+      const bool hgs = (j < num_levels - 1)
+                          ? (is_hybrid_gs(p.relax_type[1]) || is_hybrid_gs(p.relax_type[2]))
+                          : (coarse_relax != 9 || !Amat) && is_hybrid_gs(coarse_relax);
+      lv.max_row = hgs ? csr_max_row(*lv.A) : -1;
+      // row-reversed operator for backward / symmetric hybrid Gauss-Seidel
+      const bool rev = (j < num_levels - 1)
+                          ? (hgs_needs_reverse(p.relax_type[1]) || hgs_needs_reverse(p.relax_type[2]))
+                          : (coarse_relax != 9 || !Amat) && hgs_needs_reverse(coarse_relax);
+      if (rev) {
+         if (p.cycle_fp32) {
+            Csr tmp = *lv.A;          // shares A's arrays; adds the FP32 values
+            tmp.a32 = lv.A32.a32;
+            csr_reverse_rows(tmp, lv.Arev);
+         } else {
+            csr_reverse_rows(*lv.A, lv.Arev);
+         }
+      }
    }
    sync();
    setup_done = true;
@@ -254,8 +272,8 @@ void AMGSolver::relax_any(int type, int lev, const Csr &A, const double *f, cons
       if (zero) { fill(A.nrows, 0.0, const_cast<double *>(u)); }
       relax_jacobi(A, f, u, w, uo);
    } else if (is_hybrid_gs(type)) {
-      relax_hgs(type, A, f, u, l1, w, prm.outer_wt, hgs_parts(A.nrows, prm.hgs_parts, prm.hgs_rows),
-                uo, zero, cls);
+      relax_hgs(type, A, &L[lev].Arev, L[lev].max_row, f, u, l1, w, prm.outer_wt,
+                hgs_parts(A.nrows, prm.hgs_parts, prm.hgs_rows), uo, zero, cls);
    } else {
       fail(HYPRE_ERROR_ARG, "relaxation type not supported");
    }
@@ -271,7 +289,7 @@ void AMGSolver::relax_any32(int type, int lev, const Csr &A, const float *f, con
       if (zero) { fill32(A.nrows, 0.f, const_cast<float *>(u)); }
       relax_jacobi_32(A, f, u, w, uo);
    } else if (is_hybrid_gs(type)) {
-      relax_hgs_32(type, A, f, u, l1, w, (float) prm.outer_wt,
+      relax_hgs_32(type, A, &L[lev].Arev, L[lev].max_row, f, u, l1, w, (float) prm.outer_wt,
                    hgs_parts(A.nrows, prm.hgs_parts, prm.hgs_rows), uo, zero, cls);
    } else {
       fail(HYPRE_ERROR_ARG, "relaxation type not supported");

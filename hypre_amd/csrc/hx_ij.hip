@@ -1217,8 +1217,11 @@ HYPRE_Int HYPRE_MI355X_HybridGS(HYPRE_Int n, const HYPRE_Int *ia, const HYPRE_In
          dl1 = dalloc_n<double>(nn);
          h2d(dl1, l1, sizeof(double) * (size_t) n);
       }
-      relax_hgs(relax_type, A, df, du, dl1, relax_weight, omega, parts, dout, zero_guess != 0,
-                "relax");
+      Csr Arev;
+      if (hgs_needs_reverse(relax_type)) { csr_reverse_rows(A, Arev); }
+      relax_hgs(relax_type, A, &Arev, -1, df, du, dl1, relax_weight, omega, parts, dout,
+                zero_guess != 0, "relax");
+      Arev.free_all();
       d2h(u, dout, sizeof(double) * (size_t) n);
       dfree(df);
       dfree(du);

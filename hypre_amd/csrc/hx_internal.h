@@ -109,7 +109,6 @@ struct Csr
 };
 
 void csr_copy(const Csr &src, Csr &dst);
-void csr_make_fp32(Csr &A);
 void csr_to_host(const Csr &A, std::vector<int> &ia, std::vector<int> &ja, std::vector<double> &a);
 void csr_from_host(Csr &A, int n, int ncols, const int *ia, const int *ja, const double *a);
 
@@ -133,12 +132,19 @@ void relax_jacobi_32(const Csr &A, const float *f, const float *u, float w, floa
 // diagonal) and 8/13/14 (l1 = option 4 norms): `parts` row partitions
 // (hypre_partition1D), Gauss-Seidel inside one, the pre-sweep u across them.
 // Out of place: u is not modified; zero_guess reads u as zeros.
+// Backward and symmetric types (4/6/8/14) also need Arev, the row-reversed
+// copy of A (csr_reverse_rows; a32 filled for the FP32 variant).
 bool is_hybrid_gs(int type);
+bool hgs_needs_reverse(int type);
+// max_row: the longest row of A (negative: measured here)
 int  hgs_parts(int n, int parts, int rows);
-void relax_hgs(int type, const Csr &A, const double *f, const double *u, const double *l1,
-               double w, double omega, int parts, double *u_out, bool zero_guess, const char *cls);
-void relax_hgs_32(int type, const Csr &A, const float *f, const float *u, const float *l1,
-                  float w, float omega, int parts, float *u_out, bool zero_guess, const char *cls);
+int  csr_max_row(const Csr &A);
+void relax_hgs(int type, const Csr &A, const Csr *Arev, int max_row, const double *f,
+               const double *u, const double *l1, double w, double omega, int parts, double *u_out,
+               bool zero_guess, const char *cls);
+void relax_hgs_32(int type, const Csr &A, const Csr *Arev, int max_row, const float *f,
+                  const float *u, const float *l1, float w, float omega, int parts, float *u_out,
+                  bool zero_guess, const char *cls);
 // BLAS-1
 double dot(int n, const double *x, const double *y);
 void   dot_dev(int n, const double *x, const double *y, double *out_dev);
@@ -171,6 +177,9 @@ void create_s(const Csr &A, double theta, double max_row_sum, Csr &S, int *st_co
 void l1_norms(const Csr &A, double *l1);
 // option 4 on one rank (signed diagonal), for the l1 hybrid Gauss-Seidel types
 void l1_norms_diag(const Csr &A, double *l1);
+// R = A with its rows in reverse order (entries of a row kept in order);
+// a32 copied when A has it
+void csr_reverse_rows(const Csr &A, Csr &R);
 // PMIS: measure (optional input: given measure on device, else Park-Miller
 // stream seed 2747), CF out; returns number of rounds
 // st_count: the column counts of S when already known (else counted here)

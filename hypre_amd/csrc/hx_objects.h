@@ -46,6 +46,8 @@ struct Level
    double *f = nullptr, *u = nullptr, *ut = nullptr, *r = nullptr;
    float  *f32 = nullptr, *u32 = nullptr, *ut32 = nullptr, *r32 = nullptr;
    Csr     A32, P32, R32;       // FP32 value copies share index arrays (a32 filled)
+   Csr     Arev;                // rows of A reversed, for backward hybrid Gauss-Seidel
+   int     max_row = -1;        // longest row of A (hybrid Gauss-Seidel chunking)
 };
 
 struct AMGParams
@@ -76,7 +78,7 @@ struct AMGParams
    int    cycle_fp32 = 0;
    int    keep_setup_data = 0;
    int    hgs_parts = 0;         // hybrid GS partitions on every level (the reference's thread count)
-   int    hgs_rows = 64;         // else partitions of about this many rows
+   int    hgs_rows = 16;         // else partitions of about this many rows
    std::vector<double> level_theta, level_omega;
    std::vector<int>    level_sweeps;
 };

@@ -737,6 +737,55 @@ __global__ void k_l1_diag(int n, const int *__restrict__ ia, const int *__restri
    }
 }
 
+// row k of R is row n-1-k of A with its entries in storage order (the
+// backward hybrid Gauss-Seidel sweep streams R forward); K lanes per row
+template <int K>
+__global__ void k_rev_rows(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                           const double *__restrict__ a, const float *__restrict__ a32,
+                           int *__restrict__ ria, int *__restrict__ rja, double *__restrict__ ra,
+                           float *__restrict__ ra32)
+{
+   const int gl = threadIdx.x & (K - 1);
+   const int rows_per_block = blockDim.x / K;
+   const int nnz = ia[n];
+   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < n;
+        rb += (long long) gridDim.x * rows_per_block) {
+      const long long ir = rb + threadIdx.x / K;
+      if (ir >= n) { continue; }
+      const int r = (int) ir;
+      const int s = ia[r], e = ia[r + 1];
+      const int d = nnz - e;
+      if (gl == 0) {
+         ria[n - 1 - r] = d;
+         if (r == 0) { ria[n] = nnz; }
+      }
+      for (int q = s + gl; q < e; q += K) {
+         rja[d + (q - s)] = ja[q];
+         if (ra) { ra[d + (q - s)] = a[q]; }
+         if (ra32) { ra32[d + (q - s)] = a32[q]; }
+      }
+   }
+}
+
+void csr_reverse_rows(const Csr &A, Csr &R)
+{
+   R.free_all();
+   R.alloc(A.nrows, A.ncols, A.nnz, A.a != nullptr);
+   if (A.a32) { R.a32 = dalloc_n<float>((size_t) std::max<long long>(A.nnz, 1)); }
+   const int n = A.nrows;
+   if (n <= 0) {
+      dzero(R.ia, sizeof(int));
+      return;
+   }
+   const int K = lanes_per_row(A.nnz, n);
+   const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
+   dispatch_k(K, [&](auto kc) {
+      k_rev_rows<decltype(kc)::value><<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, A.a32, R.ia,
+                                                                R.ja, R.a, R.a32);
+   });
+   HX_LAUNCH_CHECK();
+}
+
 void l1_norms_diag(const Csr &A, double *l1)
 {
    int *flag = device_flags() + 40;

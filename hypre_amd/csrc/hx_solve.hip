@@ -433,32 +433,162 @@ void relax_jacobi_32(const Csr &A, const float *f, const float *u, float w, floa
 // ---------------------------------------------------------------- hybrid GS
 // src/parcsr_ls/par_relax.c:692-953 (hypre_BoomerAMGRelaxHybridGaussSeidel_core)
 // with the row kernels of par_relax.h:13-455.  The reference's threads become
-// `parts` contiguous row partitions (utilities.c:99 hypre_partition1D), one
-// lane each; a lane sweeps its rows in order, taking values of its own
-// partition that this sweep already updated from u_out (the previous row
-// from a register) and every other value from u, the copy of the iterate
-// before the sweep (the reference's Vtemp).  Folds keep the storage order.
+// `parts` contiguous row partitions (utilities.c:99 hypre_partition1D).  A
+// group of G lanes owns a partition and walks its rows in sweep order: the
+// group loads a row's entries with coalesced loads, gathers the values of
+// the iterate before the sweep (the reference's Vtemp), forms the products
+// that do not depend on this sweep and stages them in LDS; one lane then
+// folds the row in storage order, taking the values this sweep already
+// produced in the partition from a register (the previous row) or from LDS.
+// A backward sweep runs forward over the rows of a row-reversed copy of the
+// matrix, so both directions stream contiguous memory.
 bool is_hybrid_gs(int t) { return t == 3 || t == 4 || t == 6 || t == 8 || t == 13 || t == 14; }
+
+bool hgs_needs_reverse(int t) { return t == 4 || t == 6 || t == 8 || t == 14; }
 
 int hgs_parts(int n, int parts, int rows)
 {
    if (parts > 0) { return parts; }
-   rows = std::max(rows, 1);
-   if (n <= rows) { return 1; }
-   return (int) (((long long) n + rows - 1) / rows);
+   // partitions of `rows` rows, fewer rows (down to 8) on levels too small
+   // to keep ~128K partitions in flight
+   const long long want = ((long long) n + 131071) / 131072;
+   const int r = (int) std::min<long long>(std::max(rows, 1), std::max<long long>(want, 8));
+   if (n <= r) { return 1; }
+   return (int) (((long long) n + r - 1) / r);
 }
 
-constexpr int HGS_U = 8;
+constexpr int HGS_EPL = 8;     // staged entries per lane and chunk (chunk = 8 G entries)
+constexpr int HGS_CMAX = 16;   // partitions of up to this many rows keep new values in LDS
 
-// mode: 0 forward, 1 backward, 2 forward then backward
-template <typename V, typename X, bool NS, bool SKIP, bool ZERO>
+// group width by mean row length: one chunk covers a typical row
+static int hgs_group(long long nnz, int n)
+{
+   const double avg = n > 0 ? (double) nnz / n : 1.0;
+   if (avg <= 28.0) { return 4; }
+   if (avg <= 60.0) { return 8; }
+   return 16;
+}
+
+// categories of an entry (low 2 bits of the staged code; the partition-local
+// row index above them): 0 other partition (product with the copy's value),
+// 1 the row this sweep produced last, 2 an earlier row of this sweep, 3 a row
+// of this partition not reached yet in this sweep (product with its value)
+template <typename V, int G>
+__device__ __forceinline__ void hgs_load_ja(int c0, int re, int gl, const int *__restrict__ ja,
+                                            const V *__restrict__ a, int (&jv)[HGS_EPL],
+                                            V (&av)[HGS_EPL])
+{
+#pragma unroll
+   for (int t = 0; t < HGS_EPL; t++) {
+      const int q = min(c0 + gl + t * G, re - 1);
+      jv[t] = ja[q];
+      av[t] = a[q];
+   }
+}
+
+// values of the copy (xo) and, in the second sweep of a symmetric pair, the
+// first sweep's values of this partition (x1)
+template <typename X, bool ZERO, bool SECOND>
+__device__ __forceinline__ void hgs_gather(const int (&jv)[HGS_EPL], int i, int n, int ks, int ke,
+                                           bool rev, const X *__restrict__ uo, const X *un,
+                                           X (&xo)[HGS_EPL], X (&x1)[HGS_EPL])
+{
+#pragma unroll
+   for (int t = 0; t < HGS_EPL; t++) {
+      const int j = jv[t];
+      xo[t] = ZERO ? (X) 0 : uo[j];
+      if (SECOND) {
+         const int sj = rev ? n - 1 - j : j;
+         const bool in = sj >= ks && sj < ke;
+         x1[t] = un[in ? j : i];
+      } else {
+         x1[t] = xo[t];
+      }
+   }
+}
+
+template <typename V, typename X, bool NS, bool SECOND, int G>
+__device__ __forceinline__ void hgs_stage(int kr, int n, int ks, int ke, bool rev, int gl,
+                                          const int (&jv)[HGS_EPL], const V (&av)[HGS_EPL],
+                                          const X (&xo)[HGS_EPL], const X (&x1)[HGS_EPL], X *v0,
+                                          X *v1, int *cc)
+{
+   // every slot of the chunk is written (entries past the row end hold values
+   // the fold never reads), so the staging has no branches
+#pragma unroll
+   for (int t = 0; t < HGS_EPL; t++) {
+      const int e = gl + t * G;
+      const int j = jv[t];
+      const int sj = rev ? n - 1 - j : j;
+      const bool in = sj >= ks && sj < ke;
+      const bool done = in && sj < kr;
+      const int cat = !in ? 0 : (done ? (sj == kr - 1 ? 1 : 2) : 3);
+      const X A_ = (X) av[t];
+      cc[e] = cat | ((in ? sj - ks : 0) << 2);
+      v0[e] = done ? A_ : A_ * (in ? x1[t] : xo[t]);
+      if (!NS) { v1[e] = A_ * xo[t]; }
+   }
+}
+
+// One term of a row's fold.  Without weights every term is res -= v * x with
+// x = 1 for a staged product (exact: v * 1 == v), so that path has no branch.
+template <typename X, bool NS>
+__device__ __forceinline__ void hgs_fold_one(int code, X pv, X p1, X prev_val, const X *su,
+                                             bool lds_u, int ks, int n, bool rev, const X *un,
+                                             X &res, X &res0, X &res2)
+{
+   const int cat = code & 3;
+   X xc = (X) 1;
+   if (cat == 1) { xc = prev_val; }
+   if (cat == 2) {
+      if (lds_u) {
+         xc = su[(code >> 2) & (HGS_CMAX - 1)];
+      } else {
+         const int sj = ks + (code >> 2);
+         xc = un[rev ? n - 1 - sj : sj];
+      }
+   }
+   if (NS) {
+      res -= pv * xc;
+   } else if (cat == 0) {
+      res -= pv;
+   } else {
+      res0 -= pv * xc;
+      res2 += p1;
+   }
+}
+
+// MULTI: rows may be longer than one chunk (further chunks are loaded while
+// folding); otherwise every row fits one chunk and the loads of the next two
+// rows are kept in flight: row kr+2's columns and coefficients and row kr+1's
+// gathered values are issued while row kr is folded.
+template <typename V, typename X, bool NS, bool SKIP, bool ZERO, bool REV, bool SECOND, bool MULTI,
+          int G>
 __global__ __launch_bounds__(BS) void k_hgs(int n, int parts, const int *__restrict__ ia,
                                             const int *__restrict__ ja, const V *__restrict__ a,
                                             const X *__restrict__ f, const X *__restrict__ l1,
-                                            const X *__restrict__ uo, X *un, int mode, X w,
-                                            X omega, X prod, X omw)
+                                            const X *__restrict__ uo, X *un, X w, X omega, X prod,
+                                            X omw)
 {
-   const long long pl = (long long) blockIdx.x * BS + threadIdx.x;
+   constexpr int NG = 64 / G;
+   constexpr int CH = HGS_EPL * G;
+   constexpr int NV1 = NS ? 2 : CH;
+   constexpr int SK = SKIP ? 1 : 0;
+   __shared__ __align__(16) X s_v0[WPB * NG * CH];
+   __shared__ __align__(16) X s_v1[WPB * NG * NV1];
+   __shared__ __align__(16) int s_c[WPB * NG * CH];
+   __shared__ X s_u[WPB * NG * HGS_CMAX];
+   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int g = lane / G, gl = lane % G;
+   const int slot = wv * NG + g;
+   X *v0 = s_v0 + slot * CH;
+   X *v1 = s_v1 + slot * NV1;
+   int *cc = s_c + slot * CH;
+   X *su = s_u + slot * HGS_CMAX;
+   // each XCD takes a contiguous range of partitions, so the rows its resident
+   // groups walk (and their x neighbourhoods) stay close together in its L2
+   const int blk = xcd_block(blockIdx.x, gridDim.x);
+   const long long pl = ((long long) blk * WPB + wv) * NG + g;
    if (pl >= parts) { return; }
    const int p = (int) pl;
    int ns, ne;
@@ -476,75 +606,209 @@ __global__ __launch_bounds__(BS) void k_hgs(int n, int parts, const int *__restr
       }
    }
    if (ns >= ne) { return; }
-   const int nsw = (mode == 2) ? 2 : 1;
-   for (int sw = 0; sw < nsw; sw++) {
-      const int ord = (mode == 0) ? 1 : (mode == 1) ? -1 : (sw == 0 ? 1 : -1);
-      const bool second = sw == 1;
-      const int iend = ord > 0 ? ne : ns - 1;
-      int prev = -1;
-      X prev_val = (X) 0;
-      for (int i = ord > 0 ? ns : ne - 1; i != iend; i += ord) {
-         const int rs = ia[i], re = ia[i + 1];
-         const X d = l1 ? l1[i] : (re > rs ? (X) a[rs] : (X) 0);
-         const X old_i = ZERO ? (X) 0 : uo[i];
-         const X cur_i = second ? un[i] : old_i;
-         X nv = cur_i;
-         if (d != (X) 0) {
-            X res = f[i], res0 = (X) 0, res2 = (X) 0;
-            for (int k = rs + (SKIP ? 1 : 0); k < re; k += HGS_U) {
-               int jv[HGS_U];
-               V av[HGS_U];
+   // rows in stream order: k = i forward, k = n-1-i over the reversed copy
+   const int ks = REV ? n - ne : ns, ke = REV ? n - ns : ne;
+   const bool lds_u = (ne - ns) <= HGS_CMAX;
+   auto row_of = [&](int k) { return REV ? n - 1 - k : k; };
+   auto bound = [&](int k) { return ia[min(k, ke)]; };
+   X prev_val = (X) 0;
+   // bounds b0..b3 of rows kr..kr+2; pipeline registers
+   int b0 = bound(ks), b1 = bound(ks + 1), b2 = bound(ks + 2), b3 = bound(ks + 3);
+   int j0[HGS_EPL], j1[HGS_EPL];
+   V a0[HGS_EPL], a1[HGS_EPL];
+   X xo0[HGS_EPL], x10[HGS_EPL];
+   if (b0 + SK < b1) {
+      hgs_load_ja<V, G>(b0 + SK, b1, gl, ja, a, j0, a0);
+      hgs_gather<X, ZERO, SECOND>(j0, row_of(ks), n, ks, ke, REV, uo, un, xo0, x10);
+   }
+   if (!MULTI && ks + 1 < ke && b1 + SK < b2) { hgs_load_ja<V, G>(b1 + SK, b2, gl, ja, a, j1, a1); }
+   // metadata of row kr (divisor, right-hand side, current value)
+   auto meta = [&](int k, int rs, int re, X &d, X &fi, X &ci) {
+      const int i = row_of(k);
+      d = l1 ? l1[i] : (re > rs ? (X) a[rs] : (X) 0);
+      fi = (X) 0;
+      ci = (X) 0;
+      if (gl == 0) {
+         fi = f[i];
+         ci = SECOND ? un[i] : (ZERO ? (X) 0 : uo[i]);
+      }
+   };
+   X d0, f0, c0v;
+   meta(ks, b0, b1, d0, f0, c0v);
+   for (int kr = ks; kr < ke; kr++) {
+      const int i = row_of(kr);
+      const int rs = b0, re = b1;
+      const bool has = rs + SK < re;
+      if (has) { hgs_stage<V, X, NS, SECOND, G>(kr, n, ks, ke, REV, gl, j0, a0, xo0, x10, v0, v1, cc); }
+      wave_fence();
+      // issue the loads of the next rows
+      const int b4 = (kr + 3 < ke) ? ia[kr + 4] : b3;
+      X d1 = (X) 0, f1 = (X) 0, c1v = (X) 0;
+      if (kr + 1 < ke) {
+         meta(kr + 1, b1, b2, d1, f1, c1v);
+         if (MULTI) {
+            if (b1 + SK < b2) {
+               hgs_load_ja<V, G>(b1 + SK, b2, gl, ja, a, j0, a0);
+               hgs_gather<X, ZERO, SECOND>(j0, row_of(kr + 1), n, ks, ke, REV, uo, un, xo0, x10);
+            }
+         } else {
+            if (b1 + SK < b2) {
+               hgs_gather<X, ZERO, SECOND>(j1, row_of(kr + 1), n, ks, ke, REV, uo, un, xo0, x10);
 #pragma unroll
-               for (int u = 0; u < HGS_U; u++) {
-                  const int q = min(k + u, re - 1);
-                  jv[u] = ja[q];
-                  av[u] = a[q];
-               }
-               X xo[HGS_U], xn[HGS_U];
-#pragma unroll
-               for (int u = 0; u < HGS_U; u++) {
-                  const int j = jv[u];
-                  const bool in = j >= ns && j < ne;
-                  const bool upd = in && (second || (ord > 0 ? j < i : j > i));
-                  xo[u] = ZERO ? (X) 0 : uo[j];
-                  xn[u] = un[upd ? j : i];
-               }
-#pragma unroll
-               for (int u = 0; u < HGS_U; u++) {
-                  if (k + u < re) {
-                     const int j = jv[u];
-                     const bool in = j >= ns && j < ne;
-                     const bool upd = in && (second || (ord > 0 ? j < i : j > i));
-                     const X xc = upd ? (j == prev ? prev_val : xn[u]) : xo[u];
-                     const X av_ = (X) av[u];
-                     if (NS) {
-                        res -= av_ * xc;
-                     } else if (in) {
-                        res0 -= av_ * xc;
-                        res2 += av_ * xo[u];
-                     } else {
-                        res -= av_ * xo[u];
-                     }
-                  }
+               for (int t = 0; t < HGS_EPL; t++) {
+                  j0[t] = j1[t];
+                  a0[t] = a1[t];
                }
             }
-            if (NS) {
-               nv = SKIP ? res / d : cur_i + res / d;
-            } else {
-               const X base = SKIP ? cur_i * prod : cur_i;
-               nv = base + w * (omega * res + res0 + omw * res2) / d;
+            if (kr + 2 < ke && b2 + SK < b3) { hgs_load_ja<V, G>(b2 + SK, b3, gl, ja, a, j1, a1); }
+         }
+      }
+      wave_fence();
+      X nv = c0v;
+      if (d0 != (X) 0) {
+         X res = f0, res0 = (X) 0, res2 = (X) 0;
+         for (int cb = rs + SK; cb < re; cb += CH) {
+            if (MULTI && cb != rs + SK) {
+               // rows longer than one chunk: the further chunks are loaded here
+               int jw[HGS_EPL];
+               V aw[HGS_EPL];
+               X xw[HGS_EPL], xw1[HGS_EPL];
+               wave_fence();
+               hgs_load_ja<V, G>(cb, re, gl, ja, a, jw, aw);
+               hgs_gather<X, ZERO, SECOND>(jw, i, n, ks, ke, REV, uo, un, xw, xw1);
+               hgs_stage<V, X, NS, SECOND, G>(kr, n, ks, ke, REV, gl, jw, aw, xw, xw1, v0, v1, cc);
+               wave_fence();
+            }
+            if (gl == 0) {
+               const int ce = min(CH, re - cb);
+               int e = 0;
+               for (; e + 4 <= ce; e += 4) {
+                  const int4 c4 = *reinterpret_cast<const int4 *>(cc + e);
+                  const X q0 = v0[e], q1 = v0[e + 1], q2 = v0[e + 2], q3 = v0[e + 3];
+                  X r0 = (X) 0, r1 = (X) 0, r2 = (X) 0, r3 = (X) 0;
+                  if (!NS) { r0 = v1[e]; r1 = v1[e + 1]; r2 = v1[e + 2]; r3 = v1[e + 3]; }
+                  hgs_fold_one<X, NS>(c4.x, q0, r0, prev_val, su, lds_u, ks, n, REV, un, res, res0, res2);
+                  hgs_fold_one<X, NS>(c4.y, q1, r1, prev_val, su, lds_u, ks, n, REV, un, res, res0, res2);
+                  hgs_fold_one<X, NS>(c4.z, q2, r2, prev_val, su, lds_u, ks, n, REV, un, res, res0, res2);
+                  hgs_fold_one<X, NS>(c4.w, q3, r3, prev_val, su, lds_u, ks, n, REV, un, res, res0, res2);
+               }
+               for (; e < ce; e++) {
+                  hgs_fold_one<X, NS>(cc[e], v0[e], NS ? (X) 0 : v1[e], prev_val, su, lds_u, ks, n, REV,
+                                      un, res, res0, res2);
+               }
             }
          }
+         if (gl == 0) {
+            if (NS) {
+               nv = SKIP ? res / d0 : c0v + res / d0;
+            } else {
+               const X base = SKIP ? c0v * prod : c0v;
+               nv = base + w * (omega * res + res0 + omw * res2) / d0;
+            }
+         }
+      }
+      if (gl == 0) {
          un[i] = nv;
-         prev = i;
+         if (lds_u) { su[kr - ks] = nv; }
          prev_val = nv;
       }
+      b0 = b1;
+      b1 = b2;
+      b2 = b3;
+      b3 = b4;
+      d0 = d1;
+      f0 = f1;
+      c0v = c1v;
+      wave_fence();
    }
 }
 
+template <typename V, typename X, int G>
+static void launch_hgs_sweep(bool multi, bool ns, bool skip, bool zero, bool rev, bool second,
+                             const Csr &M,
+                             const V *vals, const X *f, const X *l1, const X *u, X *uo, int parts,
+                             X w, X omega, X prod, X omw)
+{
+   constexpr int NG = 64 / G;
+   const long long waves = ((long long) parts + NG - 1) / NG;
+   int nb = (int) ((waves + WPB - 1) / WPB);
+   if (nb >= 64) { nb = (nb + 7) / 8 * 8; }     // xcd_block needs a multiple of 8
+   hipStream_t s = stream();
+   auto go = [&](auto nsc, auto skc, auto zc, auto rc, auto sc) {
+      if (multi) {
+         k_hgs<V, X, decltype(nsc)::value, decltype(skc)::value, decltype(zc)::value,
+               decltype(rc)::value, decltype(sc)::value, true, G>
+            <<<nb, BS, 0, s>>>(M.nrows, parts, M.ia, M.ja, vals, f, l1, u, uo, w, omega, prod, omw);
+      } else {
+         k_hgs<V, X, decltype(nsc)::value, decltype(skc)::value, decltype(zc)::value,
+               decltype(rc)::value, decltype(sc)::value, false, G>
+            <<<nb, BS, 0, s>>>(M.nrows, parts, M.ia, M.ja, vals, f, l1, u, uo, w, omega, prod, omw);
+      }
+   };
+   using T = std::true_type;
+   using F = std::false_type;
+   auto l4 = [&](auto nsc, auto skc, auto zc) {
+      if (!rev) { go(nsc, skc, zc, F{}, F{}); }
+      else if (!second) { go(nsc, skc, zc, T{}, F{}); }
+      else { go(nsc, skc, zc, T{}, T{}); }
+   };
+   auto l3 = [&](auto nsc, auto skc) {
+      if (zero) { l4(nsc, skc, T{}); } else { l4(nsc, skc, F{}); }
+   };
+   if (ns) {
+      if (skip) { l3(T{}, T{}); } else { l3(T{}, F{}); }
+   } else {
+      if (skip) { l3(F{}, T{}); } else { l3(F{}, F{}); }
+   }
+   HX_LAUNCH_CHECK();
+}
+
 template <typename V, typename X>
-static void launch_hgs(int type, const Csr &A, const V *vals, const X *f, const X *u, const X *l1,
-                       X w, X omega, X prod, X omw, int parts, X *uo, bool zero, const char *cls)
+static void hgs_sweep(int max_row, bool ns, bool skip, bool zero, bool rev, bool second,
+                      const Csr &M, const V *vals, const X *f, const X *l1, const X *u, X *uo,
+                      int parts, X w, X omega, X prod, X omw)
+{
+   const int G = hgs_group(M.nnz, M.nrows);
+   const bool multi = max_row - (skip ? 1 : 0) > HGS_EPL * G;
+   if (G == 4) {
+      launch_hgs_sweep<V, X, 4>(multi, ns, skip, zero, rev, second, M, vals, f, l1, u, uo, parts, w,
+                                omega, prod, omw);
+   } else if (G == 8) {
+      launch_hgs_sweep<V, X, 8>(multi, ns, skip, zero, rev, second, M, vals, f, l1, u, uo, parts, w,
+                                omega, prod, omw);
+   } else {
+      launch_hgs_sweep<V, X, 16>(multi, ns, skip, zero, rev, second, M, vals, f, l1, u, uo, parts, w,
+                                 omega, prod, omw);
+   }
+}
+
+__global__ void k_row_len_max(int n, const int *__restrict__ ia, int *__restrict__ out)
+{
+   int m = 0;
+   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+      m = max(m, ia[i + 1] - ia[i]);
+   }
+   for (int o = 32; o > 0; o >>= 1) { m = max(m, __shfl_xor(m, o, 64)); }
+   if ((threadIdx.x & 63) == 0) { atomicMax(out, m); }
+}
+
+int csr_max_row(const Csr &A)
+{
+   if (A.nrows <= 0) { return 0; }
+   int *d = device_flags() + 64;
+   dzero(d, sizeof(int));
+   const int nb = std::max(1, std::min((A.nrows + BS - 1) / BS, 2048));
+   k_row_len_max<<<nb, BS, 0, stream()>>>(A.nrows, A.ia, d);
+   HX_LAUNCH_CHECK();
+   int h = 0;
+   d2h(&h, d, sizeof(int));
+   return h;
+}
+
+template <typename V, typename X>
+static void launch_hgs(int type, const Csr &A, const Csr *Arev, int max_row, const V *vals,
+                       const V *rvals, const X *f, const X *u, const X *l1, X w, X omega, X prod,
+                       X omw, int parts, X *uo, bool zero, const char *cls)
 {
    if (A.nrows <= 0) { return; }
    if (!is_hybrid_gs(type)) { fail(HYPRE_ERROR_ARG, "not a hybrid Gauss-Seidel type"); }
@@ -555,45 +819,48 @@ static void launch_hgs(int type, const Csr &A, const V *vals, const X *f, const 
    const bool skip = l1t ? !ns : true;
    const int mode = (type == 6 || type == 8) ? 2 : (type == 4 || type == 14) ? 1 : 0;
    if (l1t && !l1) { fail(HYPRE_ERROR_GENERIC, "hybrid l1 Gauss-Seidel without l1 norms"); }
-   parts = std::max(parts, 1);
-   const double xs = sizeof(X);
-   const double bytes = (double) A.nnz * (4.0 + sizeof(V) + (zero ? 0.0 : xs)) +
-                        4.0 * (A.nrows + 1) + xs * (zero ? 3.0 : 4.0) * A.nrows;
-   TimedLaunch tl(cls, bytes);
-   hipStream_t s = stream();
-   const int nb = (int) (((long long) parts + BS - 1) / BS);
-   const X *l1p = l1t ? l1 : nullptr;
-   auto go = [&](auto nsc, auto skc, auto zc) {
-      k_hgs<V, X, decltype(nsc)::value, decltype(skc)::value, decltype(zc)::value>
-         <<<nb, BS, 0, s>>>(A.nrows, parts, A.ia, A.ja, vals, f, l1p, u, uo, mode, w, omega, prod,
-                            omw);
-   };
-   using T = std::true_type;
-   using F = std::false_type;
-   auto by_zero = [&](auto nsc, auto skc) {
-      if (zero) { go(nsc, skc, T{}); } else { go(nsc, skc, F{}); }
-   };
-   if (ns) {
-      if (skip) { by_zero(T{}, T{}); } else { by_zero(T{}, F{}); }
-   } else {
-      if (skip) { by_zero(F{}, T{}); } else { by_zero(F{}, F{}); }
+   if (mode != 0 && (!Arev || Arev->nrows != A.nrows || !rvals)) {
+      fail(HYPRE_ERROR_GENERIC, "backward hybrid Gauss-Seidel without the row-reversed matrix");
    }
-   HX_LAUNCH_CHECK();
+   parts = std::max(parts, 1);
+   if (max_row < 0) { max_row = csr_max_row(A); }
+   const double xs = sizeof(X);
+   const double bytes = ((double) A.nnz * (4.0 + sizeof(V) + (zero ? 0.0 : xs)) +
+                         4.0 * (A.nrows + 1) + xs * (zero ? 3.0 : 4.0) * A.nrows) *
+                        (mode == 2 ? 2.0 : 1.0);
+   TimedLaunch tl(cls, bytes);
+   const X *l1p = l1t ? l1 : nullptr;
+   if (mode == 0) {
+      hgs_sweep<V, X>(max_row, ns, skip, zero, false, false, A, vals, f, l1p, u, uo, parts, w, omega,
+                                prod, omw);
+   } else if (mode == 1) {
+      hgs_sweep<V, X>(max_row, ns, skip, zero, true, false, *Arev, rvals, f, l1p, u, uo, parts, w,
+                                omega, prod, omw);
+   } else {
+      hgs_sweep<V, X>(max_row, ns, skip, zero, false, false, A, vals, f, l1p, u, uo, parts, w, omega,
+                                prod, omw);
+      hgs_sweep<V, X>(max_row, ns, skip, zero, true, true, *Arev, rvals, f, l1p, u, uo, parts, w,
+                                omega, prod, omw);
+   }
 }
 
-void relax_hgs(int type, const Csr &A, const double *f, const double *u, const double *l1,
-               double w, double omega, int parts, double *u_out, bool zero_guess, const char *cls)
+void relax_hgs(int type, const Csr &A, const Csr *Arev, int max_row, const double *f,
+               const double *u, const double *l1, double w, double omega, int parts, double *u_out,
+               bool zero_guess, const char *cls)
 {
    // prod and 1 - omega as the reference forms them on the host
-   launch_hgs<double, double>(type, A, A.a, f, u, l1, w, omega, 1.0 - w * omega, 1.0 - omega,
-                              parts, u_out, zero_guess, cls);
+   launch_hgs<double, double>(type, A, Arev, max_row, A.a, Arev ? Arev->a : nullptr, f, u, l1, w,
+                              omega,
+                              1.0 - w * omega, 1.0 - omega, parts, u_out, zero_guess, cls);
 }
 
-void relax_hgs_32(int type, const Csr &A, const float *f, const float *u, const float *l1,
-                  float w, float omega, int parts, float *u_out, bool zero_guess, const char *cls)
+void relax_hgs_32(int type, const Csr &A, const Csr *Arev, int max_row, const float *f,
+                  const float *u, const float *l1, float w, float omega, int parts, float *u_out,
+                  bool zero_guess, const char *cls)
 {
-   launch_hgs<float, float>(type, A, A.a32, f, u, l1, w, omega, 1.f - w * omega, 1.f - omega,
-                            parts, u_out, zero_guess, cls);
+   launch_hgs<float, float>(type, A, Arev, max_row, A.a32, Arev ? Arev->a32 : nullptr, f, u, l1, w,
+                            omega,
+                            1.f - w * omega, 1.f - omega, parts, u_out, zero_guess, cls);
 }
 
 // ---------------------------------------------------------------- BLAS-1

@@ -900,13 +900,19 @@ void orc_partition1d(int n, int p, int j, int *s, int *e)
 
 /* Number of row partitions of the hybrid Gauss-Seidel smoother on a level of
  * n rows: `parts` when positive (the reference's thread count), otherwise
- * the automatic rule of the library: partitions of about `rows` rows. */
+ * the automatic rule of the library: partitions of `rows` rows, fewer (down
+ * to 8) where that would leave fewer than about 128K partitions. */
 int orc_hgs_parts(int n, int parts, int rows)
 {
+   long long want;
+   int r;
    if (parts > 0) { return parts; }
    if (rows < 1) { rows = 1; }
-   if (n <= rows) { return 1; }
-   return (n + rows - 1) / rows;
+   want = ((long long) n + 131071) / 131072;    /* ~128K partitions when possible */
+   if (want < 8) { want = 8; }
+   r = (rows < want) ? rows : (int) want;
+   if (n <= r) { return 1; }
+   return (int) (((long long) n + r - 1) / r);
 }
 
 /* One hybrid Gauss-Seidel relaxation on a single rank whose rows are split
@@ -1049,7 +1055,7 @@ void orc_amg_default_params(orc_amg_params *p)
    p->relax_up_type = -1;
    p->outer_weight = 1.0;
    p->hgs_parts = 0;
-   p->hgs_rows = 64;
+   p->hgs_rows = 16;
 }
 
 static void csr_copy(const orc_csr *A, orc_csr *B)

@@ -118,14 +118,14 @@ typedef struct
    int    P_max_elmts;        /* 4 */
    int    max_levels;         /* 25 */
    int    max_coarse_size;    /* 9 */
-   int    relax_type;         /* 18 (l1-Jacobi) or 0 (Jacobi) on down/up */
+   int    relax_type;         /* 18 l1-Jacobi, 0 Jacobi, 3/4/6/8/13/14 hybrid GS */
    int    coarse_relax_type;  /* 9 (Gaussian elimination) */
    double relax_weight;       /* 1.0 */
    int    num_sweeps;         /* 1 */
    int    relax_up_type;      /* -1: relax_type on the up cycle too */
    double outer_weight;       /* 1.0 (hypre's omega) */
    int    hgs_parts;          /* hybrid GS partitions; 0: about hgs_rows rows each */
-   int    hgs_rows;           /* 64 */
+   int    hgs_rows;           /* 16 */
 } orc_amg_params;
 
 void orc_amg_default_params(orc_amg_params *p);

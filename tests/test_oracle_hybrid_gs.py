@@ -26,9 +26,12 @@ def test_partition1d_layout(oracle, n, p):
 
 def test_hgs_parts_rule(oracle):
     assert oracle.hgs_parts(1000, 7, 64) == 7            # explicit count wins
-    assert oracle.hgs_parts(64, 0, 64) == 1
-    assert oracle.hgs_parts(65, 0, 64) == 2
-    assert oracle.hgs_parts(6400, 0, 64) == 100
+    assert oracle.hgs_parts(8, 0, 64) == 1               # small levels: 8-row partitions
+    assert oracle.hgs_parts(65, 0, 64) == 9
+    assert oracle.hgs_parts(6400, 0, 64) == 800
+    assert oracle.hgs_parts(6400, 0, 4) == 1600          # rows below 8 are kept
+    assert oracle.hgs_parts(8000000, 0, 64) == 129033    # 62 rows: ~128K partitions
+    assert oracle.hgs_parts(256 ** 3, 0, 64) == 262144   # 64 rows
 
 
 def _matrix(rng, n, dense=0.3, nonsym=True):
