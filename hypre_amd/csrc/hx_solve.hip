@@ -460,13 +460,15 @@ int hgs_parts(int n, int parts, int rows)
 constexpr int HGS_EPL = 8;     // staged entries per lane and chunk (chunk = 8 G entries)
 constexpr int HGS_CMAX = 16;   // partitions of up to this many rows keep new values in LDS
 
-// group width by mean row length: one chunk covers a typical row
-static int hgs_group(long long nnz, int n)
+// group width: one chunk (8 G entries) covers a typical row; levels small
+// enough to be latency-bound take a whole wave per partition when their
+// longest rows would need several chunks
+static int hgs_group(long long nnz, int n, int max_row)
 {
    const double avg = n > 0 ? (double) nnz / n : 1.0;
-   if (avg <= 28.0) { return 4; }
-   if (avg <= 60.0) { return 8; }
-   return 16;
+   int G = (avg <= 28.0) ? 4 : 16;
+   if (max_row > HGS_EPL * G && n <= 262144) { G = 64; }
+   return G;
 }
 
 // categories of an entry (low 2 bits of the staged code; the partition-local
@@ -507,14 +509,16 @@ __device__ __forceinline__ void hgs_gather(const int (&jv)[HGS_EPL], int i, int 
    }
 }
 
-template <typename V, typename X, bool NS, bool SECOND, int G>
+template <typename V, typename X, bool NS, bool SECOND, bool PRE, int G>
 __device__ __forceinline__ void hgs_stage(int kr, int n, int ks, int ke, bool rev, int gl,
                                           const int (&jv)[HGS_EPL], const V (&av)[HGS_EPL],
                                           const X (&xo)[HGS_EPL], const X (&x1)[HGS_EPL], X *v0,
-                                          X *v1, int *cc)
+                                          X *v1, int *cc, const X *su)
 {
    // every slot of the chunk is written (entries past the row end hold values
-   // the fold never reads), so the staging has no branches
+   // the fold never reads), so the staging has no branches.  PRE: the rows of
+   // this partition that the sweep already produced are in LDS, so every
+   // product is formed here and the fold is a plain ordered sum.
 #pragma unroll
    for (int t = 0; t < HGS_EPL; t++) {
       const int e = gl + t * G;
@@ -522,11 +526,20 @@ __device__ __forceinline__ void hgs_stage(int kr, int n, int ks, int ke, bool re
       const int sj = rev ? n - 1 - j : j;
       const bool in = sj >= ks && sj < ke;
       const bool done = in && sj < kr;
-      const int cat = !in ? 0 : (done ? (sj == kr - 1 ? 1 : 2) : 3);
       const X A_ = (X) av[t];
-      cc[e] = cat | ((in ? sj - ks : 0) << 2);
-      v0[e] = done ? A_ : A_ * (in ? x1[t] : xo[t]);
-      if (!NS) { v1[e] = A_ * xo[t]; }
+      if (PRE) {
+         const X xv = done ? su[(sj - ks) & (HGS_CMAX - 1)] : (in ? x1[t] : xo[t]);
+         v0[e] = A_ * xv;
+         if (!NS) {
+            v1[e] = A_ * xo[t];
+            cc[e] = in ? 1 : 0;
+         }
+      } else {
+         const int cat = !in ? 0 : (done ? (sj == kr - 1 ? 1 : 2) : 3);
+         cc[e] = cat | ((in ? sj - ks : 0) << 2);
+         v0[e] = done ? A_ : A_ * (in ? x1[t] : xo[t]);
+         if (!NS) { v1[e] = A_ * xo[t]; }
+      }
    }
 }
 
@@ -563,7 +576,7 @@ __device__ __forceinline__ void hgs_fold_one(int code, X pv, X p1, X prev_val, c
 // rows are kept in flight: row kr+2's columns and coefficients and row kr+1's
 // gathered values are issued while row kr is folded.
 template <typename V, typename X, bool NS, bool SKIP, bool ZERO, bool REV, bool SECOND, bool MULTI,
-          int G>
+          bool PRE, int G>
 __global__ __launch_bounds__(BS) void k_hgs(int n, int parts, const int *__restrict__ ia,
                                             const int *__restrict__ ja, const V *__restrict__ a,
                                             const X *__restrict__ f, const X *__restrict__ l1,
@@ -639,7 +652,9 @@ __global__ __launch_bounds__(BS) void k_hgs(int n, int parts, const int *__restr
       const int i = row_of(kr);
       const int rs = b0, re = b1;
       const bool has = rs + SK < re;
-      if (has) { hgs_stage<V, X, NS, SECOND, G>(kr, n, ks, ke, REV, gl, j0, a0, xo0, x10, v0, v1, cc); }
+      if (has) {
+         hgs_stage<V, X, NS, SECOND, PRE, G>(kr, n, ks, ke, REV, gl, j0, a0, xo0, x10, v0, v1, cc, su);
+      }
       wave_fence();
       // issue the loads of the next rows
       const int b4 = (kr + 3 < ke) ? ia[kr + 4] : b3;
@@ -676,10 +691,34 @@ __global__ __launch_bounds__(BS) void k_hgs(int n, int parts, const int *__restr
                wave_fence();
                hgs_load_ja<V, G>(cb, re, gl, ja, a, jw, aw);
                hgs_gather<X, ZERO, SECOND>(jw, i, n, ks, ke, REV, uo, un, xw, xw1);
-               hgs_stage<V, X, NS, SECOND, G>(kr, n, ks, ke, REV, gl, jw, aw, xw, xw1, v0, v1, cc);
+               hgs_stage<V, X, NS, SECOND, PRE, G>(kr, n, ks, ke, REV, gl, jw, aw, xw, xw1, v0, v1, cc,
+                                                   su);
                wave_fence();
             }
-            if (gl == 0) {
+            if (gl == 0 && PRE) {
+               const int ce = min(CH, re - cb);
+               int e = 0;
+               for (; e + 4 <= ce; e += 4) {
+                  const X q0 = v0[e], q1 = v0[e + 1], q2 = v0[e + 2], q3 = v0[e + 3];
+                  if (NS) {
+                     res -= q0;
+                     res -= q1;
+                     res -= q2;
+                     res -= q3;
+                  } else {
+                     const int4 c4 = *reinterpret_cast<const int4 *>(cc + e);
+                     const X r0 = v1[e], r1 = v1[e + 1], r2 = v1[e + 2], r3 = v1[e + 3];
+                     if (c4.x) { res0 -= q0; res2 += r0; } else { res -= q0; }
+                     if (c4.y) { res0 -= q1; res2 += r1; } else { res -= q1; }
+                     if (c4.z) { res0 -= q2; res2 += r2; } else { res -= q2; }
+                     if (c4.w) { res0 -= q3; res2 += r3; } else { res -= q3; }
+                  }
+               }
+               for (; e < ce; e++) {
+                  if (NS || !cc[e]) { res -= v0[e]; }
+                  else { res0 -= v0[e]; res2 += v1[e]; }
+               }
+            } else if (gl == 0) {
                const int ce = min(CH, re - cb);
                int e = 0;
                for (; e + 4 <= ce; e += 4) {
@@ -724,41 +763,41 @@ __global__ __launch_bounds__(BS) void k_hgs(int n, int parts, const int *__restr
 }
 
 template <typename V, typename X, int G>
-static void launch_hgs_sweep(bool multi, bool ns, bool skip, bool zero, bool rev, bool second,
-                             const Csr &M,
-                             const V *vals, const X *f, const X *l1, const X *u, X *uo, int parts,
-                             X w, X omega, X prod, X omw)
+static void launch_hgs_sweep(bool multi, bool pre, bool ns, bool skip, bool zero, bool rev,
+                             bool second, const Csr &M, const V *vals, const X *f, const X *l1,
+                             const X *u, X *uo, int parts, X w, X omega, X prod, X omw)
 {
    constexpr int NG = 64 / G;
    const long long waves = ((long long) parts + NG - 1) / NG;
    int nb = (int) ((waves + WPB - 1) / WPB);
    if (nb >= 64) { nb = (nb + 7) / 8 * 8; }     // xcd_block needs a multiple of 8
    hipStream_t s = stream();
-   auto go = [&](auto nsc, auto skc, auto zc, auto rc, auto sc) {
-      if (multi) {
-         k_hgs<V, X, decltype(nsc)::value, decltype(skc)::value, decltype(zc)::value,
-               decltype(rc)::value, decltype(sc)::value, true, G>
-            <<<nb, BS, 0, s>>>(M.nrows, parts, M.ia, M.ja, vals, f, l1, u, uo, w, omega, prod, omw);
-      } else {
-         k_hgs<V, X, decltype(nsc)::value, decltype(skc)::value, decltype(zc)::value,
-               decltype(rc)::value, decltype(sc)::value, false, G>
-            <<<nb, BS, 0, s>>>(M.nrows, parts, M.ia, M.ja, vals, f, l1, u, uo, w, omega, prod, omw);
-      }
-   };
    using T = std::true_type;
    using F = std::false_type;
+   auto go = [&](auto nsc, auto skc, auto zc, auto rc, auto sc, auto mc) {
+      auto k = [&](auto pc) {
+         k_hgs<V, X, decltype(nsc)::value, decltype(skc)::value, decltype(zc)::value,
+               decltype(rc)::value, decltype(sc)::value, decltype(mc)::value, decltype(pc)::value, G>
+            <<<nb, BS, 0, s>>>(M.nrows, parts, M.ia, M.ja, vals, f, l1, u, uo, w, omega, prod, omw);
+      };
+      if (pre) { k(T{}); } else { k(F{}); }
+   };
+   auto l5 = [&](auto nsc, auto skc, auto zc, auto rc, auto sc) {
+      if (multi) { go(nsc, skc, zc, rc, sc, T{}); } else { go(nsc, skc, zc, rc, sc, F{}); }
+   };
    auto l4 = [&](auto nsc, auto skc, auto zc) {
-      if (!rev) { go(nsc, skc, zc, F{}, F{}); }
-      else if (!second) { go(nsc, skc, zc, T{}, F{}); }
-      else { go(nsc, skc, zc, T{}, T{}); }
+      if (!rev) { l5(nsc, skc, zc, F{}, F{}); }
+      else if (!second) { l5(nsc, skc, zc, T{}, F{}); }
+      else { l5(nsc, skc, zc, T{}, T{}); }
    };
    auto l3 = [&](auto nsc, auto skc) {
       if (zero) { l4(nsc, skc, T{}); } else { l4(nsc, skc, F{}); }
    };
+   // weights other than 1 always skip the diagonal (par_relax.c:1266-1383)
    if (ns) {
       if (skip) { l3(T{}, T{}); } else { l3(T{}, F{}); }
    } else {
-      if (skip) { l3(F{}, T{}); } else { l3(F{}, F{}); }
+      l3(F{}, T{});
    }
    HX_LAUNCH_CHECK();
 }
@@ -768,17 +807,20 @@ static void hgs_sweep(int max_row, bool ns, bool skip, bool zero, bool rev, bool
                       const Csr &M, const V *vals, const X *f, const X *l1, const X *u, X *uo,
                       int parts, X w, X omega, X prod, X omw)
 {
-   const int G = hgs_group(M.nnz, M.nrows);
+   const int G = hgs_group(M.nnz, M.nrows, max_row);
    const bool multi = max_row - (skip ? 1 : 0) > HGS_EPL * G;
+   // partitions hold at most ceil(n / parts) rows
+   const bool pre = ((long long) M.nrows + parts - 1) / parts <= HGS_CMAX;
+   if (!ns && !skip) { fail(HYPRE_ERROR_GENERIC, "weighted hybrid Gauss-Seidel keeps the diagonal"); }
    if (G == 4) {
-      launch_hgs_sweep<V, X, 4>(multi, ns, skip, zero, rev, second, M, vals, f, l1, u, uo, parts, w,
-                                omega, prod, omw);
-   } else if (G == 8) {
-      launch_hgs_sweep<V, X, 8>(multi, ns, skip, zero, rev, second, M, vals, f, l1, u, uo, parts, w,
-                                omega, prod, omw);
+      launch_hgs_sweep<V, X, 4>(multi, pre, ns, skip, zero, rev, second, M, vals, f, l1, u, uo,
+                                parts, w, omega, prod, omw);
+   } else if (G == 16) {
+      launch_hgs_sweep<V, X, 16>(multi, pre, ns, skip, zero, rev, second, M, vals, f, l1, u, uo,
+                                 parts, w, omega, prod, omw);
    } else {
-      launch_hgs_sweep<V, X, 16>(multi, ns, skip, zero, rev, second, M, vals, f, l1, u, uo, parts, w,
-                                 omega, prod, omw);
+      launch_hgs_sweep<V, X, 64>(multi, pre, ns, skip, zero, rev, second, M, vals, f, l1, u, uo,
+                                 parts, w, omega, prod, omw);
    }
 }
 
@@ -831,16 +873,16 @@ static void launch_hgs(int type, const Csr &A, const Csr *Arev, int max_row, con
    TimedLaunch tl(cls, bytes);
    const X *l1p = l1t ? l1 : nullptr;
    if (mode == 0) {
-      hgs_sweep<V, X>(max_row, ns, skip, zero, false, false, A, vals, f, l1p, u, uo, parts, w, omega,
-                                prod, omw);
+      hgs_sweep<V, X>(max_row, ns, skip, zero, false, false, A, vals, f, l1p, u, uo, parts, w,
+                      omega, prod, omw);
    } else if (mode == 1) {
-      hgs_sweep<V, X>(max_row, ns, skip, zero, true, false, *Arev, rvals, f, l1p, u, uo, parts, w,
-                                omega, prod, omw);
+      hgs_sweep<V, X>(max_row, ns, skip, zero, true, false, *Arev, rvals, f, l1p, u, uo, parts,
+                      w, omega, prod, omw);
    } else {
-      hgs_sweep<V, X>(max_row, ns, skip, zero, false, false, A, vals, f, l1p, u, uo, parts, w, omega,
-                                prod, omw);
-      hgs_sweep<V, X>(max_row, ns, skip, zero, true, true, *Arev, rvals, f, l1p, u, uo, parts, w,
-                                omega, prod, omw);
+      hgs_sweep<V, X>(max_row, ns, skip, zero, false, false, A, vals, f, l1p, u, uo, parts, w,
+                      omega, prod, omw);
+      hgs_sweep<V, X>(max_row, ns, skip, zero, true, true, *Arev, rvals, f, l1p, u, uo, parts,
+                      w, omega, prod, omw);
    }
 }
 

@@ -97,9 +97,9 @@ def test_vcycle_matches_oracle(hx, oracle, down, up, parts):
     A_or = oracle.gen_vardiff27(14, 4, -2.0, 2.0, 5)
     A_dev = hx.ParCSRMatrix.from_csr(A_or.ia, A_or.ja, A_or.a)
     amg = hx.BoomerAMG(max_iter=1, tol=0.0, relax_type=down, relax_up=up,
-                       hybrid_partitions=(parts, 16))
+                       hybrid_partitions=(parts, 8))
     amg.setup(A_dev)
-    H = oracle.Hierarchy(A_or, relax_type=down, relax_up_type=up, hgs_parts=parts, hgs_rows=16)
+    H = oracle.Hierarchy(A_or, relax_type=down, relax_up_type=up, hgs_parts=parts, hgs_rows=8)
     assert amg.num_levels() == H.num_levels
     f = 2 * oracle.rand_stream(3, A_or.nrows) - 1
     ug = amg.cycle_host(f)
