@@ -202,6 +202,16 @@ class ParVector:
 
 # ------------------------------------------------------------------ IJ
 class IJMatrix:
+    @classmethod
+    def _wrap(cls, h):
+        m = cls.__new__(cls)
+        m.h = h
+        return m
+
+    def print(self, path: str, binary: bool = False):
+        call("HYPRE_IJMatrixPrintBinary" if binary else "HYPRE_IJMatrixPrint", self.h,
+             path.encode())
+
     def __init__(self, ilower, iupper, jlower=None, jupper=None):
         h = C.c_void_p()
         call("HYPRE_IJMatrixCreate", 0, ilower, iupper, ilower if jlower is None else jlower,
@@ -256,6 +266,19 @@ class IJMatrix:
 
 
 class IJVector:
+    @classmethod
+    def _wrap(cls, h):
+        v = cls.__new__(cls)
+        v.h = h
+        lo, hi = C.c_int(), C.c_int()
+        call("HYPRE_IJVectorGetLocalRange", h, C.byref(lo), C.byref(hi))
+        v.n = hi.value - lo.value + 1
+        return v
+
+    def print(self, path: str, binary: bool = False):
+        call("HYPRE_IJVectorPrintBinary" if binary else "HYPRE_IJVectorPrint", self.h,
+             path.encode())
+
     def __init__(self, jlower, jupper):
         h = C.c_void_p()
         call("HYPRE_IJVectorCreate", 0, jlower, jupper, C.byref(h))
@@ -446,7 +469,7 @@ class BoomerAMG:
 class PCG:
     """HYPRE_ParCSRPCG* preconditioned by a BoomerAMG V-cycle."""
 
-    def __init__(self, precond: BoomerAMG | None = None, tol=1e-8, max_iter=1000, two_norm=1,
+    def __init__(self, precond: BoomerAMG | str | None = None, tol=1e-8, max_iter=1000, two_norm=1,
                  print_level=0):
         h = C.c_void_p()
         call("HYPRE_ParCSRPCGCreate", 0, C.byref(h))
@@ -456,7 +479,11 @@ class PCG:
         call("HYPRE_ParCSRPCGSetTwoNorm", h, two_norm)
         call("HYPRE_ParCSRPCGSetPrintLevel", h, print_level)
         self.precond = precond
-        if precond is not None:
+        if precond == "diag_scale":
+            # HYPRE_ParCSRDiagScale with no solver object, as the reference's driver does
+            call("HYPRE_ParCSRPCGSetPrecond", h, fnptr("HYPRE_ParCSRDiagScale"),
+                 fnptr("HYPRE_ParCSRDiagScaleSetup"), None)
+        elif precond is not None:
             precond.set(max_iter=1, tol=0.0)
             call("HYPRE_ParCSRPCGSetPrecond", h, fnptr("HYPRE_BoomerAMGSolve"),
                  fnptr("HYPRE_BoomerAMGSetup"), precond.h)
@@ -567,6 +594,23 @@ def hybrid_gs(A: HostCsr, f, u, relax_type: int, parts: int, l1=None, relax_weig
     call("HYPRE_MI355X_HybridGS", A.nrows, _ip(A.ia), _ip(A.ja), _dp(A.a), _dp(f), l1p, _dp(out),
          relax_type, relax_weight, omega, parts, 1 if zero_guess else 0)
     return out
+
+
+# ------------------------------------------------------------------ IJ files
+def ij_matrix_read(path: str, fmt: str = "ij") -> "IJMatrix":
+    """HYPRE_IJMatrixRead / ReadMM / ReadBinary; fmt in {"ij", "mm", "bin"}."""
+    fn = {"ij": "HYPRE_IJMatrixRead", "mm": "HYPRE_IJMatrixReadMM",
+          "bin": "HYPRE_IJMatrixReadBinary"}[fmt]
+    h = C.c_void_p()
+    call(fn, path.encode(), 0, HYPRE_PARCSR, C.byref(h))
+    return IJMatrix._wrap(h)
+
+
+def ij_vector_read(path: str, binary: bool = False) -> "IJVector":
+    h = C.c_void_p()
+    call("HYPRE_IJVectorReadBinary" if binary else "HYPRE_IJVectorRead", path.encode(), 0,
+         HYPRE_PARCSR, C.byref(h))
+    return IJVector._wrap(h)
 
 
 # ------------------------------------------------------------------ timing

@@ -776,9 +776,9 @@ HYPRE_Int HYPRE_ParCSRPCGSetup(HYPRE_Solver solver, HYPRE_ParCSRMatrix A, HYPRE_
       if (!A) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "NULL matrix"); }
       double t0 = wall_seconds();
       s->ensure_vectors(A->A.nrows);
-      if (s->precond_setup && s->precond_solver) {
-         s->precond_setup(s->precond_solver, A, b, x);
-      }
+      // the setup and solve functions are called whether or not they come
+      // with a solver object (the diagonal scaling takes none), as in pcg.c
+      if (s->precond_setup) { s->precond_setup(s->precond_solver, A, b, x); }
       sync();
       s->setup_seconds = wall_seconds() - t0;
    });
@@ -790,7 +790,7 @@ static void apply_precond(PCGSolver *s, HYPRE_ParCSRMatrix A, hypre_ParVector_st
    // ClearVector (par_krylov_func.c:260) then the preconditioner
    fill(out->n, 0.0, out->d);
    out->all_zeros = true;
-   if (s->precond && s->precond_solver) {
+   if (s->precond) {
       s->precond(s->precond_solver, A, rhs, out);
    } else {
       copy(out->n, rhs->d, out->d);         // identity preconditioner

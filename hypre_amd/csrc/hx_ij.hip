@@ -12,6 +12,8 @@
 #include "hx_objects.h"
 
 #include <algorithm>
+#include <cctype>
+#include <cstdint>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -454,43 +456,226 @@ HYPRE_Int HYPRE_IJMatrixGetValues(HYPRE_IJMatrix m, HYPRE_Int nrows, HYPRE_Int *
    });
 }
 
-HYPRE_Int HYPRE_IJMatrixPrint(HYPRE_IJMatrix m, const char *filename)
+// ================================================================== IJ matrix files
+// One rank: the reference's per-rank files carry the rank suffix .00000
+// (src/IJ_mv/IJMatrix.c:126-134, par_csr_matrix.c:1000, 1194).
+static std::string rank_file(const char *name, const char *ext)
 {
-   return guard([&] {
-      if (!m->assembled) { fail(HYPRE_ERROR_GENERIC, "IJ matrix not assembled"); }
-      FILE *fp = fopen(filename, "w");
-      if (!fp) { fail(HYPRE_ERROR_GENERIC, "cannot open file"); }
-      fprintf(fp, "%d %d %d %d\n", m->ilower, m->iupper, m->jlower, m->jupper);
-      const int nrows = m->iupper - m->ilower + 1;
-      for (int i = 0; i < nrows; i++) {
-         for (int q = m->h_ia[(size_t) i]; q < m->h_ia[(size_t) i + 1]; q++) {
-            fprintf(fp, "%d %d %.15e\n", i + m->ilower, m->h_ja[(size_t) q] + m->jlower, m->h_a[(size_t) q]);
-         }
-      }
-      fclose(fp);
-   });
+   return std::string(name) + ".00000" + ext;
+}
+
+struct FileCloser
+{
+   FILE *fp;
+   ~FileCloser() { if (fp) { fclose(fp); } }
+};
+
+// MatrixMarket banner and size line (the checks of src/utilities/mmio.c as
+// the reference applies them in src/IJ_mv/IJMatrix.c:136-170): a sparse
+// real or integer coordinate matrix, general or symmetric.
+static void mm_header(FILE *fp, int &nrow, int &ncol, int &nnz, bool &sym)
+{
+   char line[1025];
+   if (!fgets(line, sizeof line, fp)) { fail(HYPRE_ERROR_GENERIC, "Could not process Matrix Market banner."); }
+   char banner[64] = {0}, object[64] = {0}, format[64] = {0}, field[64] = {0}, symm[64] = {0};
+   if (sscanf(line, "%63s %63s %63s %63s %63s", banner, object, format, field, symm) != 5 ||
+       std::strcmp(banner, "%%MatrixMarket") != 0) {
+      fail(HYPRE_ERROR_GENERIC, "Could not process Matrix Market banner.");
+   }
+   auto lower = [](char *t) { for (; *t; t++) { *t = (char) std::tolower((unsigned char) *t); } };
+   lower(object); lower(format); lower(field); lower(symm);
+   if (std::strcmp(object, "matrix") != 0) { fail(HYPRE_ERROR_GENERIC, "Invalid Matrix Market file."); }
+   const bool real = !std::strcmp(field, "real") || !std::strcmp(field, "integer");
+   if (!real || std::strcmp(format, "coordinate") != 0) {
+      fail(HYPRE_ERROR_GENERIC, "Only sparse real-valued/integer coordinate matrices are supported");
+   }
+   if (!std::strcmp(symm, "symmetric")) {
+      sym = true;
+   } else if (!std::strcmp(symm, "general")) {
+      sym = false;
+   } else {
+      fail(HYPRE_ERROR_GENERIC, "Only general and symmetric Matrix Market matrices are supported");
+   }
+   // comment lines, then "rows cols entries"
+   while (true) {
+      if (!fgets(line, sizeof line, fp)) { fail(HYPRE_ERROR_GENERIC, "MM read size error !"); }
+      if (line[0] == '%') { continue; }
+      if (sscanf(line, "%d %d %d", &nrow, &ncol, &nnz) == 3) { break; }
+      bool blank = true;
+      for (char *t = line; *t; t++) { if (!std::isspace((unsigned char) *t)) { blank = false; } }
+      if (!blank) { fail(HYPRE_ERROR_GENERIC, "MM read size error !"); }
+   }
+}
+
+// src/IJ_mv/IJMatrix.c:109-237 hypre_IJMatrixRead (IJ text or MatrixMarket)
+static void ij_matrix_read(const char *filename, MPI_Comm comm, HYPRE_Int type,
+                           HYPRE_IJMatrix *matrix, bool mm)
+{
+   if (!filename) { g_error_arg = 1; fail(HYPRE_ERROR_ARG, "NULL file name"); }
+   if (!matrix) { g_error_arg = 4; fail(HYPRE_ERROR_ARG, "NULL matrix pointer"); }
+   const std::string path = mm ? std::string(filename) : rank_file(filename, "");
+   FileCloser fc{fopen(path.c_str(), "r")};
+   if (!fc.fp) { g_error_arg = 1; fail(HYPRE_ERROR_ARG, "cannot open " + path); }
+   int il, iu, jl, ju;
+   bool sym = false;
+   if (mm) {
+      int nrow, ncol, nnz;
+      mm_header(fc.fp, nrow, ncol, nnz, sym);
+      il = 0; iu = nrow - 1; jl = 0; ju = ncol - 1;
+   } else if (fscanf(fc.fp, "%d %d %d %d", &il, &iu, &jl, &ju) != 4) {
+      fail(HYPRE_ERROR_GENERIC, "Error in IJ matrix input file.");
+   }
+   std::vector<int> rows, cols;
+   std::vector<double> vals;
+   int I, J, ret;
+   double v;
+   // whitespace must follow the column index (IJMatrix.c:184-186)
+   while ((ret = fscanf(fc.fp, "%d %d%*[ \t]%le", &I, &J, &v)) != EOF) {
+      if (ret != 3) { fail(HYPRE_ERROR_GENERIC, "Error in IJ matrix input file."); }
+      if (mm) { I--; J--; }
+      rows.push_back(I); cols.push_back(J); vals.push_back(v);
+      if (sym && I != J) { rows.push_back(J); cols.push_back(I); vals.push_back(v); }
+   }
+   HYPRE_IJMatrix m = nullptr;
+   HYPRE_IJMatrixCreate(comm, il, iu, jl, ju, &m);
+   if (g_error_flag) { fail(HYPRE_ERROR_GENERIC, "IJ matrix create failed"); }
+   HYPRE_IJMatrixSetObjectType(m, type);
+   HYPRE_IJMatrixInitialize_v2(m, HYPRE_MEMORY_HOST);
+   if (!rows.empty()) {
+      std::vector<int> ones(rows.size(), 1);
+      ij_stage(m, (int) rows.size(), ones.data(), rows.data(), cols.data(), vals.data(), 0);
+   }
+   HYPRE_IJMatrixAssemble(m);
+   *matrix = m;
 }
 
 HYPRE_Int HYPRE_IJMatrixRead(const char *filename, MPI_Comm comm, HYPRE_Int type, HYPRE_IJMatrix *matrix)
 {
+   return guard([&] { ij_matrix_read(filename, comm, type, matrix, false); });
+}
+
+HYPRE_Int HYPRE_IJMatrixReadMM(const char *filename, MPI_Comm comm, HYPRE_Int type,
+                               HYPRE_IJMatrix *matrix)
+{
+   return guard([&] { ij_matrix_read(filename, comm, type, matrix, true); });
+}
+
+// src/parcsr_mv/par_csr_matrix.c:990-1060 hypre_ParCSRMatrixPrintIJ
+HYPRE_Int HYPRE_IJMatrixPrint(HYPRE_IJMatrix m, const char *filename)
+{
    return guard([&] {
-      FILE *fp = fopen(filename, "r");
-      if (!fp) { fail(HYPRE_ERROR_GENERIC, "cannot open file"); }
-      int il, iu, jl, ju;
-      if (fscanf(fp, "%d %d %d %d", &il, &iu, &jl, &ju) != 4) { fclose(fp); fail(HYPRE_ERROR_GENERIC, "bad header"); }
-      HYPRE_IJMatrix m;
-      HYPRE_IJMatrixCreate(comm, il, iu, jl, ju, &m);
-      HYPRE_IJMatrixSetObjectType(m, type);
-      HYPRE_IJMatrixInitialize(m);
-      int r, c;
-      double v;
-      while (fscanf(fp, "%d %d %lf", &r, &c, &v) == 3) {
-         int one = 1;
-         ij_stage(m, 1, &one, &r, &c, &v, 0);
+      if (!m || m->object_type != HYPRE_PARCSR) { g_error_arg = 1; fail(HYPRE_ERROR_ARG, "bad IJ matrix"); }
+      if (!m->assembled) { fail(HYPRE_ERROR_GENERIC, "IJ matrix not assembled"); }
+      const std::string path = rank_file(filename, "");
+      FileCloser fc{fopen(path.c_str(), "w")};
+      if (!fc.fp) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "cannot open " + path); }
+      fprintf(fc.fp, "%d %d %d %d\n", m->ilower, m->iupper, m->jlower, m->jupper);
+      const int nrows = m->iupper - m->ilower + 1;
+      for (int i = 0; i < nrows; i++) {
+         for (int q = m->h_ia[(size_t) i]; q < m->h_ia[(size_t) i + 1]; q++) {
+            fprintf(fc.fp, "%d %d %.14e\n", i + m->ilower, m->h_ja[(size_t) q] + m->jlower,
+                    m->h_a[(size_t) q]);
+         }
       }
-      fclose(fp);
+   });
+}
+
+static void read_exact_n(FILE *fp, void *dst, size_t elem, size_t n, const char *what)
+{
+   if (n && fread(dst, elem, n, fp) != n) { fail(HYPRE_ERROR_GENERIC, what); }
+}
+
+// src/IJ_mv/IJMatrix.c:252-500 hypre_IJMatrixReadBinary: header of 11 uint64
+// (version 1, index bytes, value bytes, global rows, columns and nonzeros,
+// local nonzeros, ilower, iupper, jlower, jupper), then the row index of
+// every entry, its column index and its value
+HYPRE_Int HYPRE_IJMatrixReadBinary(const char *filename, MPI_Comm comm, HYPRE_Int type,
+                                   HYPRE_IJMatrix *matrix)
+{
+   return guard([&] {
+      if (!filename) { g_error_arg = 1; fail(HYPRE_ERROR_ARG, "NULL file name"); }
+      if (!matrix) { g_error_arg = 4; fail(HYPRE_ERROR_ARG, "NULL matrix pointer"); }
+      const std::string path = rank_file(filename, ".bin");
+      FileCloser fc{fopen(path.c_str(), "rb")};
+      if (!fc.fp) { fail(HYPRE_ERROR_GENERIC, "Could not open input file " + path); }
+      uint64_t h[11];
+      read_exact_n(fc.fp, h, sizeof(*h), 11, "Could not read header entries");
+      if (h[0] != 1) { fail(HYPRE_ERROR_GENERIC, "Unsupported header version"); }
+      if (h[6] > (uint64_t) INT32_MAX) { fail(HYPRE_ERROR_GENERIC, "Detected integer overflow at 7th header entry"); }
+      if (h[1] != 4 && h[1] != 8) { fail(HYPRE_ERROR_GENERIC, "Unsupported data type for row/column indices"); }
+      if (h[2] != 4 && h[2] != 8) { fail(HYPRE_ERROR_GENERIC, "Unsupported data type for matrix coefficients"); }
+      const size_t nnz = (size_t) h[6];
+      for (int k = 7; k < 11; k++) {
+         if (h[k] > (uint64_t) INT32_MAX) { fail(HYPRE_ERROR_GENERIC, "row or column range beyond 32-bit indices"); }
+      }
+      auto read_idx = [&](std::vector<int> &out, const char *what) {
+         out.resize(nnz);
+         if (h[1] == 4) {
+            std::vector<uint32_t> b(nnz);
+            read_exact_n(fc.fp, b.data(), sizeof(*b.data()), nnz, what);
+            for (size_t k = 0; k < nnz; k++) { out[k] = (int) b[k]; }
+         } else {
+            std::vector<uint64_t> b(nnz);
+            read_exact_n(fc.fp, b.data(), sizeof(*b.data()), nnz, what);
+            for (size_t k = 0; k < nnz; k++) {
+               if (b[k] > (uint64_t) INT32_MAX) { fail(HYPRE_ERROR_GENERIC, "index beyond 32 bits"); }
+               out[k] = (int) b[k];
+            }
+         }
+      };
+      std::vector<int> rows, cols;
+      read_idx(rows, "Could not read all row indices");
+      read_idx(cols, "Could not read all column indices");
+      std::vector<double> vals(nnz);
+      if (h[2] == 4) {
+         std::vector<float> b(nnz);
+         read_exact_n(fc.fp, b.data(), sizeof(*b.data()), nnz, "Could not read all matrix coefficients");
+         for (size_t k = 0; k < nnz; k++) { vals[k] = (double) b[k]; }
+      } else {
+         read_exact_n(fc.fp, vals.data(), sizeof(*vals.data()), nnz, "Could not read all matrix coefficients");
+      }
+      HYPRE_IJMatrix m = nullptr;
+      HYPRE_IJMatrixCreate(comm, (int) h[7], (int) h[8], (int) h[9], (int) h[10], &m);
+      if (g_error_flag) { fail(HYPRE_ERROR_GENERIC, "IJ matrix create failed"); }
+      HYPRE_IJMatrixSetObjectType(m, type);
+      HYPRE_IJMatrixInitialize_v2(m, HYPRE_MEMORY_HOST);
+      if (nnz) {
+         std::vector<int> ones(nnz, 1);
+         ij_stage(m, (int) nnz, ones.data(), rows.data(), cols.data(), vals.data(), 0);
+      }
       HYPRE_IJMatrixAssemble(m);
       *matrix = m;
+   });
+}
+
+// src/parcsr_mv/par_csr_matrix.c:1120-1370 hypre_ParCSRMatrixPrintBinaryIJ
+HYPRE_Int HYPRE_IJMatrixPrintBinary(HYPRE_IJMatrix m, const char *filename)
+{
+   return guard([&] {
+      if (!m || m->object_type != HYPRE_PARCSR) { g_error_arg = 1; fail(HYPRE_ERROR_ARG, "bad IJ matrix"); }
+      if (!m->assembled) { fail(HYPRE_ERROR_GENERIC, "IJ matrix not assembled"); }
+      const std::string path = rank_file(filename, ".bin");
+      FileCloser fc{fopen(path.c_str(), "wb")};
+      if (!fc.fp) { fail(HYPRE_ERROR_GENERIC, "Could not open output file " + path); }
+      const int nrows = m->iupper - m->ilower + 1;
+      const size_t nnz = (size_t) m->h_ia[(size_t) nrows];
+      const uint64_t h[11] = {1, sizeof(HYPRE_BigInt), sizeof(HYPRE_Complex), (uint64_t) nrows,
+                              (uint64_t) (m->jupper - m->jlower + 1), (uint64_t) nnz, (uint64_t) nnz,
+                              (uint64_t) m->ilower, (uint64_t) m->iupper, (uint64_t) m->jlower,
+                              (uint64_t) m->jupper};
+      std::vector<HYPRE_BigInt> ri(nnz), ci(nnz);
+      for (int i = 0; i < nrows; i++) {
+         for (int q = m->h_ia[(size_t) i]; q < m->h_ia[(size_t) i + 1]; q++) {
+            ri[(size_t) q] = i + m->ilower;
+            ci[(size_t) q] = m->h_ja[(size_t) q] + m->jlower;
+         }
+      }
+      if (fwrite(h, sizeof(uint64_t), 11, fc.fp) != 11 ||
+          fwrite(ri.data(), sizeof(HYPRE_BigInt), nnz, fc.fp) != nnz ||
+          fwrite(ci.data(), sizeof(HYPRE_BigInt), nnz, fc.fp) != nnz ||
+          fwrite(m->h_a.data(), sizeof(HYPRE_Complex), nnz, fc.fp) != nnz) {
+         fail(HYPRE_ERROR_GENERIC, "Could not write the binary IJ matrix");
+      }
    });
 }
 
@@ -656,17 +841,116 @@ HYPRE_Int HYPRE_IJVectorSetMaxOffProcElmts(HYPRE_IJVector v, HYPRE_Int m)
    return g_error_flag;
 }
 
+// src/IJ_mv/HYPRE_IJVector.c:862-925 (text) and par_vector.c:1360-1440 (binary)
+static std::vector<double> ijv_values(hypre_IJVector_struct *v)
+{
+   if (!v) { g_error_arg = 1; fail(HYPRE_ERROR_ARG, "NULL IJ vector"); }
+   if (!v->par) { fail(HYPRE_ERROR_GENERIC, "IJ vector not initialized"); }
+   const int n = v->jupper - v->jlower + 1;
+   std::vector<double> h((size_t) std::max(n, 0));
+   if (n > 0) { d2h(h.data(), v->par->d, sizeof(double) * (size_t) n); }
+   return h;
+}
+
 HYPRE_Int HYPRE_IJVectorPrint(HYPRE_IJVector v, const char *filename)
 {
    return guard([&] {
-      const int n = v->jupper - v->jlower + 1;
-      std::vector<double> h((size_t) n);
-      d2h(h.data(), v->par->d, sizeof(double) * (size_t) n);
-      FILE *fp = fopen(filename, "w");
-      if (!fp) { fail(HYPRE_ERROR_GENERIC, "cannot open file"); }
-      fprintf(fp, "%d %d\n", v->jlower, v->jupper);
-      for (int i = 0; i < n; i++) { fprintf(fp, "%d %.15e\n", i + v->jlower, h[(size_t) i]); }
-      fclose(fp);
+      const std::vector<double> h = ijv_values(v);
+      const std::string path = rank_file(filename, "");
+      FileCloser fc{fopen(path.c_str(), "w")};
+      if (!fc.fp) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "cannot open " + path); }
+      fprintf(fc.fp, "%d %d\n", v->jlower, v->jupper);
+      for (size_t i = 0; i < h.size(); i++) { fprintf(fc.fp, "%d %.14e\n", (int) i + v->jlower, h[i]); }
+   });
+}
+
+HYPRE_Int HYPRE_IJVectorPrintBinary(HYPRE_IJVector v, const char *filename)
+{
+   return guard([&] {
+      const std::vector<double> h = ijv_values(v);
+      const std::string path = rank_file(filename, ".bin");
+      FileCloser fc{fopen(path.c_str(), "wb")};
+      if (!fc.fp) { fail(HYPRE_ERROR_GENERIC, "Could not open output file " + path); }
+      // version, value bytes, partitioning [jlower, jupper+1), global size,
+      // local size, components, storage method
+      const uint64_t hd[8] = {1, sizeof(HYPRE_Complex), (uint64_t) v->jlower,
+                              (uint64_t) (v->jupper + 1), (uint64_t) h.size(), (uint64_t) h.size(), 1,
+                              0};
+      if (fwrite(hd, sizeof(uint64_t), 8, fc.fp) != 8 ||
+          fwrite(h.data(), sizeof(double), h.size(), fc.fp) != h.size()) {
+         fail(HYPRE_ERROR_GENERIC, "Could not write the binary IJ vector");
+      }
+   });
+}
+
+// src/IJ_mv/HYPRE_IJVector.c:785-842 HYPRE_IJVectorRead
+HYPRE_Int HYPRE_IJVectorRead(const char *filename, MPI_Comm comm, HYPRE_Int type,
+                             HYPRE_IJVector *vector)
+{
+   return guard([&] {
+      if (!filename) { g_error_arg = 1; fail(HYPRE_ERROR_ARG, "NULL file name"); }
+      if (!vector) { g_error_arg = 4; fail(HYPRE_ERROR_ARG, "NULL vector pointer"); }
+      const std::string path = rank_file(filename, "");
+      FileCloser fc{fopen(path.c_str(), "r")};
+      if (!fc.fp) { g_error_arg = 1; fail(HYPRE_ERROR_ARG, "cannot open " + path); }
+      int jl, ju;
+      if (fscanf(fc.fp, "%d %d", &jl, &ju) != 2) { fail(HYPRE_ERROR_GENERIC, "Error in IJ vector input file."); }
+      std::vector<int> idx;
+      std::vector<double> val;
+      int j, ret;
+      double x;
+      while ((ret = fscanf(fc.fp, "%d%*[ \t]%le", &j, &x)) != EOF) {
+         if (ret != 2) { fail(HYPRE_ERROR_GENERIC, "Error in IJ vector input file."); }
+         idx.push_back(j);
+         val.push_back(x);
+      }
+      HYPRE_IJVector vec = nullptr;
+      HYPRE_IJVectorCreate(comm, jl, ju, &vec);
+      if (g_error_flag) { fail(HYPRE_ERROR_GENERIC, "IJ vector create failed"); }
+      HYPRE_IJVectorSetObjectType(vec, type);
+      HYPRE_IJVectorInitialize_v2(vec, HYPRE_MEMORY_HOST);
+      if (!idx.empty()) { ijv_set(vec, (int) idx.size(), idx.data(), val.data(), false); }
+      HYPRE_IJVectorAssemble(vec);
+      *vector = vec;
+   });
+}
+
+// src/IJ_mv/IJVector.c:87-245 hypre_IJVectorReadBinary
+HYPRE_Int HYPRE_IJVectorReadBinary(const char *filename, MPI_Comm comm, HYPRE_Int type,
+                                   HYPRE_IJVector *vector)
+{
+   return guard([&] {
+      if (!filename) { g_error_arg = 1; fail(HYPRE_ERROR_ARG, "NULL file name"); }
+      if (!vector) { g_error_arg = 4; fail(HYPRE_ERROR_ARG, "NULL vector pointer"); }
+      const std::string path = rank_file(filename, ".bin");
+      FileCloser fc{fopen(path.c_str(), "rb")};
+      if (!fc.fp) { fail(HYPRE_ERROR_GENERIC, "Could not open input file " + path); }
+      uint64_t h[8];
+      read_exact_n(fc.fp, h, sizeof(*h), 8, "Could not read header entries");
+      if (h[0] != 1) { fail(HYPRE_ERROR_GENERIC, "Unsupported header version"); }
+      if (h[7] == 1) { fail(HYPRE_ERROR_GENERIC, "row-wise ordering is not supported!"); }
+      if (h[5] > h[4]) { fail(HYPRE_ERROR_GENERIC, "Invalid vector size!"); }
+      if (h[6] != 1) { fail(HYPRE_ERROR_GENERIC, "multi-component vectors are not supported"); }
+      if (h[3] > (uint64_t) INT32_MAX + 1 || h[2] > h[3]) { fail(HYPRE_ERROR_GENERIC, "bad partitioning"); }
+      const size_t n = (size_t) h[5];
+      std::vector<double> val(n);
+      if (h[1] == 4) {
+         std::vector<float> b(n);
+         read_exact_n(fc.fp, b.data(), sizeof(*b.data()), n, "Could not read all vector coefficients");
+         for (size_t k = 0; k < n; k++) { val[k] = (double) b[k]; }
+      } else if (h[1] == 8) {
+         read_exact_n(fc.fp, val.data(), sizeof(*val.data()), n, "Could not read all vector coefficients");
+      } else {
+         fail(HYPRE_ERROR_GENERIC, "Unsupported data type for vector entries");
+      }
+      HYPRE_IJVector vec = nullptr;
+      HYPRE_IJVectorCreate(comm, (int) h[2], (int) (h[3] - 1), &vec);
+      if (g_error_flag) { fail(HYPRE_ERROR_GENERIC, "IJ vector create failed"); }
+      HYPRE_IJVectorSetObjectType(vec, type);
+      HYPRE_IJVectorInitialize_v2(vec, HYPRE_MEMORY_HOST);
+      if (n) { ijv_set(vec, (int) n, nullptr, val.data(), false); }
+      HYPRE_IJVectorAssemble(vec);
+      *vector = vec;
    });
 }
 

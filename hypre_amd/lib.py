@@ -47,6 +47,17 @@ SIGNATURES = {
     "HYPRE_IJVectorAssemble": (I, [P]),
     "HYPRE_IJVectorGetValues": (I, [P, I, IP, DP]),
     "HYPRE_IJVectorGetObject": (I, [P, PP]),
+    "HYPRE_IJVectorGetLocalRange": (I, [P, IP, IP]),
+    # IJ files
+    "HYPRE_IJMatrixRead": (I, [C.c_char_p, I, I, PP]),
+    "HYPRE_IJMatrixReadMM": (I, [C.c_char_p, I, I, PP]),
+    "HYPRE_IJMatrixReadBinary": (I, [C.c_char_p, I, I, PP]),
+    "HYPRE_IJMatrixPrint": (I, [P, C.c_char_p]),
+    "HYPRE_IJMatrixPrintBinary": (I, [P, C.c_char_p]),
+    "HYPRE_IJVectorRead": (I, [C.c_char_p, I, I, PP]),
+    "HYPRE_IJVectorReadBinary": (I, [C.c_char_p, I, I, PP]),
+    "HYPRE_IJVectorPrint": (I, [P, C.c_char_p]),
+    "HYPRE_IJVectorPrintBinary": (I, [P, C.c_char_p]),
     # ParCSR / ParVector
     "HYPRE_ParCSRMatrixDestroy": (I, [P]),
     "HYPRE_ParCSRMatrixMatvec": (I, [D, P, P, D, P]),

@@ -62,10 +62,21 @@ HYPRE_Int HYPRE_IJMatrixGetRowCounts(HYPRE_IJMatrix matrix, HYPRE_Int nrows,
 HYPRE_Int HYPRE_IJMatrixGetLocalRange(HYPRE_IJMatrix matrix, HYPRE_BigInt *ilower,
                                       HYPRE_BigInt *iupper, HYPRE_BigInt *jlower,
                                       HYPRE_BigInt *jupper);
-/* :1150 / :1170 -- hypre's text format "<row> <col> <value>" per line */
+/* :1201 / :1150 -- hypre's text format: "<filename>.00000", a line
+ * "ilower iupper jlower jupper", then "<row> <col> <value>" per entry */
 HYPRE_Int HYPRE_IJMatrixPrint(HYPRE_IJMatrix matrix, const char *filename);
 HYPRE_Int HYPRE_IJMatrixRead(const char *filename, MPI_Comm comm, HYPRE_Int type,
                              HYPRE_IJMatrix *matrix);
+/* :1186 -- MatrixMarket coordinate file (real or integer; general or
+ * symmetric, the symmetric triangle mirrored); the name is used as given */
+HYPRE_Int HYPRE_IJMatrixReadMM(const char *filename, MPI_Comm comm, HYPRE_Int type,
+                               HYPRE_IJMatrix *matrix);
+/* :1223 / :1168 -- hypre's binary format "<filename>.00000.bin": 11 uint64
+ * header words, then row indices, column indices (4 or 8 bytes) and values
+ * (4 or 8 bytes) of every entry; written with 4-byte indices and 8-byte values */
+HYPRE_Int HYPRE_IJMatrixPrintBinary(HYPRE_IJMatrix matrix, const char *filename);
+HYPRE_Int HYPRE_IJMatrixReadBinary(const char *filename, MPI_Comm comm, HYPRE_Int type,
+                                   HYPRE_IJMatrix *matrix);
 
 /* src/IJ_mv/HYPRE_IJVector.c:23 */
 HYPRE_Int HYPRE_IJVectorCreate(MPI_Comm comm, HYPRE_BigInt jlower, HYPRE_BigInt jupper,
@@ -94,7 +105,16 @@ HYPRE_Int HYPRE_IJVectorGetLocalRange(HYPRE_IJVector vector, HYPRE_BigInt *jlowe
 HYPRE_Int HYPRE_IJVectorSetNumComponents(HYPRE_IJVector vector, HYPRE_Int num_components);
 HYPRE_Int HYPRE_IJVectorSetComponent(HYPRE_IJVector vector, HYPRE_Int component);
 HYPRE_Int HYPRE_IJVectorSetMaxOffProcElmts(HYPRE_IJVector vector, HYPRE_Int max_off_proc_elmts);
+/* src/IJ_mv/HYPRE_IJVector.c:862 / :785 -- "<filename>.00000": "jlower jupper",
+ * then "<index> <value>" per entry */
 HYPRE_Int HYPRE_IJVectorPrint(HYPRE_IJVector vector, const char *filename);
+HYPRE_Int HYPRE_IJVectorRead(const char *filename, MPI_Comm comm, HYPRE_Int type,
+                             HYPRE_IJVector *vector);
+/* :927 / :849 -- "<filename>.00000.bin": 8 uint64 header words (version,
+ * value bytes, partitioning, sizes, components, storage), then the values */
+HYPRE_Int HYPRE_IJVectorPrintBinary(HYPRE_IJVector vector, const char *filename);
+HYPRE_Int HYPRE_IJVectorReadBinary(const char *filename, MPI_Comm comm, HYPRE_Int type,
+                                   HYPRE_IJVector *vector);
 
 #ifdef __cplusplus
 }
