@@ -145,8 +145,18 @@ __device__ __forceinline__ X rows_dot_window(int r0, int nr, int lane, const int
       }
       wave_fence();
       if (lane < nr) {
+         // four LDS reads per wait, added in storage order
          const int s = max(rs, wb), e = min(re, we);
-         for (int k = s; k < e; k++) { t += sbuf[k - wb]; }
+         const X *q = sbuf - wb;
+         int k = s;
+         for (; k + 4 <= e; k += 4) {
+            const X p0 = q[k], p1 = q[k + 1], p2 = q[k + 2], p3 = q[k + 3];
+            t += p0;
+            t += p1;
+            t += p2;
+            t += p3;
+         }
+         for (; k < e; k++) { t += q[k]; }
       }
       wave_fence();
    }
