@@ -133,3 +133,26 @@ def test_pcg_iterations_match_oracle(hx, oracle, gen):
     np.testing.assert_allclose(pcg.history()[:3], hist_or[:3], rtol=1e-8)
     r = b_h - A_or.to_scipy() @ x.get()
     assert np.linalg.norm(r) / np.linalg.norm(b_h) < 1e-8
+
+
+@pytest.mark.parametrize("down,up", [(13, 14), (8, 8)])
+def test_fp32_cycle_with_hybrid_gs(hx, oracle, down, up):
+    # the FP32 V-cycle (hybrid GS sweeps in FP32, row-reversed FP32 copies)
+    # inside FP64 PCG reaches the FP64 tolerance within one iteration of FP64
+    say("fp32 cycle hybrid GS", down, up)
+    A_or = oracle.gen_laplace27(30, 30, 30)
+    n = A_or.nrows
+    b_h = oracle.ij_rhsrand(n)
+    A = hx.ParCSRMatrix.from_csr(A_or.ia, A_or.ja, A_or.a)
+    its = {}
+    for fp32 in (0, 1):
+        b = hx.ParVector(n, b_h)
+        x = hx.ParVector(n)
+        amg = hx.BoomerAMG(max_row_sum=1.0, relax_type=down, relax_up=up, fp32_cycle=fp32)
+        pcg = hx.PCG(amg, tol=1e-8, max_iter=200, two_norm=1)
+        pcg.setup(A, b, x)
+        pcg.solve(A, b, x)
+        r = b_h - A_or.to_scipy() @ x.get()
+        assert np.linalg.norm(r) / np.linalg.norm(b_h) < 1e-8
+        its[fp32] = pcg.num_iterations()
+    assert its[1] <= its[0] + 1, its
