@@ -103,7 +103,7 @@ def main():
                     help="smoother: 18 l1-Jacobi (default); 13/14 hybrid l1 Gauss-Seidel "
                          "forward down / backward up (the reference's host default); 8 "
                          "symmetric hybrid l1 Gauss-Seidel; 3/4 and 6 without l1 norms")
-    ap.add_argument("--hgs-rows", type=int, default=8,
+    ap.add_argument("--hgs-rows", type=int, default=4,
                     help="rows per hybrid Gauss-Seidel partition (automatic partitioning)")
     ap.add_argument("--hgs-parts", type=int, default=0,
                     help="fixed number of hybrid Gauss-Seidel partitions on every level "

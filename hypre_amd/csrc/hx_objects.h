@@ -78,7 +78,7 @@ struct AMGParams
    int    cycle_fp32 = 0;
    int    keep_setup_data = 0;
    int    hgs_parts = 0;         // hybrid GS partitions on every level (the reference's thread count)
-   int    hgs_rows = 8;          // else partitions of about this many rows
+   int    hgs_rows = 4;          // else partitions of about this many rows
    std::vector<double> level_theta, level_omega;
    std::vector<int>    level_sweeps;
 };

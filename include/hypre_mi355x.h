@@ -37,7 +37,7 @@ HYPRE_Int HYPRE_MI355X_BoomerAMGClearLevelParams(HYPRE_Solver solver);
  * which relax each partition in Gauss-Seidel order and couple partitions
  * through the iterate before the sweep, like the reference's OpenMP threads:
  * parts > 0 uses that many partitions on every level (the reference run with
- * `parts` threads); parts == 0 uses partitions of `rows` rows (default 8),
+ * `parts` threads); parts == 0 uses partitions of `rows` rows (default 4),
  * fewer (down to 8) on levels with fewer than about 128K partitions. */
 HYPRE_Int HYPRE_MI355X_BoomerAMGSetHybridPartitions(HYPRE_Solver solver, HYPRE_Int parts,
                                                     HYPRE_Int rows);

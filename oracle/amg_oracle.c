@@ -1055,7 +1055,7 @@ void orc_amg_default_params(orc_amg_params *p)
    p->relax_up_type = -1;
    p->outer_weight = 1.0;
    p->hgs_parts = 0;
-   p->hgs_rows = 8;
+   p->hgs_rows = 4;
 }
 
 static void csr_copy(const orc_csr *A, orc_csr *B)

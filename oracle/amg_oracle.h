@@ -125,7 +125,7 @@ typedef struct
    int    relax_up_type;      /* -1: relax_type on the up cycle too */
    double outer_weight;       /* 1.0 (hypre's omega) */
    int    hgs_parts;          /* hybrid GS partitions; 0: about hgs_rows rows each */
-   int    hgs_rows;           /* 8 */
+   int    hgs_rows;           /* 4 */
 } orc_amg_params;
 
 void orc_amg_default_params(orc_amg_params *p);
