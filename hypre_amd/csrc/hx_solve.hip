@@ -56,17 +56,21 @@ __device__ __forceinline__ X row_dot(int s, int e, int lane, const int *__restri
 }
 
 // ---------------------------------------------------------------- windowed row sums
-// A wave owns RW consecutive rows.  Their nonzeros form one contiguous range,
-// read in windows of SW entries with fully coalesced loads (SW/64 independent
-// a/ja loads and x gathers per lane in flight); the products go to a per-wave
-// LDS buffer and lane l (< RW) adds up row l's products in storage order,
-// i.e. t = ((0 + a_0 x_0) + a_1 x_1) + ..., the order of hypre's host CSR
-// matvec (src/seq_mv/csr_matvec.c).
+// A wave owns a contiguous range of rows and streams their nonzeros, one
+// contiguous range of the CSR arrays, in windows of SW entries with fully
+// coalesced loads: lane l loads entries 2l and 2l+1 of each 128-entry slice
+// (8-byte column-index and 16-byte value loads), gathers their x values and
+// writes the products to a per-wave LDS buffer.  The rows are taken RW at a
+// time; lane l (< RW) owns row l of the current block and adds up its
+// products in storage order, i.e. t = ((0 + a_0 x_0) + a_1 x_1) + ..., the
+// order of hypre's host CSR matvec (src/seq_mv/csr_matvec.c).  A block whose
+// rows end inside a window is finished there and the next block continues in
+// the same window, so the stream never restarts between row blocks.
 constexpr int SW = 512;          // default window (entries); 1024 selectable
 constexpr int WPB = BS / 64;   // waves per block
 
-// rows per wave: one row per lane (windows are pipelined), fewer only for
-// very long rows so that a lane's serial sum stays short
+// rows per block: one row per lane, fewer only for very long rows so that a
+// lane's serial sum stays short
 static int pick_rw(long long nnz, int n)
 {
    double avg = n > 0 ? (double) nnz / n : 1.0;
@@ -94,73 +98,152 @@ static int spmv_window(long long nnz, int n)
    return w;
 }
 
-// The matrix is streamed once: its loads are non-temporal so that x stays in
-// L2.  The a/ja loads of the next window are issued before the current
-// window's products are summed, so a wave always has a window in flight.
-template <int RW, int SW, typename V, typename X>
-__device__ __forceinline__ X rows_dot_window(int r0, int nr, int lane, const int *__restrict__ ia,
-                                             const int *__restrict__ ja, const V *__restrict__ a,
-                                             const X *__restrict__ x, X *sbuf)
+typedef int hx_int2 __attribute__((ext_vector_type(2)));
+typedef double hx_dbl2 __attribute__((ext_vector_type(2)));
+typedef float hx_flt2 __attribute__((ext_vector_type(2)));
+
+// entries p and p+1 (p even) of the column indices and values
+template <typename V>
+__device__ __forceinline__ void load_pair(const int *__restrict__ ja, const V *__restrict__ a, int p,
+                                          int (&j)[2], V (&v)[2])
 {
-   constexpr int U = SW / 64;
+   const hx_int2 jj = __builtin_nontemporal_load(reinterpret_cast<const hx_int2 *>(ja + p));
+   j[0] = jj.x;
+   j[1] = jj.y;
+   if constexpr (sizeof(V) == 8) {
+      const hx_dbl2 vv = __builtin_nontemporal_load(reinterpret_cast<const hx_dbl2 *>(a + p));
+      v[0] = vv.x;
+      v[1] = vv.y;
+   } else {
+      const hx_flt2 vv = __builtin_nontemporal_load(reinterpret_cast<const hx_flt2 *>(a + p));
+      v[0] = vv.x;
+      v[1] = vv.y;
+   }
+}
+
+// Rows [R0, R1) of A times x; epi(row, t) is called once per row, by the lane
+// that owns it, with t the row's sum.  The matrix is streamed once: its loads
+// are non-temporal so that x stays in L2, and the next window's loads are
+// issued before the current window's products are summed.  Windows start at
+// the even entry at or below the range's first entry; pair loads past the
+// range read the last pair inside it (every CSR array is allocated with at
+// least one element of slack) and are masked when used.
+template <int RW, int SW, typename V, typename X, class Epi>
+__device__ __forceinline__ void rows_stream(int R0, int R1, int lane, const int *__restrict__ ia,
+                                            const int *__restrict__ ja, const V *__restrict__ a,
+                                            const X *__restrict__ x, X *sbuf, Epi epi)
+{
+   constexpr int G = SW / 128;
+   if (R0 >= R1) { return; }
+   int rb = R0;
+   int nr = min(RW, R1 - rb);
    int rs = 0, re = 0;
    if (lane < nr) {
-      rs = ia[r0 + lane];
-      re = ia[r0 + lane + 1];
+      rs = ia[rb + lane];
+      re = ia[rb + lane + 1];
+   }
+   // bounds of the next block, loaded one block ahead
+   int nr2 = min(RW, R1 - (rb + RW));
+   int rs2 = 0, re2 = 0;
+   if (lane < nr2) {
+      rs2 = ia[rb + RW + lane];
+      re2 = ia[rb + RW + lane + 1];
    }
    const int base = __shfl(rs, 0, 64);
-   const int end = __shfl(re, nr - 1, 64);
+   const int end = ia[R1];
+   int bend = __shfl(re, nr - 1, 64);
    X t = 0;
-   if (base >= end) { return t; }
-   // Loads are unconditional (positions past the range read its last entry
-   // and are masked when used) so that the compiler issues each batch of
-   // matrix loads and of x gathers back to back and waits once per batch.
-   int jn[U];
-   V an[U];
-#pragma unroll
-   for (int u = 0; u < U; u++) {
-      const int j = min(base + u * 64 + lane, end - 1);
-      jn[u] = __builtin_nontemporal_load(ja + j);
-      an[u] = __builtin_nontemporal_load(a + j);
+   if (base >= end) {
+      // no entries: every row sums to zero
+      while (nr > 0) {
+         if (lane < nr) { epi(rb + lane, t); }
+         rb += RW;
+         nr = min(RW, R1 - rb);
+      }
+      return;
    }
-   for (int wb = base; wb < end; wb += SW) {
-      const int we = min(end, wb + SW);
-      X xv[U];
+   const int wb0 = base & ~1;
+   const int last = (end - 1) & ~1;
+   // Loads are unconditional (clamped to the last pair) so that the compiler
+   // issues each batch of matrix loads and of x gathers back to back and
+   // waits once per batch.
+   int jn[G][2];
+   V an[G][2];
 #pragma unroll
-      for (int u = 0; u < U; u++) { xv[u] = x[jn[u]]; }
+   for (int g = 0; g < G; g++) { load_pair<V>(ja, a, min(wb0 + 2 * (g * 64 + lane), last), jn[g], an[g]); }
+   for (int wb = wb0; wb < end; wb += SW) {
+      const int we = min(end, wb + SW);
+      X xv[G][2];
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+#pragma unroll
+         for (int c = 0; c < 2; c++) {
+            const int q = wb + 2 * (g * 64 + lane) + c;
+            xv[g][c] = x[(q < we) ? jn[g][c] : 0];
+         }
+      }
       // products straight to LDS (no register copy), then the next window's loads
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-         const int j = wb + u * 64 + lane;
-         sbuf[u * 64 + lane] = (j < we) ? (X) an[u] * xv[u] : (X) 0;
+      for (int g = 0; g < G; g++) {
+#pragma unroll
+         for (int c = 0; c < 2; c++) {
+            const int q = wb + 2 * (g * 64 + lane) + c;
+            sbuf[2 * (g * 64 + lane) + c] = (q < we) ? (X) an[g][c] * xv[g][c] : (X) 0;
+         }
       }
       const int nxt = wb + SW;
       if (nxt < end) {
 #pragma unroll
-         for (int u = 0; u < U; u++) {
-            const int j = min(nxt + u * 64 + lane, end - 1);
-            jn[u] = __builtin_nontemporal_load(ja + j);
-            an[u] = __builtin_nontemporal_load(a + j);
+         for (int g = 0; g < G; g++) {
+            load_pair<V>(ja, a, min(nxt + 2 * (g * 64 + lane), last), jn[g], an[g]);
          }
       }
       wave_fence();
-      if (lane < nr) {
-         // four LDS reads per wait, added in storage order
-         const int s = max(rs, wb), e = min(re, we);
-         const X *q = sbuf - wb;
-         int k = s;
-         for (; k + 4 <= e; k += 4) {
-            const X p0 = q[k], p1 = q[k + 1], p2 = q[k + 2], p3 = q[k + 3];
-            t += p0;
-            t += p1;
-            t += p2;
-            t += p3;
+      const X *q = sbuf - wb;
+      while (true) {
+         if (lane < nr) {
+            // four LDS reads per wait, added in storage order
+            const int s = max(rs, wb), e = min(re, we);
+            int k = s;
+            for (; k + 4 <= e; k += 4) {
+               const X p0 = q[k], p1 = q[k + 1], p2 = q[k + 2], p3 = q[k + 3];
+               t += p0;
+               t += p1;
+               t += p2;
+               t += p3;
+            }
+            for (; k < e; k++) { t += q[k]; }
          }
-         for (; k < e; k++) { t += q[k]; }
+         if (bend > we) { break; }
+         // the block's rows end in this window
+         if (lane < nr) { epi(rb + lane, t); }
+         t = 0;
+         rb += RW;
+         nr = nr2;
+         rs = rs2;
+         re = re2;
+         if (nr <= 0) { break; }
+         bend = __shfl(re, nr - 1, 64);
+         nr2 = min(RW, R1 - (rb + RW));
+         if (lane < nr2) {
+            rs2 = ia[rb + RW + lane];
+            re2 = ia[rb + RW + lane + 1];
+         }
       }
       wave_fence();
    }
-   return t;
+}
+
+// rows of wave wv of block blk: ceil(groups / waves) blocks of RW rows each
+template <int RW>
+__device__ __forceinline__ void wave_rows(int n, int blk, int wv, int nb, int &R0, int &R1)
+{
+   const long long ng = ((long long) n + RW - 1) / RW;
+   const long long nw = (long long) nb * WPB;
+   const long long per = (ng + nw - 1) / nw * RW;
+   const long long r0 = ((long long) blk * WPB + wv) * per;
+   R0 = (int) (r0 < n ? r0 : n);
+   R1 = (int) (r0 + per < n ? r0 + per : n);
 }
 
 template <int RW, int SW, typename V, typename X, bool DOT>
@@ -174,21 +257,14 @@ __global__ __launch_bounds__(BS) void k_spmv_w(int n, const int *__restrict__ ia
    const int blk = xcd_block(blockIdx.x, nb);
    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
    X *sbuf = sbuf_all + wv * SW;
-   const int ng = (n + RW - 1) / RW;
-   const int gpb = (ng + nb - 1) / nb;
-   const int g0 = blk * gpb, g1 = min(ng, g0 + gpb);
+   int R0, R1;
+   wave_rows<RW>(n, blk, wv, nb, R0, R1);
    double acc = 0.0;
-   for (int g = g0 + wv; g < g1; g += WPB) {
-      const int r0 = g * RW;
-      const int nr = min(RW, n - r0);
-      const X t = rows_dot_window<RW, SW, V, X>(r0, nr, lane, ia, ja, a, x, sbuf);
-      if (lane < nr) {
-         const int row = r0 + lane;
-         const X v = (beta == (X) 0) ? alpha * t : alpha * t + beta * b[row];
-         y[row] = v;
-         if (DOT) { acc += (double) v * (double) x[row]; }
-      }
-   }
+   rows_stream<RW, SW, V, X>(R0, R1, lane, ia, ja, a, x, sbuf, [&](int row, X t) {
+      const X v = (beta == (X) 0) ? alpha * t : alpha * t + beta * b[row];
+      y[row] = v;
+      if (DOT) { acc += (double) v * (double) x[row]; }
+   });
    if (DOT) {
       __shared__ double sh[BS];
       sh[threadIdx.x] = acc;
@@ -343,19 +419,12 @@ __global__ __launch_bounds__(BS) void k_relax_l1_w(int n, const int *__restrict_
    const int blk = xcd_block(blockIdx.x, nb);
    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
    X *sbuf = sbuf_all + wv * SW;
-   const int ng = (n + RW - 1) / RW;
-   const int gpb = (ng + nb - 1) / nb;
-   const int g0 = blk * gpb, g1 = min(ng, g0 + gpb);
-   for (int g = g0 + wv; g < g1; g += WPB) {
-      const int r0 = g * RW;
-      const int nr = min(RW, n - r0);
-      const X t = rows_dot_window<RW, SW, V, X>(r0, nr, lane, ia, ja, a, u, sbuf);
-      if (lane < nr) {
-         const int row = r0 + lane;
-         const X v = w * f[row] - w * t;
-         uo[row] = u[row] + v / l1[row];
-      }
-   }
+   int R0, R1;
+   wave_rows<RW>(n, blk, wv, nb, R0, R1);
+   rows_stream<RW, SW, V, X>(R0, R1, lane, ia, ja, a, u, sbuf, [&](int row, X t) {
+      const X v = w * f[row] - w * t;
+      uo[row] = u[row] + v / l1[row];
+   });
 }
 
 // zero initial guess: u_out = (0 + w f) / l1, one thread per row
