@@ -362,6 +362,7 @@ __global__ void k_strength_recount(int n, const int *__restrict__ ia, const doub
 // S^T), aggregated per block in an LDS table over SF_ROWS consecutive rows.
 constexpr int SF_ROWS = 256;
 constexpr int SF_SLOTS = 4096;
+constexpr int SF_U = 4;
 
 template <int K>
 __global__ __launch_bounds__(256) void k_strength_fill(int n, const int *__restrict__ ia,
@@ -395,34 +396,48 @@ __global__ __launch_bounds__(256) void k_strength_fill(int n, const int *__restr
       const bool neg = fl & 1, allweak = fl & 2;
       const int o = act ? S_ia[i] : 0;
       int c = 0;
-      for (int kb = s + 1; __any(kb < e); kb += K) {
-         const int k = kb + gl;
-         bool strong = false;
-         int j = -1;
-         if (k < e && !allweak) {
-            const double v = a[k];
-            strong = neg ? !(v <= t) : !(v >= t);
-            if (strong) { j = ja[k]; }
+      // SF_U strided steps of the row per batch: every load of a batch is
+      // issued before any is used (positions past the row end read its last
+      // entry and are masked)
+      const int klast = max(e - 1, 0);
+      for (int kb = s + 1; __any(kb < e); kb += SF_U * K) {
+         double vv[SF_U];
+         int jj[SF_U];
+#pragma unroll
+         for (int u = 0; u < SF_U; u++) {
+            const int kq = min(kb + gl + u * K, klast);
+            vv[u] = a[kq];
+            jj[u] = ja[kq];
          }
-         const unsigned long long m = (__ballot(strong) >> gshift) & gmask;
-         if (strong) {
-            S_ja[o + c + __popcll(m & ((1ULL << gl) - 1))] = j;
-            if (stc) {
-               unsigned h = ((unsigned) j * 2654435761u) >> (32 - ilog2(SF_SLOTS));
-               bool done = false;
-               for (int probe = 0; probe < 32; probe++) {
-                  const int old = atomicCAS(&hkey[h], -1, j);
-                  if (old == -1 || old == j) {
-                     atomicAdd(&hcnt[h], 1);
-                     done = true;
-                     break;
-                  }
-                  h = (h + 1) & (SF_SLOTS - 1);
-               }
-               if (!done) { atomicAdd(&stc[j], 1); }
+#pragma unroll
+         for (int u = 0; u < SF_U; u++) {
+            const int k = kb + gl + u * K;
+            bool strong = false;
+            const int j = jj[u];
+            if (k < e && !allweak) {
+               const double v = vv[u];
+               strong = neg ? !(v <= t) : !(v >= t);
             }
+            const unsigned long long m = (__ballot(strong) >> gshift) & gmask;
+            if (strong) {
+               S_ja[o + c + __popcll(m & ((1ULL << gl) - 1))] = j;
+               if (stc) {
+                  unsigned h = ((unsigned) j * 2654435761u) >> (32 - ilog2(SF_SLOTS));
+                  bool done = false;
+                  for (int probe = 0; probe < 32; probe++) {
+                     const int old = atomicCAS(&hkey[h], -1, j);
+                     if (old == -1 || old == j) {
+                        atomicAdd(&hcnt[h], 1);
+                        done = true;
+                        break;
+                     }
+                     h = (h + 1) & (SF_SLOTS - 1);
+                  }
+                  if (!done) { atomicAdd(&stc[j], 1); }
+               }
+            }
+            c += __popcll(m);
          }
-         c += __popcll(m);
       }
    }
    if (stc) {
