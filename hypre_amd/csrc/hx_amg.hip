@@ -310,8 +310,9 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
    for (int l = 0; l < nl - 1; l++) {
       const Level &lv = L[l];
       const Csr &A = *lv.A;
-      const char *cls = (l == 0) ? "relax_A0" : "relax";
       for (int s = 0; s < sweeps(l, 1); s++) {
+         // level-0 sweeps from a zero guess are timed as a class of their own
+         const char *cls = (l == 0) ? (zero ? "relaxz_A0" : "relax_A0") : "relax";
          relax_any(prm.relax_type[1], l, A, F[l], cur[l], lv.l1, spare[l], zero, cls);
          std::swap(cur[l], spare[l]);
          zero = false;
@@ -362,8 +363,8 @@ void AMGSolver::cycle32(const double *f, double *u)
    bool zero = true;
    for (int l = 0; l < nl - 1; l++) {
       Level &lv = L[l];
-      const char *cls = (l == 0) ? "relax_A0" : "relax";
       for (int s = 0; s < sweeps(l, 1); s++) {
+         const char *cls = (l == 0) ? (zero ? "relaxz_A0" : "relax_A0") : "relax";
          relax_any32(prm.relax_type[1], l, lv.A32, lv.f32, cur[l], lv.l1f, spare[l], zero, cls);
          std::swap(cur[l], spare[l]);
          zero = false;

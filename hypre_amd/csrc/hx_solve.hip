@@ -946,8 +946,10 @@ static void launch_hgs(int type, const Csr &A, const Csr *Arev, int max_row, con
    parts = std::max(parts, 1);
    if (max_row < 0) { max_row = csr_max_row(A); }
    const double xs = sizeof(X);
-   const double bytes = ((double) A.nnz * (4.0 + sizeof(V) + (zero ? 0.0 : xs)) +
-                         4.0 * (A.nrows + 1) + xs * (zero ? 3.0 : 4.0) * A.nrows) *
+   // algorithmic bytes: the matrix once, the copy u once (not with a zero
+   // guess), f, l1 and the new u
+   const double bytes = ((double) A.nnz * (4.0 + sizeof(V)) + 4.0 * (A.nrows + 1) +
+                         xs * (zero ? 3.0 : 4.0) * A.nrows) *
                         (mode == 2 ? 2.0 : 1.0);
    TimedLaunch tl(cls, bytes);
    const X *l1p = l1t ? l1 : nullptr;
