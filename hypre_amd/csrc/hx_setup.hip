@@ -3005,17 +3005,128 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
 
 // ================================================================== transpose
 // src/seq_mv/csr_matop.c:1071 (stable counting sort): rows of R list the fine
-// rows in ascending order.  Fill with atomics, then sort each (short) row.
-__global__ void k_t_fill(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                         const double *__restrict__ a, int *__restrict__ fillp,
-                         int *__restrict__ Rj, double *__restrict__ Ra)
+// rows in ascending order.  Count the entries of each column and fill with
+// atomics, then sort each (short) row.
+// Both steps aggregate their atomics per chunk in an LDS table
+// keyed by column: counts are added once per distinct column of a chunk of
+// consecutive entries, and the fill reserves one range per distinct column of
+// a chunk of 256 rows, each entry taking its place in the range from an LDS
+// counter (entries that find no slot, or past the first TF_E of their row, use
+// the global counter directly).  The order inside a row of R is fixed by
+// k_t_sort afterwards.
+constexpr int TF_SLOTS = 4096;
+constexpr int TF_E = 8;
+constexpr int TC_U = 8;
+
+__device__ __forceinline__ unsigned tf_hash(int c)
 {
-   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-      for (int k = ia[i]; k < ia[i + 1]; k++) {
-         int pos = atomicAdd(&fillp[ja[k]], 1);
+   return ((unsigned) c * 2654435761u) >> (32 - ilog2(TF_SLOTS));
+}
+
+__global__ __launch_bounds__(256) void k_col_count_h(long long nnz, const int *__restrict__ ja,
+                                                     int *__restrict__ cnt)
+{
+   __shared__ int hkey[TF_SLOTS];
+   __shared__ int hval[TF_SLOTS];
+   constexpr long long CH = 256LL * TC_U;
+   for (long long c0 = (long long) blockIdx.x * CH; c0 < nnz; c0 += (long long) gridDim.x * CH) {
+      for (int q = threadIdx.x; q < TF_SLOTS; q += 256) {
+         hkey[q] = -1;
+         hval[q] = 0;
+      }
+      __syncthreads();
+      int jv[TC_U];
+#pragma unroll
+      for (int u = 0; u < TC_U; u++) {
+         const long long k = c0 + threadIdx.x + 256LL * u;
+         jv[u] = (k < nnz) ? ja[k] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < TC_U; u++) {
+         const int c = jv[u];
+         if (c < 0) { continue; }
+         unsigned h = tf_hash(c);
+         bool done = false;
+         for (int probe = 0; probe < 32; probe++) {
+            const int old = atomicCAS(&hkey[h], -1, c);
+            if (old == -1 || old == c) {
+               atomicAdd(&hval[h], 1);
+               done = true;
+               break;
+            }
+            h = (h + 1) & (TF_SLOTS - 1);
+         }
+         if (!done) { atomicAdd(&cnt[c], 1); }
+      }
+      __syncthreads();
+      for (int q = threadIdx.x; q < TF_SLOTS; q += 256) {
+         if (hkey[q] >= 0) { atomicAdd(&cnt[hkey[q]], hval[q]); }
+      }
+      __syncthreads();
+   }
+}
+
+__global__ __launch_bounds__(256) void k_t_fill_h(int n, const int *__restrict__ ia,
+                                                  const int *__restrict__ ja,
+                                                  const double *__restrict__ a,
+                                                  int *__restrict__ fillp, int *__restrict__ Rj,
+                                                  double *__restrict__ Ra)
+{
+   __shared__ int hkey[TF_SLOTS];
+   __shared__ int hval[TF_SLOTS];
+   for (long long r0 = (long long) blockIdx.x * 256; r0 < n; r0 += (long long) gridDim.x * 256) {
+      for (int q = threadIdx.x; q < TF_SLOTS; q += 256) {
+         hkey[q] = -1;
+         hval[q] = 0;
+      }
+      __syncthreads();
+      const long long il = r0 + threadIdx.x;
+      const int i = (int) il;
+      int s = 0, e = 0;
+      if (il < n) {
+         s = ia[i];
+         e = ia[i + 1];
+      }
+      int slot[TF_E], li[TF_E];
+#pragma unroll
+      for (int t = 0; t < TF_E; t++) {
+         slot[t] = -1;
+         li[t] = 0;
+         if (s + t < e) {
+            const int c = ja[s + t];
+            unsigned h = tf_hash(c);
+            for (int probe = 0; probe < 32; probe++) {
+               const int old = atomicCAS(&hkey[h], -1, c);
+               if (old == -1 || old == c) {
+                  slot[t] = (int) h;
+                  li[t] = atomicAdd(&hval[h], 1);
+                  break;
+               }
+               h = (h + 1) & (TF_SLOTS - 1);
+            }
+         }
+      }
+      __syncthreads();
+      // one range per distinct column of the chunk
+      for (int q = threadIdx.x; q < TF_SLOTS; q += 256) {
+         if (hkey[q] >= 0) { hval[q] = atomicAdd(&fillp[hkey[q]], hval[q]); }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < TF_E; t++) {
+         if (s + t < e) {
+            const int k = s + t;
+            const int pos = (slot[t] >= 0) ? hval[slot[t]] + li[t] : atomicAdd(&fillp[ja[k]], 1);
+            Rj[pos] = i;
+            Ra[pos] = a[k];
+         }
+      }
+      for (int k = s + TF_E; k < e; k++) {
+         const int pos = atomicAdd(&fillp[ja[k]], 1);
          Rj[pos] = i;
          Ra[pos] = a[k];
       }
+      __syncthreads();
    }
 }
 
@@ -3078,14 +3189,15 @@ void transpose(const Csr &P, Csr &R)
    R.a = dalloc_n<double>((size_t) std::max<long long>(P.nnz, 1));
    R.a32 = nullptr;
    dzero(R.ia, sizeof(int) * ((size_t) nc + 1));
-   int nbe = (int) std::max<long long>(1, std::min<long long>((P.nnz + 255) / 256, 8192));
-   if (P.nnz > 0) { k_col_count<<<nbe, 256, 0, stream()>>>(P.nnz, P.ja, R.ia); }
+   const long long nch = (P.nnz + 256LL * TC_U - 1) / (256LL * TC_U);
+   const int nbe = (int) std::max<long long>(1, std::min<long long>(nch, 8192));
+   if (P.nnz > 0) { k_col_count_h<<<nbe, 256, 0, stream()>>>(P.nnz, P.ja, R.ia); }
    HX_LAUNCH_CHECK();
    exclusive_scan(R.ia, R.ia, nc);
    int *fillp = dalloc_n<int>((size_t) nc + 1);
    d2d(fillp, R.ia, sizeof(int) * ((size_t) nc + 1));
    int nb = std::max(1, std::min((n + 255) / 256, 8192));
-   k_t_fill<<<nb, 256, 0, stream()>>>(n, P.ia, P.ja, P.a, fillp, R.ja, R.a);
+   k_t_fill_h<<<nb, 256, 0, stream()>>>(n, P.ia, P.ja, P.a, fillp, R.ja, R.a);
    k_t_sort<<<launch_blocks(nc), WAVE, 0, stream()>>>(nc, R.ia, R.ja, R.a);
    HX_LAUNCH_CHECK();
    dfree(fillp);
