@@ -906,6 +906,8 @@ __global__ void k_pmis_init(int n, const int *__restrict__ S_ia, const int *__re
    }
 }
 
+constexpr int PM_U = 4;   // strided steps of a row whose loads are issued together
+
 // par_coarsen.c:2506-2550: drop candidates with a larger-measure S-neighbour
 // (K lanes per row; every write is a 0, so their order does not matter)
 template <int K>
@@ -925,14 +927,22 @@ __global__ void k_pmis_remove(int nw, const int *__restrict__ list, const int *_
       const bool cand = mi > 1;
       const int s = (act && cand) ? S_ia[i] : 0, e = (act && cand) ? S_ia[i + 1] : 0;
       bool lose = false;
-      for (int kb = s; __any(kb < e); kb += K) {
-         const int k = kb + gl;
-         if (k < e) {
-            const int j = S_ja[k];
-            const double mj = m[j];
-            if (mj > 1) {
-               if (mi > mj) { CF[j] = 0; }
-               else if (mj > mi) { lose = true; }
+      // PM_U strided steps per batch: their columns, then their measures, are
+      // loaded before any is used (positions past the row end read its last
+      // entry and are masked)
+      const int klast = max(e - 1, 0);
+      for (int kb = s; __any(kb < e); kb += PM_U * K) {
+         int jv[PM_U];
+         double mv[PM_U];
+#pragma unroll
+         for (int u = 0; u < PM_U; u++) { jv[u] = S_ja[min(kb + gl + u * K, klast)]; }
+#pragma unroll
+         for (int u = 0; u < PM_U; u++) { mv[u] = m[jv[u]]; }
+#pragma unroll
+         for (int u = 0; u < PM_U; u++) {
+            if (kb + gl + u * K < e && mv[u] > 1) {
+               if (mi > mv[u]) { CF[jv[u]] = 0; }
+               else if (mv[u] > mi) { lose = true; }
             }
          }
       }
@@ -964,9 +974,17 @@ __global__ void k_pmis_cf(int nw, const int *__restrict__ list, const int *__res
       const bool scan = in_graph && !(c > 0);
       const int s = scan ? S_ia[i] : 0, e = scan ? S_ia[i + 1] : 0;
       bool hitc = false;
-      for (int kb = s; __any(kb < e); kb += K) {
-         const int k = kb + gl;
-         if (k < e && CF[S_ja[k]] > 0) { hitc = true; }
+      const int klast = max(e - 1, 0);
+      for (int kb = s; __any(kb < e); kb += PM_U * K) {
+         int jv[PM_U], cv[PM_U];
+#pragma unroll
+         for (int u = 0; u < PM_U; u++) { jv[u] = S_ja[min(kb + gl + u * K, klast)]; }
+#pragma unroll
+         for (int u = 0; u < PM_U; u++) { cv[u] = CF[jv[u]]; }
+#pragma unroll
+         for (int u = 0; u < PM_U; u++) {
+            if (kb + gl + u * K < e && cv[u] > 0) { hitc = true; }
+         }
       }
       const unsigned long long hm = (__ballot(hitc) >> gshift) & gmask;
       if (in_graph && gl == 0) {
