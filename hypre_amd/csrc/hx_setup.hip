@@ -2864,9 +2864,24 @@ __global__ void k_compact_rows(int n, const int *__restrict__ base, const int *_
    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
       int b = base ? base[i] : i * stride;
       int o = out_ia[i], c = cnt[i];
-      for (int k = 0; k < c; k++) {
-         oj[o + k] = in_j[b + k];
-         oa[o + k] = in_a[b + k];
+      // four entries' loads issued together
+      for (int k0 = 0; k0 < c; k0 += 4) {
+         int jv[4];
+         double av[4];
+#pragma unroll
+         for (int t = 0; t < 4; t++) {
+            if (k0 + t < c) {
+               jv[t] = in_j[b + k0 + t];
+               av[t] = in_a[b + k0 + t];
+            }
+         }
+#pragma unroll
+         for (int t = 0; t < 4; t++) {
+            if (k0 + t < c) {
+               oj[o + k0 + t] = jv[t];
+               oa[o + k0 + t] = av[t];
+            }
+         }
       }
    }
 }
@@ -3159,16 +3174,44 @@ __global__ __launch_bounds__(WAVE) void k_t_sort(int nc, const int *__restrict__
    __shared__ int sk[TS_MAX];
    __shared__ double sv[TS_MAX];
    const int lane = threadIdx.x;
+   // rows of up to 64 entries are held one entry per lane; the next row's
+   // bounds and entries are loaded while the current one is ranked
+   int b = 0, e = 0, key = 0;
+   double val = 0.0;
+   auto fetch = [&](int r) {
+      b = Ria[r];
+      e = Ria[r + 1];
+      if (e - b <= WAVE && lane < e - b) {
+         key = Rj[b + lane];
+         val = Ra[b + lane];
+      }
+   };
+   if (blockIdx.x < nc) { fetch(blockIdx.x); }
    for (int r = blockIdx.x; r < nc; r += gridDim.x) {
-      const int b = Ria[r], e = Ria[r + 1], L = e - b;
+      const int cb = b, ce = e, L = e - b;
+      const int ckey = key;
+      const double cval = val;
+      if (r + (int) gridDim.x < nc) { fetch(r + gridDim.x); }
       if (L <= 1) { continue; }
+      if (L <= WAVE) {
+         if (lane < L) { sk[lane] = ckey; }
+         wave_sync();
+         if (lane < L) {
+            int rank = 0;
+            for (int m = 0; m < L; m++) { rank += (sk[m] < ckey) ? 1 : 0; }
+            Rj[cb + rank] = ckey;
+            Ra[cb + rank] = cval;
+         }
+         wave_sync();
+         continue;
+      }
       if (L > TS_MAX) {
          if (lane == 0) {
-            for (int k = b + 1; k < e; k++) {
+            for (int k = cb + 1; k < ce; k++) {
                const int key = Rj[k];
                const double v = Ra[k];
                int m = k - 1;
-               while (m >= b && Rj[m] > key) {
+               while (m >= cb && Rj[m] > key) {
                   Rj[m + 1] = Rj[m];
                   Ra[m + 1] = Ra[m];
                   m--;
@@ -3180,16 +3223,16 @@ __global__ __launch_bounds__(WAVE) void k_t_sort(int nc, const int *__restrict__
          continue;
       }
       for (int k = lane; k < L; k += WAVE) {
-         sk[k] = Rj[b + k];
-         sv[k] = Ra[b + k];
+         sk[k] = Rj[cb + k];
+         sv[k] = Ra[cb + k];
       }
       wave_sync();
       for (int k = lane; k < L; k += WAVE) {
          const int key = sk[k];
          int rank = 0;
          for (int m = 0; m < L; m++) { rank += (sk[m] < key) ? 1 : 0; }
-         Rj[b + rank] = key;
-         Ra[b + rank] = sv[k];
+         Rj[cb + rank] = key;
+         Ra[cb + rank] = sv[k];
       }
       wave_sync();
    }
