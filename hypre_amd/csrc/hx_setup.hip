@@ -2771,6 +2771,7 @@ __device__ void qsort2_abs_strided(int *v, double *w, int cnt, int keep, int *st
 constexpr int TR_ROWS = 64;
 constexpr int TR_SS = 65;
 constexpr int TR_STACK = 8;    // >= log2(64) + 1: the smaller side is always refined first
+constexpr int TR_U = 8;        // staging loads in flight per thread
 
 template <int MC>
 __global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__restrict__ rowlist,
@@ -2792,11 +2793,30 @@ __global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__rest
       const long long i = base + t;
       sc[t] = (i < n) ? cnt_raw[i] : -1;
       __syncthreads();
-      for (int k = t; k < TR_ROWS * MC; k += TR_ROWS) {
-         const int r = k / MC, e = k % MC;
-         if (e < sc[r]) {
-            sj[e * TR_SS + r] = raw_j[(size_t) (base + r) * MC + e];
-            sw[e * TR_SS + r] = raw_a[(size_t) (base + r) * MC + e];
+      // TR_U loads per thread issued before their LDS stores
+      for (int k0 = t; k0 < TR_ROWS * MC; k0 += TR_ROWS * TR_U) {
+         int jv[TR_U];
+         double wv[TR_U];
+         bool ok[TR_U];
+#pragma unroll
+         for (int u = 0; u < TR_U; u++) {
+            const int k = k0 + u * TR_ROWS;
+            const bool in = k < TR_ROWS * MC;
+            const int r = in ? k / MC : 0, e = in ? k % MC : 0;
+            ok[u] = in && e < sc[r];
+            if (ok[u]) {
+               jv[u] = raw_j[(size_t) (base + r) * MC + e];
+               wv[u] = raw_a[(size_t) (base + r) * MC + e];
+            }
+         }
+#pragma unroll
+         for (int u = 0; u < TR_U; u++) {
+            if (ok[u]) {
+               const int k = k0 + u * TR_ROWS;
+               const int r = k / MC, e = k % MC;
+               sj[e * TR_SS + r] = jv[u];
+               sw[e * TR_SS + r] = wv[u];
+            }
          }
       }
       __syncthreads();
