@@ -307,10 +307,15 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
    cur[0] = cur0; spare[0] = spare0; F[0] = f;
    for (int l = 1; l < nl; l++) { cur[l] = L[l].u; spare[l] = L[l].ut; F[l] = L[l].f; }
    bool zero = u_zero;
+   bool formed = false;   // this level's first sweep was formed by the restriction
    for (int l = 0; l < nl - 1; l++) {
       const Level &lv = L[l];
       const Csr &A = *lv.A;
-      for (int s = 0; s < sweeps(l, 1); s++) {
+      if (formed) {
+         std::swap(cur[l], spare[l]);
+         zero = false;
+      }
+      for (int s = formed ? 1 : 0; s < sweeps(l, 1); s++) {
          // level-0 sweeps from a zero guess are timed as a class of their own
          const char *cls = (l == 0) ? (zero ? "relaxz_A0" : "relax_A0") : "relax";
          relax_any(prm.relax_type[1], l, A, F[l], cur[l], lv.l1, spare[l], zero, cls);
@@ -319,7 +324,15 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
       }
       // residual and restriction (par_cycle.c:755-790)
       spmv(A, -1.0, cur[l], 1.0, F[l], lv.r, l == 0 ? "resid_A0" : "spmv");
-      spmv(lv.R, 1.0, lv.r, 0.0, nullptr, L[l + 1].f, "restrict");
+      // a next level that opens with an l1-Jacobi sweep from a zero guess gets
+      // that sweep, u = (0 + w f)/l1, from the restriction's epilogue
+      formed = (l + 1 < nl - 1) && prm.relax_type[1] == 18 && sweeps(l + 1, 1) >= 1;
+      if (formed) {
+         spmv_zero_relax(lv.R, lv.r, L[l + 1].f, L[l + 1].l1, omega(l + 1), spare[l + 1],
+                         "restrict");
+      } else {
+         spmv(lv.R, 1.0, lv.r, 0.0, nullptr, L[l + 1].f, "restrict");
+      }
       zero = true;
    }
    {
@@ -361,16 +374,27 @@ void AMGSolver::cycle32(const double *f, double *u)
    for (int l = 0; l < nl; l++) { cur[l] = L[l].u32; spare[l] = L[l].ut32; }
    d2f(L[0].A->nrows, f, L[0].f32);
    bool zero = true;
+   bool formed = false;
    for (int l = 0; l < nl - 1; l++) {
       Level &lv = L[l];
-      for (int s = 0; s < sweeps(l, 1); s++) {
+      if (formed) {
+         std::swap(cur[l], spare[l]);
+         zero = false;
+      }
+      for (int s = formed ? 1 : 0; s < sweeps(l, 1); s++) {
          const char *cls = (l == 0) ? (zero ? "relaxz_A0" : "relax_A0") : "relax";
          relax_any32(prm.relax_type[1], l, lv.A32, lv.f32, cur[l], lv.l1f, spare[l], zero, cls);
          std::swap(cur[l], spare[l]);
          zero = false;
       }
       spmv32(lv.A32, -1.f, cur[l], 1.f, lv.f32, lv.r32, l == 0 ? "resid_A0" : "spmv");
-      spmv32(lv.R32, 1.f, lv.r32, 0.f, nullptr, L[l + 1].f32, "restrict");
+      formed = (l + 1 < nl - 1) && prm.relax_type[1] == 18 && sweeps(l + 1, 1) >= 1;
+      if (formed) {
+         spmv32_zero_relax(lv.R32, lv.r32, L[l + 1].f32, L[l + 1].l1f, (float) omega(l + 1),
+                           spare[l + 1], "restrict");
+      } else {
+         spmv32(lv.R32, 1.f, lv.r32, 0.f, nullptr, L[l + 1].f32, "restrict");
+      }
       zero = true;
    }
    {

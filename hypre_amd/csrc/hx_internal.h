@@ -118,6 +118,11 @@ void spmv(const Csr &A, double alpha, const double *x, double beta, const double
           const char *cls = "spmv");
 void spmv32(const Csr &A, float alpha, const float *x, float beta, const float *b, float *y,
             const char *cls = "spmv");
+// y = A x and u = (0 + w y)./l1 (the zero-guess l1-Jacobi sweep on y) in one pass
+void spmv_zero_relax(const Csr &A, const double *x, double *y, const double *l1, double w,
+                     double *u, const char *cls);
+void spmv32_zero_relax(const Csr &A, const float *x, float *y, const float *l1, float w, float *u,
+                       const char *cls);
 // s = A p and part[] = per-block partial sums of s.*p (deterministic)
 void spmv_dot(const Csr &A, const double *p, double *s, double *dot_out_dev, const char *cls);
 // l1-Jacobi: u_out = u + (w*f - w*A*u)./l1 ; zero_guess: u_out = (w*f)./l1

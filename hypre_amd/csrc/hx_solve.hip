@@ -246,11 +246,15 @@ __device__ __forceinline__ void wave_rows(int n, int blk, int wv, int nb, int &R
    R1 = (int) (r0 + per < n ? r0 + per : n);
 }
 
-template <int RW, int SW, typename V, typename X, bool DOT>
+// ZR: also u = (0 + zw y) ./ zl1, the zero-guess l1-Jacobi sweep on y, from
+// the value just formed (the bits of a separate sweep reading y back)
+template <int RW, int SW, typename V, typename X, bool DOT, bool ZR = false>
 __global__ __launch_bounds__(BS) void k_spmv_w(int n, const int *__restrict__ ia,
                                                const int *__restrict__ ja, const V *__restrict__ a,
                                                const X *__restrict__ x, X alpha, X beta,
-                                               const X *b, X *y, double *__restrict__ part)
+                                               const X *b, X *y, double *__restrict__ part,
+                                               const X *__restrict__ zl1 = nullptr, X zw = (X) 0,
+                                               X *__restrict__ zu = nullptr)
 {
    __shared__ X sbuf_all[WPB * SW];
    const int nb = gridDim.x;
@@ -264,6 +268,7 @@ __global__ __launch_bounds__(BS) void k_spmv_w(int n, const int *__restrict__ ia
       const X v = (beta == (X) 0) ? alpha * t : alpha * t + beta * b[row];
       y[row] = v;
       if (DOT) { acc += (double) v * (double) x[row]; }
+      if (ZR) { zu[row] = (X) 0 + (zw * v) / zl1[row]; }
    });
    if (DOT) {
       __shared__ double sh[BS];
@@ -323,9 +328,10 @@ static void dispatch_rw(int rw, F f)
    }
 }
 
-template <typename V, typename X, bool DOT>
+template <typename V, typename X, bool DOT, bool ZR = false>
 static int launch_spmv(const Csr &A, const V *vals, X alpha, const X *x, X beta, const X *b,
-                       X *y, double *part, const char *cls, double vbytes)
+                       X *y, double *part, const char *cls, double vbytes,
+                       const X *zl1 = nullptr, X zw = (X) 0, X *zu = nullptr)
 {
    if (A.nrows <= 0) { return 0; }
    const int rw = pick_rw(A.nnz, A.nrows);
@@ -337,14 +343,14 @@ static int launch_spmv(const Csr &A, const V *vals, X alpha, const X *x, X beta,
    const int sw = spmv_window(A.nnz, A.nrows);
    dispatch_rw(rw, [&](auto c) {
       if (sw == 1024) {
-         k_spmv_w<decltype(c)::value, 1024, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals,
-                                                                        x, alpha, beta, b, y, part);
+         k_spmv_w<decltype(c)::value, 1024, V, X, DOT, ZR><<<nb, BS, 0, s>>>(
+            A.nrows, A.ia, A.ja, vals, x, alpha, beta, b, y, part, zl1, zw, zu);
       } else if (sw == 256) {
-         k_spmv_w<decltype(c)::value, 256, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals,
-                                                                       x, alpha, beta, b, y, part);
+         k_spmv_w<decltype(c)::value, 256, V, X, DOT, ZR><<<nb, BS, 0, s>>>(
+            A.nrows, A.ia, A.ja, vals, x, alpha, beta, b, y, part, zl1, zw, zu);
       } else {
-         k_spmv_w<decltype(c)::value, SW, V, X, DOT><<<nb, BS, 0, s>>>(A.nrows, A.ia, A.ja, vals, x,
-                                                                      alpha, beta, b, y, part);
+         k_spmv_w<decltype(c)::value, SW, V, X, DOT, ZR><<<nb, BS, 0, s>>>(
+            A.nrows, A.ia, A.ja, vals, x, alpha, beta, b, y, part, zl1, zw, zu);
       }
    });
    HX_LAUNCH_CHECK();
@@ -362,6 +368,21 @@ void spmv32(const Csr &A, float alpha, const float *x, float beta, const float *
 {
    if (!A.a32) { fail(HYPRE_ERROR_GENERIC, "FP32 values missing"); }
    launch_spmv<float, float, false>(A, A.a32, alpha, x, beta, b, y, nullptr, cls, 4.0);
+}
+
+void spmv_zero_relax(const Csr &A, const double *x, double *y, const double *l1, double w,
+                     double *u, const char *cls)
+{
+   launch_spmv<double, double, false, true>(A, A.a, 1.0, x, 0.0, nullptr, y, nullptr, cls, 8.0, l1,
+                                            w, u);
+}
+
+void spmv32_zero_relax(const Csr &A, const float *x, float *y, const float *l1, float w, float *u,
+                       const char *cls)
+{
+   if (!A.a32) { fail(HYPRE_ERROR_GENERIC, "FP32 values missing"); }
+   launch_spmv<float, float, false, true>(A, A.a32, 1.f, x, 0.f, nullptr, y, nullptr, cls, 4.0, l1,
+                                          w, u);
 }
 
 void spmv_dot(const Csr &A, const double *p, double *s, double *dot_out_dev, const char *cls)
