@@ -367,6 +367,7 @@ class BoomerAMG:
         "cycle_type": "HYPRE_BoomerAMGSetCycleType",
         "fp32_cycle": "HYPRE_MI355X_BoomerAMGSetCyclePrecision",
         "keep_setup_data": "HYPRE_MI355X_BoomerAMGSetKeepSetupData",
+        "solve_format": "HYPRE_MI355X_BoomerAMGSetSolveFormat",
     }
 
     def set(self, **params):
@@ -398,7 +399,7 @@ class BoomerAMG:
 
     def num_levels(self) -> int:
         v = C.c_int()
-        call("HYPRE_BoomerAMGGetNumLevels", self.h, C.byref(v))
+        call("HYPRE_MI355X_BoomerAMGGetNumLevels", self.h, C.byref(v))
         return v.value
 
     def num_iterations(self) -> int:
@@ -430,20 +431,26 @@ class BoomerAMG:
              _ip(ja), None if a is None else _dp(a))
         return HostCsr(n.value, nc.value, ia, ja[: nnz.value], None if a is None else a[: nnz.value])
 
+    def level_rows(self, lev: int) -> int:
+        n, nc, nnz = C.c_int(), C.c_int(), C.c_int()
+        call("HYPRE_MI355X_BoomerAMGGetLevelMatrix", self.h, lev, C.c_char(b"A"), C.byref(n),
+             C.byref(nc), C.byref(nnz), None, None, None)
+        return n.value
+
     def level_cf(self, lev: int) -> np.ndarray:
-        n = self.level_matrix(lev, "A").nrows
+        n = self.level_rows(lev)
         out = np.empty(n, np.int32)
         call("HYPRE_MI355X_BoomerAMGGetLevelCF", self.h, lev, _ip(out))
         return out
 
     def level_l1(self, lev: int) -> np.ndarray:
-        n = self.level_matrix(lev, "A").nrows
+        n = self.level_rows(lev)
         out = np.empty(n, np.float64)
         call("HYPRE_MI355X_BoomerAMGGetLevelL1", self.h, lev, _dp(out))
         return out
 
     def level_measure(self, lev: int) -> np.ndarray:
-        n = self.level_matrix(lev, "A").nrows
+        n = self.level_rows(lev)
         out = np.empty(n, np.float64)
         call("HYPRE_MI355X_BoomerAMGGetLevelMeasure", self.h, lev, _dp(out))
         return out

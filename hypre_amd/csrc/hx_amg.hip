@@ -46,6 +46,8 @@ void AMGSolver::free_hierarchy()
       l.R.free_all();
       l.S.free_all();
       l.Arev.free_all();
+      l.ell.free_all();
+      l.Av = Csr();
       dfree(l.A32.a32); dfree(l.P32.a32); dfree(l.R32.a32);
       dfree(l.CF); dfree(l.measure); dfree(l.l1); dfree(l.l1f);
       dfree(l.f); dfree(l.u); dfree(l.ut); dfree(l.r);
@@ -242,6 +244,17 @@ void AMGSolver::setup(const Csr &A0)
          }
       }
    }
+   // solve-phase copies of the level operators: sliced ELL where the slices'
+   // padding stays within 25% of the entries (levels of at least 4096 rows)
+   for (int j = 0; j < num_levels; j++) {
+      Level &lv = L[j];
+      lv.Av = *lv.A;
+      lv.Av.ell = nullptr;
+      if (p.solve_format == 1 && ell_build(*lv.A, lv.ell, 0.25, 4096, p.cycle_fp32 != 0)) {
+         lv.Av.ell = &lv.ell;
+         if (p.cycle_fp32) { lv.A32.ell = &lv.ell; }
+      }
+   }
    sync();
    setup_done = true;
    setup_seconds = wall_seconds() - t0;
@@ -310,7 +323,7 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
    bool formed = false;   // this level's first sweep was formed by the restriction
    for (int l = 0; l < nl - 1; l++) {
       const Level &lv = L[l];
-      const Csr &A = *lv.A;
+      const Csr &A = lv.Av;
       if (formed) {
          std::swap(cur[l], spare[l]);
          zero = false;
@@ -326,7 +339,8 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
       spmv(A, -1.0, cur[l], 1.0, F[l], lv.r, l == 0 ? "resid_A0" : "spmv");
       // a next level that opens with an l1-Jacobi sweep from a zero guess gets
       // that sweep, u = (0 + w f)/l1, from the restriction's epilogue
-      formed = (l + 1 < nl - 1) && prm.relax_type[1] == 18 && sweeps(l + 1, 1) >= 1;
+      formed = prm.solve_format == 1 && (l + 1 < nl - 1) && prm.relax_type[1] == 18 &&
+               sweeps(l + 1, 1) >= 1;
       if (formed) {
          spmv_zero_relax(lv.R, lv.r, L[l + 1].f, L[l + 1].l1, omega(l + 1), spare[l + 1],
                          "restrict");
@@ -342,7 +356,7 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
          gselim_solve(coarse_n, Amat, F[l], cur[l]);
       } else {
          for (int s = 0; s < coarse_sweeps; s++) {
-            relax_any(coarse_relax, l, *lv.A, F[l], cur[l], lv.l1, spare[l], zero, "relax");
+            relax_any(coarse_relax, l, lv.Av, F[l], cur[l], lv.l1, spare[l], zero, "relax");
             std::swap(cur[l], spare[l]);
             zero = false;
          }
@@ -354,7 +368,7 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
       spmv(lv.P, 1.0, cur[l + 1], 1.0, cur[l], cur[l], "interp");
       const char *cls = (l == 0) ? "relax_A0" : "relax";
       for (int s = 0; s < sweeps(l, 2); s++) {
-         relax_any(prm.relax_type[2], l, *lv.A, F[l], cur[l], lv.l1, spare[l], false, cls);
+         relax_any(prm.relax_type[2], l, lv.Av, F[l], cur[l], lv.l1, spare[l], false, cls);
          std::swap(cur[l], spare[l]);
       }
    }
@@ -388,7 +402,8 @@ void AMGSolver::cycle32(const double *f, double *u)
          zero = false;
       }
       spmv32(lv.A32, -1.f, cur[l], 1.f, lv.f32, lv.r32, l == 0 ? "resid_A0" : "spmv");
-      formed = (l + 1 < nl - 1) && prm.relax_type[1] == 18 && sweeps(l + 1, 1) >= 1;
+      formed = prm.solve_format == 1 && (l + 1 < nl - 1) && prm.relax_type[1] == 18 &&
+               sweeps(l + 1, 1) >= 1;
       if (formed) {
          spmv32_zero_relax(lv.R32, lv.r32, L[l + 1].f32, L[l + 1].l1f, (float) omega(l + 1),
                            spare[l + 1], "restrict");
@@ -424,8 +439,15 @@ void AMGSolver::cycle32(const double *f, double *u)
 }
 
 // src/parcsr_ls/par_amg_solve.c:22-424
-int AMGSolver::solve(const Csr &A, hypre_ParVector_struct *b, hypre_ParVector_struct *x)
+const Csr &AMGSolver::solve_view(const Csr &A) const
 {
+   if (setup_done && !L.empty() && L[0].A == &A && L[0].Av.ell) { return L[0].Av; }
+   return A;
+}
+
+int AMGSolver::solve(const Csr &A_in, hypre_ParVector_struct *b, hypre_ParVector_struct *x)
+{
+   const Csr &A = solve_view(A_in);
    const int n = A.nrows;
    const double tol = prm.tol;
    const int max_iter = prm.max_iter;
@@ -565,7 +587,17 @@ HYPRE_Int HYPRE_BoomerAMGSolve(HYPRE_Solver solver, HYPRE_ParCSRMatrix A, HYPRE_
       if (!b) { g_error_arg = 3; fail(HYPRE_ERROR_ARG, "NULL rhs"); }
       if (!x) { g_error_arg = 4; fail(HYPRE_ERROR_ARG, "NULL solution"); }
       if (!s->setup_done) { fail(HYPRE_ERROR_GENERIC, "BoomerAMGSolve before setup"); }
-      s->solve(mat_of(A), b, x);
+      // the reference reads past short vectors here; on the device that is a
+      // memory fault, so lengths that do not fit A and the set-up hierarchy
+      // are refused
+      const Csr &M = mat_of(A);
+      if (M.nrows != s->L[0].A->nrows || M.ncols != M.nrows) {
+         g_error_arg = 2;
+         fail(HYPRE_ERROR_ARG, "matrix size differs from the set-up hierarchy");
+      }
+      if (b->n != M.nrows) { g_error_arg = 3; fail(HYPRE_ERROR_ARG, "rhs length differs from the matrix"); }
+      if (x->n != M.nrows) { g_error_arg = 4; fail(HYPRE_ERROR_ARG, "solution length differs from the matrix"); }
+      s->solve(M, b, x);
    });
 }
 
@@ -768,7 +800,7 @@ HYPRE_Int HYPRE_BoomerAMGGetFinalRelativeResidualNorm(HYPRE_Solver solver, HYPRE
    return guard([&] { *v = as_amg(solver)->rel_resid; });
 }
 
-HYPRE_Int HYPRE_BoomerAMGGetNumLevels(HYPRE_Solver solver, HYPRE_Int *v)
+HYPRE_Int HYPRE_MI355X_BoomerAMGGetNumLevels(HYPRE_Solver solver, HYPRE_Int *v)
 {
    return guard([&] { *v = as_amg(solver)->num_levels; });
 }
@@ -823,6 +855,19 @@ static void apply_precond(PCGSolver *s, HYPRE_ParCSRMatrix A, hypre_ParVector_st
    out->all_zeros = false;
 }
 
+// The operator PCG multiplies with: when the preconditioner is a BoomerAMG
+// hierarchy set up on this same matrix, its solve-phase copy (sliced ELL, the
+// same products in the same order), else the matrix itself.  Like hypre's
+// device SpMV analysis data, the copy belongs to the setup: a matrix whose
+// values are changed in place after setup needs a new setup.
+static const Csr &pcg_operator(const PCGSolver *s, HYPRE_ParCSRMatrix A)
+{
+   if (s->precond_solver && s->precond_solver->kind == 1) {
+      return static_cast<const AMGSolver *>(s->precond_solver)->solve_view(A->A);
+   }
+   return A->A;
+}
+
 // src/krylov/pcg.c:313-1013 (hypre_PCGSolve)
 HYPRE_Int HYPRE_ParCSRPCGSolve(HYPRE_Solver solver, HYPRE_ParCSRMatrix A, HYPRE_ParVector b,
                                HYPRE_ParVector x)
@@ -832,8 +877,11 @@ HYPRE_Int HYPRE_ParCSRPCGSolve(HYPRE_Solver solver, HYPRE_ParCSRMatrix A, HYPRE_
       if (!A) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "NULL matrix"); }
       if (!b) { g_error_arg = 3; fail(HYPRE_ERROR_ARG, "NULL rhs"); }
       if (!x) { g_error_arg = 4; fail(HYPRE_ERROR_ARG, "NULL solution"); }
-      const Csr &M = A->A;
+      const Csr &M = pcg_operator(s, A);
       const int n = M.nrows;
+      if (M.ncols != n) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "PCG needs a square matrix"); }
+      if (b->n != n) { g_error_arg = 3; fail(HYPRE_ERROR_ARG, "rhs length differs from the matrix"); }
+      if (x->n != n) { g_error_arg = 4; fail(HYPRE_ERROR_ARG, "solution length differs from the matrix"); }
       s->ensure_vectors(n);
       double t0 = wall_seconds();
       hypre_ParVector_struct *p = s->p, *sv = s->s, *r = s->r;
@@ -1137,6 +1185,14 @@ HYPRE_Int HYPRE_MI355X_BoomerAMGSetLevelStrongThreshold(HYPRE_Solver solver, HYP
                                                         HYPRE_Real theta)
 {
    return guard([&] { set_level_vec(as_amg(solver)->prm.level_theta, level, theta); });
+}
+
+HYPRE_Int HYPRE_MI355X_BoomerAMGSetSolveFormat(HYPRE_Solver solver, HYPRE_Int format)
+{
+   return guard([&] {
+      if (format != 0 && format != 1) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "format must be 0 or 1"); }
+      as_amg(solver)->prm.solve_format = format;
+   });
 }
 
 HYPRE_Int HYPRE_MI355X_BoomerAMGSetHybridPartitions(HYPRE_Solver solver, HYPRE_Int parts,

@@ -995,21 +995,43 @@ HYPRE_Int HYPRE_ParCSRMatrixRestoreRow(HYPRE_ParCSRMatrix M, HYPRE_BigInt row, H
    return g_error_flag;
 }
 
+// Size checks of src/parcsr_mv/par_csr_matvec.c:69-82 (y = A x) and
+// :596-609 (y = A^T x; the reference compares x with A's columns there too):
+// 11 when x's length is off, 12 when y's is, 13 when both are, returned as the
+// call's value without touching the global error flag.  The reference still
+// multiplies (the code is informational); here a product whose operands do not
+// fit is not launched, since it would read and write past the vectors.
+static int matvec_size_code(const Csr &A, int xn, int yn)
+{
+   const bool bx = A.ncols != xn, by = A.nrows != yn;
+   return (bx && by) ? 13 : bx ? 11 : by ? 12 : 0;
+}
+
 HYPRE_Int HYPRE_ParCSRMatrixMatvec(HYPRE_Complex alpha, HYPRE_ParCSRMatrix A, HYPRE_ParVector x,
                                    HYPRE_Complex beta, HYPRE_ParVector y)
 {
-   return guard([&] {
+   int code = 0;
+   const int flag = guard([&] {
       if (!A || !x || !y) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "NULL argument"); }
+      code = matvec_size_code(A->A, x->n, y->n);
+      if (code) { return; }
       spmv(A->A, alpha, x->d, beta, y->d, y->d, "spmv");
       y->all_zeros = false;
    });
+   return code ? code : flag;
 }
 
 HYPRE_Int HYPRE_ParCSRMatrixMatvecT(HYPRE_Complex alpha, HYPRE_ParCSRMatrix A, HYPRE_ParVector x,
                                     HYPRE_Complex beta, HYPRE_ParVector y)
 {
-   return guard([&] {
+   int code = 0;
+   const int flag = guard([&] {
       if (!A || !x || !y) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "NULL argument"); }
+      code = matvec_size_code(A->A, x->n, y->n);
+      if (x->n != A->A.nrows || y->n != A->A.ncols) {
+         if (!code) { code = 13; }
+         return;
+      }
       // explicit transpose (a stored transpose is what the V-cycle uses)
       Csr At;
       Csr tmpP = A->A;
@@ -1018,6 +1040,7 @@ HYPRE_Int HYPRE_ParCSRMatrixMatvecT(HYPRE_Complex alpha, HYPRE_ParCSRMatrix A, H
       At.free_all();
       y->all_zeros = false;
    });
+   return code ? code : flag;
 }
 
 // ================================================================== ParVector

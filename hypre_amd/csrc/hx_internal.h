@@ -97,6 +97,21 @@ struct TimedLaunch
 };
 
 // ------------------------------------------------------------------ CSR
+// Sliced-ELL copy of an operator for the solve phase: slices of 64
+// consecutive rows; a slice of width w (its longest row) stores 64*w entries
+// column-major (entry k of its 64 rows are 64 consecutive words), each row's
+// entries in CSR storage order, padding masked by the CSR row length.
+struct Ell
+{
+   int       nslices = 0;
+   long long stored = 0;          // 64 * sum of slice widths
+   int      *sw = nullptr;        // exclusive prefix of slice widths (nslices + 1)
+   int      *j = nullptr;
+   double   *a = nullptr;
+   float    *a32 = nullptr;
+   void      free_all();
+};
+
 struct Csr
 {
    int       nrows = 0, ncols = 0;
@@ -104,9 +119,16 @@ struct Csr
    int      *ia = nullptr, *ja = nullptr;
    double   *a = nullptr;
    float    *a32 = nullptr;
+   // not owned: when set, SpMV and l1-Jacobi run on this copy (same results)
+   const Ell *ell = nullptr;
    void      free_all();
    void      alloc(int n, int nc, long long nz, bool values = true);
 };
+
+// Builds E from A when the padding stays within max_pad (stored entries <=
+// (1 + max_pad) nnz) and A has at least min_rows rows; returns whether it did.
+// fp32: also the FP32 values.
+bool ell_build(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32);
 
 void csr_copy(const Csr &src, Csr &dst);
 void csr_to_host(const Csr &A, std::vector<int> &ia, std::vector<int> &ja, std::vector<double> &a);

@@ -48,6 +48,8 @@ struct Level
    Csr     A32, P32, R32;       // FP32 value copies share index arrays (a32 filled)
    Csr     Arev;                // rows of A reversed, for backward hybrid Gauss-Seidel
    int     max_row = -1;        // longest row of A (hybrid Gauss-Seidel chunking)
+   Ell     ell;                 // sliced-ELL copy of A for the solve phase (when built)
+   Csr     Av;                  // the operator as the solve phase sees it: *A plus ell
 };
 
 struct AMGParams
@@ -79,6 +81,7 @@ struct AMGParams
    int    keep_setup_data = 0;
    int    hgs_parts = 0;         // hybrid GS partitions on every level (the reference's thread count)
    int    hgs_rows = 4;          // else partitions of about this many rows
+   int    solve_format = 1;      // 1: sliced-ELL copies of the level operators where padding allows; 0: CSR
    std::vector<double> level_theta, level_omega;
    std::vector<int>    level_sweeps;
 };
@@ -110,6 +113,8 @@ struct AMGSolver : public hypre_Solver_struct
                     const double *l1, double *uo, bool zero, const char *cls) const;
    void   relax_any32(int type, int lev, const Csr &A, const float *f, const float *u,
                       const float *l1, float *uo, bool zero, const char *cls) const;
+   // the solve-phase view of A when A is this hierarchy's fine operator, else A
+   const Csr &solve_view(const Csr &A) const;
 };
 
 struct PCGSolver : public hypre_Solver_struct

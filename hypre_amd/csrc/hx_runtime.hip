@@ -420,6 +420,14 @@ void Csr::free_all()
    dfree(ia); dfree(ja); dfree(a); dfree(a32);
    ia = ja = nullptr; a = nullptr; a32 = nullptr;
    nrows = ncols = 0; nnz = 0;
+   ell = nullptr;
+}
+
+void Ell::free_all()
+{
+   dfree(sw); dfree(j); dfree(a); dfree(a32);
+   sw = j = nullptr; a = nullptr; a32 = nullptr;
+   nslices = 0; stored = 0;
 }
 
 void Csr::alloc(int n, int nc, long long nz, bool values)

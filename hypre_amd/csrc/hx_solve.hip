@@ -314,6 +314,199 @@ static int grid_for_groups(int n, int rw)
    return (int) blocks;
 }
 
+// ---------------------------------------------------------------- sliced ELL
+// A wave owns a contiguous range of 64-row slices and takes them one at a
+// time, lane = row.  Step k of a slice loads entry k of its 64 rows: one
+// contiguous 256-byte index load and one 512-byte value load per wave
+// (non-temporal: the matrix is read once, x stays in L2), then the 64 x
+// gathers, which for a stencil operator are neighbours of 64 consecutive rows.
+// U steps are issued together (index and value loads, then the gathers, then
+// the adds), so each round trip to memory carries U steps.  Each lane adds its
+// row's products in storage order, t = ((0 + a_0 x_0) + a_1 x_1) + ..., the
+// order of hypre's host CSR matvec (src/seq_mv/csr_matvec.c:22) and of the
+// windowed CSR kernels above, so both formats give the same bits.
+constexpr int ELL_U = 9;
+
+template <int U, typename V, typename X, class Epi>
+__device__ __forceinline__ void ell_slice(int sl, int lane, int n, const int *__restrict__ ia,
+                                          const int *__restrict__ sw, const int *__restrict__ ej,
+                                          const V *__restrict__ ea, const X *__restrict__ x, Epi epi)
+{
+   const int r = sl * 64 + lane;
+   const int w0 = sw[sl], w = sw[sl + 1] - w0;
+   const size_t base = (size_t) w0 * 64 + lane;
+   int len = 0;
+   if (r < n) { len = ia[r + 1] - ia[r]; }
+   X t = 0;
+   for (int k0 = 0; k0 < w; k0 += U) {
+      int jj[U];
+      V vv[U];
+      X xx[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+         // steps past the slice width re-read its last step (in bounds, masked)
+         const size_t q = base + (size_t) min(k0 + u, w - 1) * 64;
+         jj[u] = __builtin_nontemporal_load(ej + q);
+         vv[u] = __builtin_nontemporal_load(ea + q);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) { xx[u] = x[(k0 + u < len) ? jj[u] : 0]; }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+         if (k0 + u < len) { t += (X) vv[u] * xx[u]; }
+      }
+   }
+   if (r < n) { epi(r, t); }
+}
+
+// slices of wave wv of block blk: a contiguous range, ranges in block order
+__device__ __forceinline__ void wave_slices(int ns, int blk, int wv, int nb, int &S0, int &S1)
+{
+   const long long nw = (long long) nb * WPB;
+   const long long per = (ns + nw - 1) / nw;
+   const long long s0 = ((long long) blk * WPB + wv) * per;
+   S0 = (int) (s0 < ns ? s0 : ns);
+   S1 = (int) (s0 + per < ns ? s0 + per : ns);
+}
+
+template <typename V, typename X, bool DOT, bool ZR = false>
+__global__ __launch_bounds__(BS) void k_spmv_ell(int n, int ns, const int *__restrict__ ia,
+                                                 const int *__restrict__ sw,
+                                                 const int *__restrict__ ej,
+                                                 const V *__restrict__ ea, const X *__restrict__ x,
+                                                 X alpha, X beta, const X *b, X *y,
+                                                 double *__restrict__ part,
+                                                 const X *__restrict__ zl1 = nullptr,
+                                                 X zw = (X) 0, X *__restrict__ zu = nullptr)
+{
+   const int nb = gridDim.x;
+   const int blk = xcd_block(blockIdx.x, nb);
+   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   int S0, S1;
+   wave_slices(ns, blk, wv, nb, S0, S1);
+   double acc = 0.0;
+   for (int sl = S0; sl < S1; sl++) {
+      ell_slice<ELL_U, V, X>(sl, lane, n, ia, sw, ej, ea, x, [&](int row, X t) {
+         const X v = (beta == (X) 0) ? alpha * t : alpha * t + beta * b[row];
+         y[row] = v;
+         if (DOT) { acc += (double) v * (double) x[row]; }
+         if (ZR) { zu[row] = (X) 0 + (zw * v) / zl1[row]; }
+      });
+   }
+   if (DOT) {
+      __shared__ double sh[BS];
+      sh[threadIdx.x] = acc;
+      __syncthreads();
+      for (int off = BS / 2; off > 0; off >>= 1) {
+         if (threadIdx.x < off) { sh[threadIdx.x] += sh[threadIdx.x + off]; }
+         __syncthreads();
+      }
+      if (threadIdx.x == 0) { part[blk] = sh[0]; }
+   }
+}
+
+template <typename V, typename X>
+__global__ __launch_bounds__(BS) void k_relax_l1_ell(int n, int ns, const int *__restrict__ ia,
+                                                     const int *__restrict__ sw,
+                                                     const int *__restrict__ ej,
+                                                     const V *__restrict__ ea,
+                                                     const X *__restrict__ f,
+                                                     const X *__restrict__ u,
+                                                     const X *__restrict__ l1, X w,
+                                                     X *__restrict__ uo)
+{
+   const int nb = gridDim.x;
+   const int blk = xcd_block(blockIdx.x, nb);
+   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   int S0, S1;
+   wave_slices(ns, blk, wv, nb, S0, S1);
+   for (int sl = S0; sl < S1; sl++) {
+      ell_slice<ELL_U, V, X>(sl, lane, n, ia, sw, ej, ea, u, [&](int row, X t) {
+         const X v = w * f[row] - w * t;
+         uo[row] = u[row] + v / l1[row];
+      });
+   }
+}
+
+template <typename V>
+static const V *ell_values(const Ell &E)
+{
+   if constexpr (sizeof(V) == 8) {
+      return E.a;
+   } else {
+      if (!E.a32) { fail(HYPRE_ERROR_GENERIC, "FP32 sliced-ELL values missing"); }
+      return E.a32;
+   }
+}
+
+// slice widths: the longest row of each 64-row slice
+__global__ void k_ell_width(int n, int ns, const int *__restrict__ ia, int *__restrict__ wout)
+{
+   const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+   if (sl >= ns) { return; }
+   const int r0 = sl * 64, r1 = min(n, r0 + 64);
+   int w = 0;
+   for (int r = r0; r < r1; r++) { w = max(w, ia[r + 1] - ia[r]); }
+   wout[sl] = w;
+}
+
+// one wave per slice, lane = row: entry k of row r goes to sw[s]*64 + k*64 +
+// lane; padding holds the row's own index and a zero value (never used)
+__global__ __launch_bounds__(BS) void k_ell_fill(int n, int ns, const int *__restrict__ ia,
+                                                 const int *__restrict__ ja,
+                                                 const double *__restrict__ a,
+                                                 const int *__restrict__ sw, int *__restrict__ ej,
+                                                 double *__restrict__ ea)
+{
+   const int sl = (int) ((blockIdx.x * (size_t) BS + threadIdx.x) >> 6);
+   const int lane = threadIdx.x & 63;
+   if (sl >= ns) { return; }
+   const int r = sl * 64 + lane;
+   const int w0 = sw[sl], w = sw[sl + 1] - w0;
+   const size_t base = (size_t) w0 * 64 + lane;
+   int rs = 0, len = 0;
+   if (r < n) {
+      rs = ia[r];
+      len = ia[r + 1] - rs;
+   }
+   const int pad_col = (r < n) ? r : 0;
+   for (int k = 0; k < w; k++) {
+      const bool in = k < len;
+      ej[base + (size_t) k * 64] = in ? ja[rs + k] : pad_col;
+      ea[base + (size_t) k * 64] = in ? a[rs + k] : 0.0;
+   }
+}
+
+bool ell_build(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32)
+{
+   E.free_all();
+   if (A.nrows < std::max(min_rows, 1) || A.nnz <= 0 || A.ncols <= 0) { return false; }
+   const int ns = (A.nrows + 63) / 64;
+   int *w = dalloc_n<int>((size_t) ns + 1);
+   k_ell_width<<<(ns + BS - 1) / BS, BS, 0, stream()>>>(A.nrows, ns, A.ia, w);
+   HX_LAUNCH_CHECK();
+   const long long tot = exclusive_scan(w, w, ns);
+   const long long stored = 64LL * tot;
+   if ((double) stored > (1.0 + max_pad) * (double) A.nnz || stored > 0x7fffffffLL) {
+      dfree(w);
+      return false;
+   }
+   E.nslices = ns;
+   E.stored = stored;
+   E.sw = w;
+   E.j = dalloc_n<int>((size_t) stored);
+   E.a = dalloc_n<double>((size_t) stored);
+   const long long threads = 64LL * ns;
+   k_ell_fill<<<(unsigned) ((threads + BS - 1) / BS), BS, 0, stream()>>>(A.nrows, ns, A.ia, A.ja, A.a,
+                                                                         E.sw, E.j, E.a);
+   HX_LAUNCH_CHECK();
+   if (fp32) {
+      E.a32 = dalloc_n<float>((size_t) stored);
+      d2f((int) stored, E.a, E.a32);
+   }
+   return true;
+}
+
 template <class F>
 static void dispatch_rw(int rw, F f)
 {
@@ -334,10 +527,22 @@ static int launch_spmv(const Csr &A, const V *vals, X alpha, const X *x, X beta,
                        const X *zl1 = nullptr, X zw = (X) 0, X *zu = nullptr)
 {
    if (A.nrows <= 0) { return 0; }
+   // ZR also reads l1 and writes u on the rows of y
+   double bytes = (double) A.nnz * (4.0 + vbytes) + 4.0 * (A.nrows + 1) +
+                  (double) sizeof(X) * (A.ncols + A.nrows) + (beta != (X) 0 ? sizeof(X) * (double) A.nrows : 0.0) +
+                  (ZR ? 2.0 * sizeof(X) * A.nrows : 0.0);
+   if (A.ell) {
+      const Ell &E = *A.ell;
+      const int nb = grid_for_groups(A.nrows, 64);
+      TimedLaunch tl(cls, bytes);
+      k_spmv_ell<V, X, DOT, ZR><<<nb, BS, 0, stream()>>>(A.nrows, E.nslices, A.ia, E.sw, E.j,
+                                                        ell_values<V>(E), x, alpha, beta, b, y,
+                                                        part, zl1, zw, zu);
+      HX_LAUNCH_CHECK();
+      return nb;
+   }
    const int rw = pick_rw(A.nnz, A.nrows);
    const int nb = grid_for_groups(A.nrows, rw);
-   double bytes = (double) A.nnz * (4.0 + vbytes) + 4.0 * (A.nrows + 1) +
-                  (double) sizeof(X) * (A.ncols + A.nrows) + (beta != (X) 0 ? sizeof(X) * (double) A.nrows : 0.0);
    TimedLaunch tl(cls, bytes);
    hipStream_t s = stream();
    const int sw = spmv_window(A.nnz, A.nrows);
@@ -471,6 +676,11 @@ static void launch_relax_l1(const Csr &A, const V *vals, const X *f, const X *u,
    if (zero) {
       const int nb = std::max(1, std::min((A.nrows + BS - 1) / BS, 16384));
       k_relax_l1_zero<X><<<nb, BS, 0, s>>>(A.nrows, f, l1, w, uo);
+   } else if (A.ell) {
+      const Ell &E = *A.ell;
+      const int nb = grid_for_groups(A.nrows, 64);
+      k_relax_l1_ell<V, X><<<nb, BS, 0, s>>>(A.nrows, E.nslices, A.ia, E.sw, E.j, ell_values<V>(E),
+                                             f, u, l1, w, uo);
    } else {
       const int rw = pick_rw(A.nnz, A.nrows);
       const int nb = grid_for_groups(A.nrows, rw);

@@ -97,7 +97,7 @@ SIGNATURES = {
     "HYPRE_BoomerAMGSetCycleType": (I, [P, I]),
     "HYPRE_BoomerAMGGetNumIterations": (I, [P, IP]),
     "HYPRE_BoomerAMGGetFinalRelativeResidualNorm": (I, [P, DP]),
-    "HYPRE_BoomerAMGGetNumLevels": (I, [P, IP]),
+    "HYPRE_MI355X_BoomerAMGGetNumLevels": (I, [P, IP]),
     # PCG
     "HYPRE_ParCSRPCGCreate": (I, [I, PP]),
     "HYPRE_ParCSRPCGDestroy": (I, [P]),
@@ -115,6 +115,7 @@ SIGNATURES = {
     "HYPRE_MI355X_BoomerAMGSetLevelStrongThreshold": (I, [P, I, D]),
     "HYPRE_MI355X_BoomerAMGSetLevelRelaxWt": (I, [P, I, D]),
     "HYPRE_MI355X_BoomerAMGSetHybridPartitions": (I, [P, I, I]),
+    "HYPRE_MI355X_BoomerAMGSetSolveFormat": (I, [P, I]),
     "HYPRE_MI355X_BoomerAMGSetLevelNumSweeps": (I, [P, I, I]),
     "HYPRE_MI355X_BoomerAMGClearLevelParams": (I, [P]),
     "HYPRE_MI355X_BoomerAMGSetKeepSetupData": (I, [P, I]),

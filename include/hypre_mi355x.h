@@ -33,6 +33,9 @@ HYPRE_Int HYPRE_MI355X_BoomerAMGSetLevelRelaxWt(HYPRE_Solver solver, HYPRE_Int l
 HYPRE_Int HYPRE_MI355X_BoomerAMGSetLevelNumSweeps(HYPRE_Solver solver, HYPRE_Int level,
                                                   HYPRE_Int sweeps);
 HYPRE_Int HYPRE_MI355X_BoomerAMGClearLevelParams(HYPRE_Solver solver);
+/* number of levels of the set-up hierarchy (hypre keeps it in the solver
+ * object, hypre_ParAMGDataNumLevels, but has no public getter) */
+HYPRE_Int HYPRE_MI355X_BoomerAMGGetNumLevels(HYPRE_Solver solver, HYPRE_Int *num_levels);
 /* row partitions of the hybrid Gauss-Seidel smoothers (relax 3/4/6/8/13/14),
  * which relax each partition in Gauss-Seidel order and couple partitions
  * through the iterate before the sweep, like the reference's OpenMP threads:
@@ -41,6 +44,16 @@ HYPRE_Int HYPRE_MI355X_BoomerAMGClearLevelParams(HYPRE_Solver solver);
  * fewer (down to 8) on levels with fewer than about 128K partitions. */
 HYPRE_Int HYPRE_MI355X_BoomerAMGSetHybridPartitions(HYPRE_Solver solver, HYPRE_Int parts,
                                                     HYPRE_Int rows);
+/* kernels of the solve phase.  1 (default): at setup, a sliced-ELL copy of
+ * each level operator (64-row slices, entries column-major, padding at most
+ * 25% of the entries, levels of at least 4096 rows) for the SpMV and
+ * l1-Jacobi kernels and PCG's products with the fine operator, and each
+ * coarse level's first l1-Jacobi sweep from a zero guess formed by the
+ * restriction that produces its right-hand side.  0: CSR kernels and one
+ * launch per sweep, the plain sequence of hypre's cycle.  Both give the same
+ * bits.  The copy belongs to the setup: values of A changed in place
+ * afterwards need a new setup. */
+HYPRE_Int HYPRE_MI355X_BoomerAMGSetSolveFormat(HYPRE_Solver solver, HYPRE_Int format);
 /* keep the strength pattern S and the PMIS measure of every level for
  * inspection (off by default; they are freed during setup otherwise) */
 HYPRE_Int HYPRE_MI355X_BoomerAMGSetKeepSetupData(HYPRE_Solver solver, HYPRE_Int keep);

@@ -58,3 +58,40 @@ def test_c_driver_uses_only_the_declared_api():
     assert {"HYPRE_IJMatrixSetValues", "HYPRE_ParCSRPCGSolve", "HYPRE_BoomerAMGSetup"} <= hypre
     assert hypre <= declared_functions()
     assert not any(s.startswith("HYPRE_MI355X") for s in hypre)
+
+
+def test_drop_in_headers_declare_only_reference_names():
+    """Every hypre function name that include/HYPRE*.h declares exists in the
+    reference's public headers of this path (names listed by
+    tests/golden/list_reference_api.py); extensions carry the HYPRE_MI355X_
+    prefix and live in include/hypre_mi355x.h."""
+    with open(os.path.join(ROOT, "tests", "golden", "reference_api_names.txt")) as f:
+        ref = set(f.read().split())
+    pat = re.compile(r"^\s*(?:HYPRE_Int|HYPRE_ParCSRMatrix|void)\s+\*?\s*((?:HYPRE_|Generate)\w+)\s*\(", re.M)
+    invented = {}
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        with open(os.path.join(ROOT, "include", fn)) as f:
+            names = set(pat.findall(f.read()))
+        if fn.startswith("HYPRE"):
+            assert not any(n.startswith("HYPRE_MI355X") for n in names), fn
+            extra = sorted(names - ref)
+            if extra:
+                invented[fn] = extra
+        else:
+            assert all(n.startswith("HYPRE_MI355X_") for n in names), fn
+    assert not invented, invented
+
+
+def test_c_driver_source_is_plain_hypre():
+    """The example passes its communicator as the integer handle of hypre's
+    sequential headers (src/utilities/HYPRE_utilities.h:175-177) and names no
+    macro or function that those headers do not provide."""
+    with open(EXAMPLE + ".c") as f:
+        src = f.read()
+    assert "MPI_COMM_WORLD" not in src and "HYPRE_MI355X" not in src
+    with open(os.path.join(ROOT, "tests", "golden", "reference_api_names.txt")) as f:
+        ref = set(f.read().split())
+    called = set(re.findall(r"\b(HYPRE_[A-Za-z0-9_]+)\s*\(", src))
+    assert called and called <= ref, sorted(called - ref)

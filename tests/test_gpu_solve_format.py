@@ -1,0 +1,77 @@
+"""The solve phase's two kernel sets give the same bits (through the C-ABI).
+
+Format 1 (default) runs the SpMV and l1-Jacobi sweeps on sliced-ELL copies of
+the level operators and forms each coarse level's first zero-guess sweep in
+the restriction's epilogue; format 0 runs the windowed CSR kernels and one
+launch per sweep, the plain kernel sequence of hypre's cycle
+(src/parcsr_ls/par_cycle.c:23).  Format 0 is the path the oracle parity tests
+pin; here a V-cycle (FP64 and FP32) must agree bit for bit between the two,
+also with per-level relaxation weights and two sweeps on coarse levels, and
+PCG must take the same number of iterations.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-solve-format]", *a, flush=True)
+    sys.stdout.flush()
+
+
+PROBLEMS = {
+    "lap27_40": lambda hx: hx.ParCSRMatrix.laplacian27(40, 40, 40),
+    "lap7_48": lambda hx: hx.ParCSRMatrix.laplacian7(48, 48, 48),
+    "jump27_32": lambda hx: hx.ParCSRMatrix.var_diffusion(32, 27, 8, -4.0, 4.0, 3),
+    "aniso7_40": lambda hx: hx.ParCSRMatrix.aniso7(40, 0.01, 0.7),
+}
+
+
+def cycle(hx, A, fmt, f, fp32=False, level_params=False):
+    amg = hx.BoomerAMG(max_iter=1, tol=0.0, fp32_cycle=1 if fp32 else 0, solve_format=fmt)
+    if level_params:
+        amg.set_level_params(omega=[0.9, 1.1, 0.8, 1.2, 0.95, 1.05], sweeps=[1, 2, 2, 1, 2, 2])
+    amg.setup(A)
+    u = amg.cycle_host(f)
+    amg.destroy()
+    return u
+
+
+@pytest.mark.parametrize("name", sorted(PROBLEMS))
+@pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("level_params", [False, True])
+def test_vcycle_bits_equal_across_formats(hx, oracle, name, fp32, level_params):
+    say(name, "fp32" if fp32 else "fp64", "level params" if level_params else "defaults")
+    A = PROBLEMS[name](hx)
+    n, _ = A.dims()
+    f = 2 * oracle.rand_stream(7, n) - 1
+    u0 = cycle(hx, A, 0, f, fp32, level_params)
+    u1 = cycle(hx, A, 1, f, fp32, level_params)
+    assert np.isfinite(u0).all() and np.abs(u0).max() > 0
+    bad = np.nonzero(u0.view(np.int64) != u1.view(np.int64))[0]
+    assert bad.size == 0, f"{bad.size} entries differ, first at {bad[:5]}"
+
+
+@pytest.mark.parametrize("name", ["lap27_40", "jump27_32"])
+def test_pcg_iterations_equal_across_formats(hx, oracle, name):
+    say("pcg", name)
+    A = PROBLEMS[name](hx)
+    n, _ = A.dims()
+    bh = 2 * oracle.rand_stream(11, n) - 1
+    out = []
+    for fmt in (0, 1):
+        b = hx.ParVector(n, bh)
+        x = hx.ParVector(n)
+        amg = hx.BoomerAMG(solve_format=fmt)
+        pcg = hx.PCG(amg, tol=1e-8)
+        pcg.setup(A, b, x)
+        pcg.solve(A, b, x)
+        out.append((pcg.num_iterations(), np.array(pcg.history()), x.get()))
+    (i0, h0, x0), (i1, h1, x1) = out
+    assert i0 == i1
+    # the fused PCG dot sums its partials in a different order: last bits only
+    np.testing.assert_allclose(h1, h0, rtol=1e-9)
+    np.testing.assert_allclose(x1, x0, rtol=1e-8, atol=1e-12 * np.abs(x0).max())
