@@ -55,25 +55,31 @@ def measured_traffic(n_grid: int, problem: str = "lap27"):
     return None
 
 
-def cpu_baseline(sample_n: int = 100):
-    """The CPU oracle (serial C port of the reference's host path) on a bounded
-    sample of the same workload: the 27-point Laplacian at sample_n^3, setup +
-    PCG solve, scaled to 200^3 by the nonzero ratio."""
+def cpu_baseline(N: int = 200):
+    """The CPU oracle (C restatement of the reference's host path, its
+    independent row loops under OpenMP on every host core) on the same
+    workload: 27-point Laplacian N^3, the same settings and RHS, hierarchy
+    setup + PCG solve to 1e-8, timed once (no scaling)."""
+    import numpy as np
     from oracle import oracle as O
-    A = O.gen_laplace27(sample_n, sample_n, sample_n)
+    A = O.gen_laplace27(N, N, N)
     b = 2.0 * O.rand_stream(0, A.nrows) - 1.0
     t0 = time.perf_counter()
-    H = O.Hierarchy(A)
+    H = O.Hierarchy(A, max_row_sum=0.9)
+    t1 = time.perf_counter()
     x, it, rr, hist, err = H.pcg(b, tol=1e-8)
-    t = time.perf_counter() - t0
-    nnz_full = (3 * 200 - 2) ** 3
+    t2 = time.perf_counter()
+    r = b - O.matvec(1.0, A, x)
     return {
-        "value": round(t * nnz_full / A.nnz, 3),
-        "unit": "s (setup+solve, scaled to 200^3 by nnz)",
-        "cores": 1,
+        "value": round(t2 - t0, 3),
+        "unit": "s (setup+solve)",
+        "cores": O.num_threads(),
         "kind": "port",
-        "sample": f"oracle setup+PCG on 27pt {sample_n}^3 ({A.nrows} rows, {A.nnz} nnz): "
-                  f"{t:.2f} s, {it} iterations, scaled x{nnz_full / A.nnz:.3f}",
+        "sample": f"the full workload: oracle setup + PCG on 27pt {N}^3 ({A.nrows} rows, {A.nnz} nnz), "
+                  f"setup {t1 - t0:.2f} s, solve {t2 - t1:.2f} s, {it} iterations, true rel. res. "
+                  f"{float(np.linalg.norm(r) / np.linalg.norm(b)):.3e}; host cores {os.cpu_count()}",
+        "context": "the reference's own OpenMP build (ij driver, 8 threads, a different host): "
+                   "21.56 s setup + 4.65 s solve on this problem with max_row_sum 1.0",
     }
 
 

@@ -3,17 +3,75 @@
  * Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
  * the checker; never linked into or called by the product library.
  *
- * Serial restatement of hypre 3.1.0's host path (single rank: every `offd`
- * block is empty).  Loops keep the reference's traversal and floating-point
- * operation order so that patterns and values are reproduced exactly; the
- * file is compiled with -ffp-contract=off.  Citations are file:line into the
- * hypre source tree (src/...).
+ * Restatement of hypre 3.1.0's host path (single rank: every `offd` block is
+ * empty).  Loops keep the reference's traversal and floating-point operation
+ * order so that patterns and values are reproduced exactly; the file is
+ * compiled with -ffp-contract=off.  Citations are file:line into the hypre
+ * source tree (src/...).
+ *
+ * Row loops whose rows are independent run under OpenMP (strength, PMIS
+ * measure and rounds, ext+i, truncation, transpose, RAP, l1 norms, SpMV,
+ * l1-Jacobi): every row is still computed by one thread with the serial
+ * arithmetic, rows that build a CSR matrix are counted first or buffered per
+ * thread in row order, and the first-seen markers are private to a thread
+ * whose rows come in ascending order, so the results are those of the serial
+ * loops bit for bit, for any thread count.  Dot products stay serial.
  */
 #include "amg_oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static int orc_threads(void)
+{
+#ifdef _OPENMP
+   return omp_get_max_threads();
+#else
+   return 1;
+#endif
+}
+
+static int orc_thread_id(void)
+{
+#ifdef _OPENMP
+   return omp_get_thread_num();
+#else
+   return 0;
+#endif
+}
+
+/* threads of the current team (1 outside a parallel region) */
+static int orc_team(void)
+{
+#ifdef _OPENMP
+   return omp_get_num_threads();
+#else
+   return 1;
+#endif
+}
+
+/* threads the oracle's parallel loops use */
+int orc_num_threads(void) { return orc_threads(); }
+
+/* rows [*r0, *r1) of thread t of nt: contiguous, in thread order */
+static void orc_chunk(int n, int nt, int t, int *r0, int *r1)
+{
+   long q = n / nt, rem = n % nt;
+   *r0 = (int) (t * q + (t < rem ? t : rem));
+   *r1 = *r0 + (int) q + (t < rem ? 1 : 0);
+}
+
+/* in-place exclusive prefix sum: c[0..n] holds counts in c[1..n] -> offsets */
+static void orc_prefix(int *c, int n)
+{
+   int i;
+   c[0] = 0;
+   for (i = 0; i < n; i++) { c[i + 1] += c[i]; }
+}
 
 #define ORC_MAX(a, b) (((a) < (b)) ? (b) : (a))
 #define ORC_MIN(a, b) (((a) < (b)) ? (a) : (b))
@@ -241,56 +299,64 @@ int orc_gen_aniso7(int N, double eps, double theta, orc_csr *A)
  * num_functions == 1, empty offd.  S keeps A's column order without the
  * (first) diagonal entry.
  * ====================================================================== */
-int orc_strength(const orc_csr *A, double theta, double max_row_sum, orc_csr *S)
+/* one row of S: strong columns in A's order into out (when non-NULL); returns
+ * their number */
+static int strength_row(const orc_csr *A, int i, double theta, double max_row_sum, int *out)
 {
-   int n = A->nrows, i, jA, cnt = 0;
    const int *ia = A->ia, *ja = A->ja;
    const double *a = A->a;
+   int jA, cnt = 0;
+   double diag = a[ia[i]], row_scale = 0.0, row_sum = diag;
+   if (diag < 0)
+   {
+      for (jA = ia[i] + 1; jA < ia[i + 1]; jA++)
+      {
+         row_scale = ORC_MAX(row_scale, a[jA]);
+         row_sum += a[jA];
+      }
+   }
+   else
+   {
+      for (jA = ia[i] + 1; jA < ia[i + 1]; jA++)
+      {
+         row_scale = ORC_MIN(row_scale, a[jA]);
+         row_sum += a[jA];
+      }
+   }
+   /* par_strength.c:362 all dependencies weak */
+   if ((fabs(row_sum) > fabs(diag) * max_row_sum) && (max_row_sum < 1.0))
+   {
+      /* no strong entries */
+   }
+   else if (diag < 0)
+   {
+      for (jA = ia[i] + 1; jA < ia[i + 1]; jA++)
+      {
+         if (!(a[jA] <= theta * row_scale)) { if (out) { out[cnt] = ja[jA]; } cnt++; }
+      }
+   }
+   else
+   {
+      for (jA = ia[i] + 1; jA < ia[i + 1]; jA++)
+      {
+         if (!(a[jA] >= theta * row_scale)) { if (out) { out[cnt] = ja[jA]; } cnt++; }
+      }
+   }
+   return cnt;
+}
+
+int orc_strength(const orc_csr *A, double theta, double max_row_sum, orc_csr *S)
+{
+   int n = A->nrows, i;
    S->nrows = n; S->ncols = A->ncols;
    S->ia = (int *) xmalloc(sizeof(int) * (size_t)(n + 1));
-   S->ja = (int *) xmalloc(sizeof(int) * (size_t) ia[n]);
    S->a = NULL;
-   S->ia[0] = 0;
-   for (i = 0; i < n; i++)
-   {
-      double diag = a[ia[i]], row_scale = 0.0, row_sum = diag;
-      if (diag < 0)
-      {
-         for (jA = ia[i] + 1; jA < ia[i + 1]; jA++)
-         {
-            row_scale = ORC_MAX(row_scale, a[jA]);
-            row_sum += a[jA];
-         }
-      }
-      else
-      {
-         for (jA = ia[i] + 1; jA < ia[i + 1]; jA++)
-         {
-            row_scale = ORC_MIN(row_scale, a[jA]);
-            row_sum += a[jA];
-         }
-      }
-      /* par_strength.c:362 all dependencies weak */
-      if ((fabs(row_sum) > fabs(diag) * max_row_sum) && (max_row_sum < 1.0))
-      {
-         /* no strong entries */
-      }
-      else if (diag < 0)
-      {
-         for (jA = ia[i] + 1; jA < ia[i + 1]; jA++)
-         {
-            if (!(a[jA] <= theta * row_scale)) { S->ja[cnt++] = ja[jA]; }
-         }
-      }
-      else
-      {
-         for (jA = ia[i] + 1; jA < ia[i + 1]; jA++)
-         {
-            if (!(a[jA] >= theta * row_scale)) { S->ja[cnt++] = ja[jA]; }
-         }
-      }
-      S->ia[i + 1] = cnt;
-   }
+#pragma omp parallel for schedule(static)
+   for (i = 0; i < n; i++) { S->ia[i + 1] = strength_row(A, i, theta, max_row_sum, NULL); }
+   orc_prefix(S->ia, n);
+   S->ja = (int *) xmalloc(sizeof(int) * (size_t) (S->ia[n] > 0 ? S->ia[n] : 1));
+#pragma omp parallel for schedule(static)
+   for (i = 0; i < n; i++) { strength_row(A, i, theta, max_row_sum, S->ja + S->ia[i]); }
    return 0;
 }
 
@@ -301,8 +367,20 @@ int orc_pmis_measure(const orc_csr *S, int seed, double *measure)
 {
    int n = S->nrows, i;
    /* par_coarsen.c:2248-2262 column counts of S */
-   for (i = 0; i < n; i++) { measure[i] = 0.0; }
-   for (i = 0; i < S->ia[n]; i++) { measure[S->ja[i]] += 1.0; }
+   {
+      /* integer counts: the same values as adding 1.0 per entry in any order */
+      int *cnt = (int *) xcalloc((size_t) (n > 0 ? n : 1), sizeof(int));
+      const long nnz = S->ia[n];
+      long k;
+#pragma omp parallel for schedule(static)
+      for (k = 0; k < nnz; k++)
+      {
+#pragma omp atomic update
+         cnt[S->ja[k]]++;
+      }
+      for (i = 0; i < n; i++) { measure[i] = (double) cnt[i]; }
+      free(cnt);
+   }
    /* par_indepset.c:50-75 hypre_BoomerAMGIndepSetInit, seed 2747 + rank */
    orc_seed_rand(seed);
    for (i = 0; i < n; i++) { measure[i] += orc_rand(); }
@@ -338,12 +416,16 @@ int orc_pmis_with_measure(const orc_csr *S, const double *measure_in, int *CF)
    while (1)   /* par_coarsen.c:2460 THE INDEPENDENT SET COARSENING LOOP */
    {
       if (graph_size == 0) { break; }
-      /* par_coarsen.c:2485-2550 pick the independent set */
+      /* par_coarsen.c:2485-2550 pick the independent set.  Every write of
+       * the second loop stores 0, so the outcome does not depend on the
+       * order of the rows. */
+#pragma omp parallel for private(i) schedule(static)
       for (ig = 0; ig < graph_size; ig++)
       {
          i = graph[ig];
          if (measure[i] > 1) { CF[i] = 1; }
       }
+#pragma omp parallel for private(i, j, jS) schedule(static)
       for (ig = 0; ig < graph_size; ig++)
       {
          i = graph[ig];
@@ -354,30 +436,49 @@ int orc_pmis_with_measure(const orc_csr *S, const double *measure_in, int *CF)
                j = Sj[jS];
                if (measure[j] > 1)
                {
-                  if (measure[i] > measure[j])      { CF[j] = 0; }
-                  else if (measure[j] > measure[i]) { CF[i] = 0; }
+                  if (measure[i] > measure[j])
+                  {
+#pragma omp atomic write
+                     CF[j] = 0;
+                  }
+                  else if (measure[j] > measure[i])
+                  {
+#pragma omp atomic write
+                     CF[i] = 0;
+                  }
                }
             }
          }
       }
       iter++;
-      /* par_coarsen.c:2603-2660 set C and F points */
+      /* par_coarsen.c:2603-2660 set C and F points.  Which points have
+       * CF > 0 does not change in this loop (C points stay 1, the others
+       * become -1 or stay 0), so neighbours read the same either way. */
+#pragma omp parallel for private(i, j, jS) schedule(static)
       for (ig = 0; ig < graph_size; ig++)
       {
+         int c;
          i = graph[ig];
-         if (measure[i] < 1) { CF[i] = F_PT; }
-         if (CF[i] > 0)
+#pragma omp atomic read
+         c = CF[i];
+         if (measure[i] < 1) { c = F_PT; }
+         if (c > 0)
          {
-            CF[i] = C_PT;
+            c = C_PT;
          }
          else
          {
             for (jS = Si[i]; jS < Si[i + 1]; jS++)
             {
+               int cj;
                j = Sj[jS];
-               if (CF[j] > 0) { CF[i] = F_PT; }
+#pragma omp atomic read
+               cj = CF[j];
+               if (cj > 0) { c = F_PT; }
             }
          }
+#pragma omp atomic write
+         CF[i] = c;
       }
       /* par_coarsen.c:2690-2750 update the subgraph */
       cnt = 0;
@@ -410,163 +511,182 @@ int orc_pmis(const orc_csr *S, int *CF)
  * ====================================================================== */
 int orc_extpi_interp_raw(const orc_csr *A, const orc_csr *S, const int *CF, orc_csr *P)
 {
-   int n = A->nrows, i, i1, i2, jj, kk, k1, jj1, nc = 0;
+   int n = A->nrows, i, nc = 0;
    const int *Ai = A->ia, *Aj = A->ja, *Si = S->ia, *Sj = S->ja;
    const double *Ad = A->a;
-   int *f2c = (int *) xmalloc(sizeof(int) * (size_t) n);
-   int *P_marker = (int *) xmalloc(sizeof(int) * (size_t) n);
-   int jj_counter = 0, strong_f_marker = -2;
+   int *f2c = (int *) xmalloc(sizeof(int) * (size_t) (n > 0 ? n : 1));
    long cap;
 
-   /* first pass: fine_to_coarse and row sizes (par_lr_interp.c:1230-1400) */
+   /* fine_to_coarse (par_lr_interp.c:1230-1400) */
+   for (i = 0; i < n; i++) { f2c[i] = (CF[i] >= 0) ? nc++ : -1; }
    P->nrows = n;
-   P->ia = (int *) xmalloc(sizeof(int) * (size_t)(n + 1));
-   for (i = 0; i < n; i++) { P_marker[i] = -1; f2c[i] = -1; }
-   for (i = 0; i < n; i++)
-   {
-      P->ia[i] = jj_counter;
-      if (CF[i] >= 0)
-      {
-         jj_counter++;
-         f2c[i] = nc++;
-      }
-      else if (CF[i] != SF_PT)
-      {
-         for (jj = Si[i]; jj < Si[i + 1]; jj++)
-         {
-            i1 = Sj[jj];
-            if (CF[i1] >= 0)
-            {
-               if (P_marker[i1] < P->ia[i]) { P_marker[i1] = jj_counter; jj_counter++; }
-            }
-            else if (CF[i1] != SF_PT)
-            {
-               for (kk = Si[i1]; kk < Si[i1 + 1]; kk++)
-               {
-                  k1 = Sj[kk];
-                  if (CF[k1] >= 0)
-                  {
-                     if (P_marker[k1] < P->ia[i]) { P_marker[k1] = jj_counter; jj_counter++; }
-                  }
-               }
-            }
-         }
-      }
-   }
-   P->ia[n] = jj_counter;
    P->ncols = nc;
-   cap = jj_counter;
-   P->ja = (int *) xmalloc(sizeof(int) * (size_t) cap);
-   P->a = (double *) xmalloc(sizeof(double) * (size_t) cap);
+   P->ia = (int *) xmalloc(sizeof(int) * (size_t)(n + 1));
 
-   /* second pass: fill (par_lr_interp.c:1485-1810) */
-   for (i = 0; i < n; i++) { P_marker[i] = -1; }
-   for (i = 0; i < n; i++)
+   /* first pass: row sizes (par_lr_interp.c:1230-1400).  P_marker holds
+    * positions of a running counter; each thread keeps its own marker array
+    * and counter and takes its rows in ascending order, which is all the
+    * "marker < row start" test needs. */
+#pragma omp parallel private(i)
    {
-      int jj_begin_row = P->ia[i], jj_end_row;
-      int jj_counter2 = jj_begin_row;
-      if (CF[i] >= 0)
+      int *P_marker = (int *) xmalloc(sizeof(int) * (size_t) (n > 0 ? n : 1));
+      int jj_counter = 0, jj, kk, i1, k1;
+      for (i = 0; i < n; i++) { P_marker[i] = -1; }
+#pragma omp for schedule(static)
+      for (i = 0; i < n; i++)
       {
-         P->ja[jj_counter2] = f2c[i];
-         P->a[jj_counter2] = 1.0;
-         jj_counter2++;
-      }
-      else if (CF[i] != SF_PT)
-      {
-         double diagonal, sum, distribute;
-         int sgn;
-         strong_f_marker--;
-         for (jj = Si[i]; jj < Si[i + 1]; jj++)
+         const int row_begin = jj_counter;
+         if (CF[i] >= 0)
          {
-            i1 = Sj[jj];
-            if (CF[i1] >= 0)
+            jj_counter++;
+         }
+         else if (CF[i] != SF_PT)
+         {
+            for (jj = Si[i]; jj < Si[i + 1]; jj++)
             {
-               if (P_marker[i1] < jj_begin_row)
+               i1 = Sj[jj];
+               if (CF[i1] >= 0)
                {
-                  P_marker[i1] = jj_counter2;
-                  P->ja[jj_counter2] = f2c[i1];
-                  P->a[jj_counter2] = 0.0;
-                  jj_counter2++;
+                  if (P_marker[i1] < row_begin) { P_marker[i1] = jj_counter; jj_counter++; }
                }
-            }
-            else if (CF[i1] != SF_PT)
-            {
-               P_marker[i1] = strong_f_marker;
-               for (kk = Si[i1]; kk < Si[i1 + 1]; kk++)
+               else if (CF[i1] != SF_PT)
                {
-                  k1 = Sj[kk];
-                  if (CF[k1] >= 0)
+                  for (kk = Si[i1]; kk < Si[i1 + 1]; kk++)
                   {
-                     if (P_marker[k1] < jj_begin_row)
+                     k1 = Sj[kk];
+                     if (CF[k1] >= 0)
                      {
-                        P_marker[k1] = jj_counter2;
-                        P->ja[jj_counter2] = f2c[k1];
-                        P->a[jj_counter2] = 0.0;
-                        jj_counter2++;
+                        if (P_marker[k1] < row_begin) { P_marker[k1] = jj_counter; jj_counter++; }
                      }
                   }
                }
             }
          }
-         jj_end_row = jj_counter2;
-         diagonal = Ad[Ai[i]];
-         for (jj = Ai[i] + 1; jj < Ai[i + 1]; jj++)
+         P->ia[i + 1] = jj_counter - row_begin;
+      }
+      free(P_marker);
+   }
+   orc_prefix(P->ia, n);
+   cap = P->ia[n];
+   P->ja = (int *) xmalloc(sizeof(int) * (size_t) (cap > 0 ? cap : 1));
+   P->a = (double *) xmalloc(sizeof(double) * (size_t) (cap > 0 ? cap : 1));
+
+   /* second pass: fill (par_lr_interp.c:1485-1810).  The strong-F tag of a
+    * row, -2 - i, is unique to the row, like the reference's decreasing
+    * strong_f_marker, and below every position. */
+#pragma omp parallel private(i)
+   {
+      int *P_marker = (int *) xmalloc(sizeof(int) * (size_t) (n > 0 ? n : 1));
+      int jj, kk, jj1, i1, i2, k1;
+      for (i = 0; i < n; i++) { P_marker[i] = -1; }
+#pragma omp for schedule(static)
+      for (i = 0; i < n; i++)
+      {
+         int jj_begin_row = P->ia[i], jj_end_row;
+         int jj_counter2 = jj_begin_row;
+         const int strong_f_marker = -2 - i;
+         if (CF[i] >= 0)
          {
-            i1 = Aj[jj];
-            if (P_marker[i1] >= jj_begin_row)
+            P->ja[jj_counter2] = f2c[i];
+            P->a[jj_counter2] = 1.0;
+            jj_counter2++;
+         }
+         else if (CF[i] != SF_PT)
+         {
+            double diagonal, sum, distribute;
+            int sgn;
+            for (jj = Si[i]; jj < Si[i + 1]; jj++)
             {
-               P->a[P_marker[i1]] += Ad[jj];
-            }
-            else if (P_marker[i1] == strong_f_marker)
-            {
-               sum = 0.0;
-               sgn = 1;
-               if (Ad[Ai[i1]] < 0) { sgn = -1; }
-               for (jj1 = Ai[i1] + 1; jj1 < Ai[i1 + 1]; jj1++)
+               i1 = Sj[jj];
+               if (CF[i1] >= 0)
                {
-                  i2 = Aj[jj1];
-                  if ((P_marker[i2] >= jj_begin_row || i2 == i) && (sgn * Ad[jj1]) < 0)
+                  if (P_marker[i1] < jj_begin_row)
                   {
-                     sum += Ad[jj1];
+                     P_marker[i1] = jj_counter2;
+                     P->ja[jj_counter2] = f2c[i1];
+                     P->a[jj_counter2] = 0.0;
+                     jj_counter2++;
                   }
                }
-               if (sum != 0)
+               else if (CF[i1] != SF_PT)
                {
-                  distribute = Ad[jj] / sum;
+                  P_marker[i1] = strong_f_marker;
+                  for (kk = Si[i1]; kk < Si[i1 + 1]; kk++)
+                  {
+                     k1 = Sj[kk];
+                     if (CF[k1] >= 0)
+                     {
+                        if (P_marker[k1] < jj_begin_row)
+                        {
+                           P_marker[k1] = jj_counter2;
+                           P->ja[jj_counter2] = f2c[k1];
+                           P->a[jj_counter2] = 0.0;
+                           jj_counter2++;
+                        }
+                     }
+                  }
+               }
+            }
+            jj_end_row = jj_counter2;
+            diagonal = Ad[Ai[i]];
+            for (jj = Ai[i] + 1; jj < Ai[i + 1]; jj++)
+            {
+               i1 = Aj[jj];
+               if (P_marker[i1] >= jj_begin_row)
+               {
+                  P->a[P_marker[i1]] += Ad[jj];
+               }
+               else if (P_marker[i1] == strong_f_marker)
+               {
+                  sum = 0.0;
+                  sgn = 1;
+                  if (Ad[Ai[i1]] < 0) { sgn = -1; }
                   for (jj1 = Ai[i1] + 1; jj1 < Ai[i1 + 1]; jj1++)
                   {
                      i2 = Aj[jj1];
-                     if (P_marker[i2] >= jj_begin_row && (sgn * Ad[jj1]) < 0)
+                     if ((P_marker[i2] >= jj_begin_row || i2 == i) && (sgn * Ad[jj1]) < 0)
                      {
-                        P->a[P_marker[i2]] += distribute * Ad[jj1];
-                     }
-                     if (i2 == i && (sgn * Ad[jj1]) < 0)
-                     {
-                        diagonal += distribute * Ad[jj1];
+                        sum += Ad[jj1];
                      }
                   }
+                  if (sum != 0)
+                  {
+                     distribute = Ad[jj] / sum;
+                     for (jj1 = Ai[i1] + 1; jj1 < Ai[i1 + 1]; jj1++)
+                     {
+                        i2 = Aj[jj1];
+                        if (P_marker[i2] >= jj_begin_row && (sgn * Ad[jj1]) < 0)
+                        {
+                           P->a[P_marker[i2]] += distribute * Ad[jj1];
+                        }
+                        if (i2 == i && (sgn * Ad[jj1]) < 0)
+                        {
+                           diagonal += distribute * Ad[jj1];
+                        }
+                     }
+                  }
+                  else
+                  {
+                     diagonal += Ad[jj];
+                  }
                }
-               else
+               else if (CF[i1] != SF_PT)
                {
                   diagonal += Ad[jj];
                }
             }
-            else if (CF[i1] != SF_PT)
+            if (diagonal != 0.0)
             {
-               diagonal += Ad[jj];
-            }
-         }
-         if (diagonal != 0.0)
-         {
-            for (jj = jj_begin_row; jj < jj_end_row; jj++)
-            {
-               P->a[jj] /= -diagonal;
+               for (jj = jj_begin_row; jj < jj_end_row; jj++)
+               {
+                  P->a[jj] /= -diagonal;
+               }
             }
          }
       }
-      strong_f_marker--;
+      free(P_marker);
    }
-   free(f2c); free(P_marker);
+   free(f2c);
    return 0;
 }
 
@@ -643,59 +763,70 @@ int orc_truncate(orc_csr *P, double tol, int max_row_elmts)
    }
    if (max_row_elmts > 0)
    {
-      int mx = 0, cnt_diag = 0;
-      int *aux_j;
-      double *aux_d;
+      int mx = 0;
       for (i = 0; i < n; i++) { mx = ORC_MAX(mx, Pi[i + 1] - Pi[i]); }
       if (mx > max_row_elmts)
       {
-         aux_j = (int *) xmalloc(sizeof(int) * (size_t) mx);
-         aux_d = (double *) xmalloc(sizeof(double) * (size_t) mx);
-         for (i = 0; i < n; i++)
+         /* rows keep min(length, max_row_elmts) entries; written to new
+          * arrays at offsets counted first (the reference compacts in place
+          * with a running cnt_diag, which yields the same arrays) */
+         int *ni = (int *) xmalloc(sizeof(int) * (size_t) (n + 1));
+         int *nj;
+         double *nd;
+         for (i = 0; i < n; i++) { ni[i + 1] = ORC_MIN(Pi[i + 1] - Pi[i], max_row_elmts); }
+         orc_prefix(ni, n);
+         nj = (int *) xmalloc(sizeof(int) * (size_t) (ni[n] > 0 ? ni[n] : 1));
+         nd = (double *) xmalloc(sizeof(double) * (size_t) (ni[n] > 0 ? ni[n] : 1));
+#pragma omp parallel private(i, j)
          {
-            int last_index = Pi[i + 1];
-            int num_elmts = last_index - Pi[i];
-            double row_sum = 0, scale;
-            if (max_row_elmts < num_elmts)
+            int *aux_j = (int *) xmalloc(sizeof(int) * (size_t) mx);
+            double *aux_d = (double *) xmalloc(sizeof(double) * (size_t) mx);
+#pragma omp for schedule(static)
+            for (i = 0; i < n; i++)
             {
-               int cnt = 0, row_begin = cnt_diag;
-               for (j = Pi[i]; j < last_index; j++)
+               int last_index = Pi[i + 1];
+               int num_elmts = last_index - Pi[i];
+               int cnt_diag = ni[i];
+               double row_sum = 0, scale;
+               if (max_row_elmts < num_elmts)
                {
-                  aux_j[cnt] = Pj[j];
-                  aux_d[cnt++] = Pd[j];
-                  row_sum += Pd[j];
-               }
-               orc_qsort2_abs(aux_j, aux_d, 0, cnt - 1);
-               scale = 0;
-               Pi[i] = cnt_diag;
-               for (j = 0; j < max_row_elmts; j++)
-               {
-                  scale += aux_d[j];
-                  Pj[cnt_diag] = aux_j[j];
-                  Pd[cnt_diag++] = aux_d[j];
-               }
-               if (scale != 0.)
-               {
-                  if (scale != row_sum)
+                  int cnt = 0, row_begin = cnt_diag;
+                  for (j = Pi[i]; j < last_index; j++)
                   {
-                     scale = row_sum / scale;
-                     for (j = row_begin; j < cnt_diag; j++) { Pd[j] *= scale; }
+                     aux_j[cnt] = Pj[j];
+                     aux_d[cnt++] = Pd[j];
+                     row_sum += Pd[j];
+                  }
+                  orc_qsort2_abs(aux_j, aux_d, 0, cnt - 1);
+                  scale = 0;
+                  for (j = 0; j < max_row_elmts; j++)
+                  {
+                     scale += aux_d[j];
+                     nj[cnt_diag] = aux_j[j];
+                     nd[cnt_diag++] = aux_d[j];
+                  }
+                  if (scale != 0.)
+                  {
+                     if (scale != row_sum)
+                     {
+                        scale = row_sum / scale;
+                        for (j = row_begin; j < cnt_diag; j++) { nd[j] *= scale; }
+                     }
+                  }
+               }
+               else
+               {
+                  for (j = Pi[i]; j < last_index; j++)
+                  {
+                     nj[cnt_diag] = Pj[j];
+                     nd[cnt_diag++] = Pd[j];
                   }
                }
             }
-            else
-            {
-               int start_j = Pi[i];
-               Pi[i] = cnt_diag;
-               for (j = start_j; j < last_index; j++)
-               {
-                  Pj[cnt_diag] = Pj[j];
-                  Pd[cnt_diag++] = Pd[j];
-               }
-            }
+            free(aux_j); free(aux_d);
          }
-         Pi[n] = cnt_diag;
-         free(aux_j); free(aux_d);
+         free(P->ia); free(P->ja); free(P->a);
+         P->ia = ni; P->ja = nj; P->a = nd;
       }
    }
    return 0;
@@ -723,22 +854,50 @@ int orc_extpi_interp(const orc_csr *A, const orc_csr *S, int *CF,
  * sort, so each row of R lists the fine rows of P in ascending order. */
 int orc_transpose(const orc_csr *P, orc_csr *R)
 {
-   int n = P->nrows, nc = P->ncols, i, j, nnz = P->ia[n];
-   int *cnt = (int *) xcalloc((size_t)(nc + 1), sizeof(int));
+   int n = P->nrows, nc = P->ncols, nnz = P->ia[n], c;
+   const int nt = orc_threads();
+   /* per-thread column counts over contiguous row chunks; offsets ordered by
+    * (column, thread) keep each column's rows ascending, as the serial sort */
+   int *cnt = (int *) xcalloc((size_t) nt * (size_t) (nc + 1), sizeof(int));
    R->nrows = nc; R->ncols = n;
    R->ia = (int *) xmalloc(sizeof(int) * (size_t)(nc + 1));
-   R->ja = (int *) xmalloc(sizeof(int) * (size_t) nnz);
-   R->a = (double *) xmalloc(sizeof(double) * (size_t) nnz);
-   for (j = 0; j < nnz; j++) { cnt[P->ja[j] + 1]++; }
-   for (i = 0; i < nc; i++) { cnt[i + 1] += cnt[i]; }
-   memcpy(R->ia, cnt, sizeof(int) * (size_t)(nc + 1));
-   for (i = 0; i < n; i++)
+   R->ja = (int *) xmalloc(sizeof(int) * (size_t) (nnz > 0 ? nnz : 1));
+   R->a = (double *) xmalloc(sizeof(double) * (size_t) (nnz > 0 ? nnz : 1));
+#pragma omp parallel num_threads(nt)
    {
-      for (j = P->ia[i]; j < P->ia[i + 1]; j++)
+      const int t = orc_thread_id(), team = orc_team();
+      int r0, r1, i, j;
+      int *ct = cnt + (size_t) t * (nc + 1);
+      orc_chunk(n, team, t, &r0, &r1);
+      for (i = r0; i < r1; i++)
       {
-         int c = P->ja[j], pos = cnt[c]++;
-         R->ja[pos] = i;
-         R->a[pos] = P->a[j];
+         for (j = P->ia[i]; j < P->ia[i + 1]; j++) { ct[P->ja[j]]++; }
+      }
+#pragma omp barrier
+#pragma omp single
+      {
+         int run = 0, tt;
+         for (c = 0; c < nc; c++)
+         {
+            R->ia[c] = run;
+            for (tt = 0; tt < team; tt++)
+            {
+               int *cc = cnt + (size_t) tt * (nc + 1);
+               const int k = cc[c];
+               cc[c] = run;
+               run += k;
+            }
+         }
+         R->ia[nc] = run;
+      }
+      for (i = r0; i < r1; i++)
+      {
+         for (j = P->ia[i]; j < P->ia[i + 1]; j++)
+         {
+            const int pos = ct[P->ja[j]]++;
+            R->ja[pos] = i;
+            R->a[pos] = P->a[j];
+         }
       }
    }
    free(cnt);
@@ -750,98 +909,129 @@ int orc_transpose(const orc_csr *P, orc_csr *R)
  * diagonal entry is created first (par_rap.c:1612-1620). */
 int orc_rap(const orc_csr *R, const orc_csr *A, const orc_csr *P, orc_csr *Ac)
 {
-   int nc = R->nrows, nf = A->nrows, ic, jj1, jj2, i1, i2;
-   int *A_marker = (int *) xmalloc(sizeof(int) * (size_t) nf);
-   int *P_marker = (int *) xmalloc(sizeof(int) * (size_t) nc);
-   int *RA_j = (int *) xmalloc(sizeof(int) * (size_t) nf);
-   double *RA_d = (double *) xmalloc(sizeof(double) * (size_t) nf);
-   long cap = 1024, cnt = 0;
-   int ra_nnz = 0;
+   int nc = R->nrows, nf = A->nrows;
+   const int nt = orc_threads();
+   /* each thread builds the rows of a contiguous chunk into its own buffer
+    * (positions counted from 0 for its first row, so the marker tests of the
+    * serial loop hold unchanged), then the buffers are concatenated in order */
+   int **tj = (int **) xcalloc((size_t) nt, sizeof(int *));
+   double **td = (double **) xcalloc((size_t) nt, sizeof(double *));
+   long *tcnt = (long *) xcalloc((size_t) nt, sizeof(long));
+   int team_size = 1;
    Ac->nrows = nc; Ac->ncols = nc;
    Ac->ia = (int *) xmalloc(sizeof(int) * (size_t)(nc + 1));
-   Ac->ja = (int *) xmalloc(sizeof(int) * (size_t) cap);
-   Ac->a = (double *) xmalloc(sizeof(double) * (size_t) cap);
-   for (i1 = 0; i1 < nf; i1++) { A_marker[i1] = -1; }
-   for (i1 = 0; i1 < nc; i1++) { P_marker[i1] = -1; }
-   for (ic = 0; ic < nc; ic++)
+#pragma omp parallel num_threads(nt)
    {
-      long jj_row_begin = cnt;
-      int ra_row_begin = ra_nnz;
-      int ra_cnt = 0;
-      Ac->ia[ic] = (int) cnt;
-      /* RA row: par_rap.c:1712-1800 (A_marker holds absolute RA positions) */
-      for (jj1 = R->ia[ic]; jj1 < R->ia[ic + 1]; jj1++)
+      const int t = orc_thread_id(), team = orc_team();
+      int ic, jj1, jj2, i1, i2, r0, r1;
+      int *A_marker = (int *) xmalloc(sizeof(int) * (size_t) (nf > 0 ? nf : 1));
+      int *P_marker = (int *) xmalloc(sizeof(int) * (size_t) (nc > 0 ? nc : 1));
+      int *RA_j = (int *) xmalloc(sizeof(int) * (size_t) (nf > 0 ? nf : 1));
+      double *RA_d = (double *) xmalloc(sizeof(double) * (size_t) (nf > 0 ? nf : 1));
+      long cap = 1024, cnt = 0;
+      int ra_nnz = 0;
+      int *Acj = (int *) xmalloc(sizeof(int) * (size_t) cap);
+      double *Acd = (double *) xmalloc(sizeof(double) * (size_t) cap);
+      if (t == 0) { team_size = team; }
+      orc_chunk(nc, team, t, &r0, &r1);
+      for (i1 = 0; i1 < nf; i1++) { A_marker[i1] = -1; }
+      for (i1 = 0; i1 < nc; i1++) { P_marker[i1] = -1; }
+      for (ic = r0; ic < r1; ic++)
       {
-         double r_entry = R->a[jj1];
-         i1 = R->ja[jj1];
-         for (jj2 = A->ia[i1]; jj2 < A->ia[i1 + 1]; jj2++)
+         long jj_row_begin = cnt;
+         int ra_row_begin = ra_nnz;
+         int ra_cnt = 0;
+         /* RA row: par_rap.c:1712-1800 (A_marker holds absolute RA positions) */
+         for (jj1 = R->ia[ic]; jj1 < R->ia[ic + 1]; jj1++)
          {
-            double a_entry = A->a[jj2];
-            int marker;
-            i2 = A->ja[jj2];
-            marker = A_marker[i2];
-            if (marker < ra_row_begin)
+            double r_entry = R->a[jj1];
+            i1 = R->ja[jj1];
+            for (jj2 = A->ia[i1]; jj2 < A->ia[i1 + 1]; jj2++)
             {
-               A_marker[i2] = ra_nnz;
-               RA_d[ra_nnz - ra_row_begin] = r_entry * a_entry;
-               RA_j[ra_nnz - ra_row_begin] = i2;
-               ra_nnz++;
-            }
-            else
-            {
-               RA_d[marker - ra_row_begin] += r_entry * a_entry;
+               double a_entry = A->a[jj2];
+               int marker;
+               i2 = A->ja[jj2];
+               marker = A_marker[i2];
+               if (marker < ra_row_begin)
+               {
+                  A_marker[i2] = ra_nnz;
+                  RA_d[ra_nnz - ra_row_begin] = r_entry * a_entry;
+                  RA_j[ra_nnz - ra_row_begin] = i2;
+                  ra_nnz++;
+               }
+               else
+               {
+                  RA_d[marker - ra_row_begin] += r_entry * a_entry;
+               }
             }
          }
-      }
-      ra_cnt = ra_nnz - ra_row_begin;
-      /* make room for the row of RAP: at most 1 + sum |P(i1,:)| entries */
-      {
-         long bound = 1;
+         ra_cnt = ra_nnz - ra_row_begin;
+         /* make room for the row of RAP: at most 1 + sum |P(i1,:)| entries */
+         {
+            long bound = 1;
+            for (jj1 = 0; jj1 < ra_cnt; jj1++)
+            {
+               i1 = RA_j[jj1];
+               bound += P->ia[i1 + 1] - P->ia[i1];
+            }
+            if (cnt + bound > cap)
+            {
+               while (cnt + bound > cap) { cap *= 2; }
+               Acj = (int *) realloc(Acj, sizeof(int) * (size_t) cap);
+               Acd = (double *) realloc(Acd, sizeof(double) * (size_t) cap);
+               if (!Acj || !Acd) { abort(); }
+            }
+         }
+         /* diagonal first */
+         P_marker[ic] = (int) cnt;
+         Acd[cnt] = 0.0;
+         Acj[cnt] = ic;
+         cnt++;
+         /* (RA) x P: par_rap.c:1860-1920 */
          for (jj1 = 0; jj1 < ra_cnt; jj1++)
          {
+            double r_a_product = RA_d[jj1];
             i1 = RA_j[jj1];
-            bound += P->ia[i1 + 1] - P->ia[i1];
+            for (jj2 = P->ia[i1]; jj2 < P->ia[i1 + 1]; jj2++)
+            {
+               double p_entry = P->a[jj2];
+               int marker;
+               i2 = P->ja[jj2];
+               marker = P_marker[i2];
+               if (marker < jj_row_begin)
+               {
+                  P_marker[i2] = (int) cnt;
+                  Acd[cnt] = r_a_product * p_entry;
+                  Acj[cnt] = i2;
+                  cnt++;
+               }
+               else
+               {
+                  Acd[marker] += r_a_product * p_entry;
+               }
+            }
          }
-         if (cnt + bound > cap)
-         {
-            while (cnt + bound > cap) { cap *= 2; }
-            Ac->ja = (int *) realloc(Ac->ja, sizeof(int) * (size_t) cap);
-            Ac->a = (double *) realloc(Ac->a, sizeof(double) * (size_t) cap);
-            if (!Ac->ja || !Ac->a) { abort(); }
-         }
+         Ac->ia[ic + 1] = (int) (cnt - jj_row_begin);
       }
-      /* diagonal first */
-      P_marker[ic] = (int) cnt;
-      Ac->a[cnt] = 0.0;
-      Ac->ja[cnt] = ic;
-      cnt++;
-      /* (RA) x P: par_rap.c:1860-1920 */
-      for (jj1 = 0; jj1 < ra_cnt; jj1++)
+      tj[t] = Acj; td[t] = Acd; tcnt[t] = cnt;
+      free(A_marker); free(P_marker); free(RA_j); free(RA_d);
+   }
+   {
+      long total = 0, off = 0;
+      int t;
+      for (t = 0; t < team_size; t++) { total += tcnt[t]; }
+      orc_prefix(Ac->ia, nc);
+      Ac->ja = (int *) xmalloc(sizeof(int) * (size_t) (total > 0 ? total : 1));
+      Ac->a = (double *) xmalloc(sizeof(double) * (size_t) (total > 0 ? total : 1));
+      for (t = 0; t < team_size; t++)
       {
-         double r_a_product = RA_d[jj1];
-         i1 = RA_j[jj1];
-         for (jj2 = P->ia[i1]; jj2 < P->ia[i1 + 1]; jj2++)
-         {
-            double p_entry = P->a[jj2];
-            int marker;
-            i2 = P->ja[jj2];
-            marker = P_marker[i2];
-            if (marker < jj_row_begin)
-            {
-               P_marker[i2] = (int) cnt;
-               Ac->a[cnt] = r_a_product * p_entry;
-               Ac->ja[cnt] = i2;
-               cnt++;
-            }
-            else
-            {
-               Ac->a[marker] += r_a_product * p_entry;
-            }
-         }
+         memcpy(Ac->ja + off, tj[t], sizeof(int) * (size_t) tcnt[t]);
+         memcpy(Ac->a + off, td[t], sizeof(double) * (size_t) tcnt[t]);
+         off += tcnt[t];
+         free(tj[t]); free(td[t]);
       }
    }
-   Ac->ia[nc] = (int) cnt;
-   free(A_marker); free(P_marker); free(RA_j); free(RA_d);
+   free(tj); free(td); free(tcnt);
    return 0;
 }
 
@@ -850,13 +1040,11 @@ int orc_rap(const orc_csr *R, const orc_csr *A, const orc_csr *P, orc_csr *Ac)
 int orc_l1_norms(const orc_csr *A, double *l1)
 {
    int i, j, bad = 0;
+#pragma omp parallel for private(j) schedule(static)
    for (i = 0; i < A->nrows; i++)
    {
       l1[i] = 0.0;
       for (j = A->ia[i]; j < A->ia[i + 1]; j++) { l1[i] += fabs(A->a[j]); }
-   }
-   for (i = 0; i < A->nrows; i++)
-   {
       if (A->a[A->ia[i]] < 0.0) { l1[i] = -l1[i]; }
    }
    for (i = 0; i < A->nrows; i++)
@@ -995,6 +1183,7 @@ void orc_matvec(double alpha, const orc_csr *A, const double *x, double beta,
                 const double *b, double *y)
 {
    int i, j;
+#pragma omp parallel for private(j) schedule(static)
    for (i = 0; i < A->nrows; i++)
    {
       double t = 0.0;
@@ -1210,12 +1399,14 @@ static void relax(orc_amg *H, int lev, int type, const double *f, double *u)
    }
    if (type == 18)
    {
+#pragma omp parallel for private(j) schedule(static)
       for (i = 0; i < n; i++)
       {
          double t = 0.0;
          for (j = A->ia[i]; j < A->ia[i + 1]; j++) { t += A->a[j] * u[A->ja[j]]; }
          v[i] = -w * t + w * f[i];
       }
+#pragma omp parallel for schedule(static)
       for (i = 0; i < n; i++) { u[i] += v[i] / H->l1[lev][i]; }
    }
    else   /* Jacobi */
@@ -1291,6 +1482,7 @@ int orc_amg_cycle(orc_amg *H, const double *f, double *u)
          const orc_csr *R = &H->R[lev];   /* F_c = P^T r via the stored transpose */
          int ic, jj;
          (void) n;
+#pragma omp parallel for private(jj) schedule(static)
          for (ic = 0; ic < R->nrows; ic++)
          {
             double t = 0.0;
@@ -1392,8 +1584,8 @@ int orc_pcg_solve(orc_amg *H, const double *b, double *x, double tol, int max_it
       alpha = gamma / sdotp;
       if (alpha <= 0.0) { err = 256; break; }
       gamma_old = gamma;
-      for (k = 0; k < n; k++) { x[k] += alpha * p[k]; }
-      for (k = 0; k < n; k++) { r[k] += -alpha * s[k]; }
+#pragma omp parallel for schedule(static)
+      for (k = 0; k < n; k++) { x[k] += alpha * p[k]; r[k] += -alpha * s[k]; }
       memset(s, 0, sizeof(double) * (size_t) n);
       orc_amg_cycle(H, r, s);
       gamma = orc_dot(n, r, s);
@@ -1402,8 +1594,8 @@ int orc_pcg_solve(orc_amg *H, const double *b, double *x, double tol, int max_it
       if (i_prod / bi_prod < eps) { break; }
       if (gamma <= 0.0) { err = 256; break; }
       beta = gamma / gamma_old;
-      for (k = 0; k < n; k++) { p[k] = beta * p[k]; }
-      for (k = 0; k < n; k++) { p[k] += s[k]; }
+#pragma omp parallel for schedule(static)
+      for (k = 0; k < n; k++) { p[k] = beta * p[k]; p[k] += s[k]; }
    }
    *iters = i;
    *rel_res = sqrt(i_prod / bi_prod);

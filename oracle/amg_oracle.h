@@ -1,8 +1,9 @@
 /*
  * CPU oracle for the BoomerAMG-PCG hot path (TEST INFRASTRUCTURE ONLY).
  *
- * A plain-C, serial restatement of the host code path that hypre 3.1.0 runs
- * for PMIS coarsening + extended+i interpolation + l1-Jacobi V-cycle + PCG.
+ * A plain-C restatement of the host code path that hypre 3.1.0 runs for
+ * PMIS coarsening + extended+i interpolation + l1-Jacobi V-cycle + PCG; its
+ * independent row loops run under OpenMP with the serial results.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it, and only as the checker.  The product library never calls it.
  *
@@ -29,6 +30,9 @@ typedef struct
 } orc_csr;
 
 void orc_csr_free(orc_csr *M);
+
+/* threads of the OpenMP loops (OMP_NUM_THREADS, else the host's cores) */
+int orc_num_threads(void);
 
 /* ---------------- Park-Miller RNG (src/utilities/random.c) -------------- */
 void   orc_seed_rand(int seed);                 /* random.c hypre_SeedRand */

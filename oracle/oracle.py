@@ -164,6 +164,11 @@ def _from_c(s: _Csr, free: bool = True, with_values: bool = True) -> Csr:
 
 
 # ---------------------------------------------------------------- generators
+def num_threads() -> int:
+    """Threads of the oracle's OpenMP row loops."""
+    return int(lib().orc_num_threads())
+
+
 def rand_stream(seed: int, n: int) -> np.ndarray:
     out = np.empty(n, dtype=np.float64)
     lib().orc_rand_stream(seed, n, _dp(out))
