@@ -78,6 +78,7 @@ static void orc_prefix(int *c, int n)
 
 #define C_PT  1
 #define F_PT -1
+#define Z_PT -2
 #define SF_PT -3
 
 static void *xmalloc(size_t n)
@@ -387,28 +388,64 @@ int orc_pmis_measure(const orc_csr *S, int seed, double *measure)
    return 0;
 }
 
-int orc_pmis_with_measure(const orc_csr *S, const double *measure_in, int *CF)
+/* CF_init 0: CF is set here; CF_init 1 (HMIS): CF holds the first pass's
+ * split on entry and its C points are the first independent set */
+static int pmis_core(const orc_csr *S, const double *measure_in, int *CF, int cf_init)
 {
    int n = S->nrows, i, j, jS, ig, cnt, graph_size, iter = 0;
    const int *Si = S->ia, *Sj = S->ja;
-   double *measure = (double *) xmalloc(sizeof(double) * (size_t) n);
-   int *graph = (int *) xmalloc(sizeof(int) * (size_t) n);
-   int *graph2 = (int *) xmalloc(sizeof(int) * (size_t) n);
+   double *measure = (double *) xmalloc(sizeof(double) * (size_t) (n > 0 ? n : 1));
+   int *graph = (int *) xmalloc(sizeof(int) * (size_t) (n > 0 ? n : 1));
+   int *graph2 = (int *) xmalloc(sizeof(int) * (size_t) (n > 0 ? n : 1));
    memcpy(measure, measure_in, sizeof(double) * (size_t) n);
 
-   /* par_coarsen.c:2386-2407 (CF_init == 0) */
    cnt = 0;
-   for (i = 0; i < n; i++)
+   if (cf_init == 1)
    {
-      CF[i] = 0;
-      if (Si[i + 1] - Si[i] == 0)
+      /* par_coarsen.c:2346-2374 (one rank: no offd entries) */
+      for (i = 0; i < n; i++)
       {
-         CF[i] = SF_PT;
-         measure[i] = 0;
+         if (CF[i] != SF_PT)
+         {
+            if (CF[i] == -1) { CF[i] = 0; }
+            if (CF[i] == Z_PT)
+            {
+               if (measure[i] >= 1.0 || Si[i + 1] - Si[i] > 0)
+               {
+                  CF[i] = 0;
+                  graph[cnt++] = i;
+               }
+               else
+               {
+                  CF[i] = F_PT;
+               }
+            }
+            else
+            {
+               graph[cnt++] = i;
+            }
+         }
+         else
+         {
+            measure[i] = 0;
+         }
       }
-      else
+   }
+   else
+   {
+      /* par_coarsen.c:2386-2407 (CF_init == 0) */
+      for (i = 0; i < n; i++)
       {
-         graph[cnt++] = i;
+         CF[i] = 0;
+         if (Si[i + 1] - Si[i] == 0)
+         {
+            CF[i] = SF_PT;
+            measure[i] = 0;
+         }
+         else
+         {
+            graph[cnt++] = i;
+         }
       }
    }
    graph_size = cnt;
@@ -416,6 +453,10 @@ int orc_pmis_with_measure(const orc_csr *S, const double *measure_in, int *CF)
    while (1)   /* par_coarsen.c:2460 THE INDEPENDENT SET COARSENING LOOP */
    {
       if (graph_size == 0) { break; }
+      /* par_coarsen.c:2487: with CF_init 1 the first independent set is the
+       * first pass's C points */
+      if (!cf_init || iter)
+      {
       /* par_coarsen.c:2485-2550 pick the independent set.  Every write of
        * the second loop stores 0, so the outcome does not depend on the
        * order of the rows. */
@@ -449,6 +490,7 @@ int orc_pmis_with_measure(const orc_csr *S, const double *measure_in, int *CF)
                }
             }
          }
+      }
       }
       iter++;
       /* par_coarsen.c:2603-2660 set C and F points.  Which points have
@@ -493,6 +535,269 @@ int orc_pmis_with_measure(const orc_csr *S, const double *measure_in, int *CF)
    }
    free(measure); free(graph); free(graph2);
    return iter;
+}
+
+int orc_pmis_with_measure(const orc_csr *S, const double *measure_in, int *CF)
+{
+   return pmis_core(S, measure_in, CF, 0);
+}
+
+/* ======================================================================
+ * HMIS (coarsen_type 10): src/parcsr_ls/par_coarsen.c:2842-2906
+ * hypre_BoomerAMGCoarsenHMIS = the Ruge-Stueben first pass
+ * (hypre_BoomerAMGCoarsenRuge, par_coarsen.c:911, called with coarsen_type
+ * 10: undecided points that no point depends on become Z_PT, :1119-1123,
+ * and the routine returns after the first pass, :1384-1391), then PMIS with
+ * CF_init 1 (:2904).  One rank, measure_type 0, cut_factor 0.
+ *
+ * The first pass keeps undecided points on lists of equal measure ordered
+ * by decreasing measure (src/utilities/amg_linklist.c): a point joins the
+ * tail of its measure's list, and the next C point is the head of the
+ * list of largest measure.  Restated here with that list of lists.
+ * ====================================================================== */
+#define LIST_HEAD_MARK -1
+#define LIST_TAIL_MARK -2
+
+typedef struct lol_elt
+{
+   int data, head, tail;
+   struct lol_elt *next, *prev;
+} lol_elt;
+
+typedef struct
+{
+   lol_elt *head, *tail;
+   int *lists, *where;      /* next / previous point of each point on its list */
+} lol_t;
+
+static lol_elt *lol_new(int measure, int index, lol_t *L)
+{
+   lol_elt *e = (lol_elt *) xcalloc(1, sizeof(lol_elt));
+   e->data = measure;
+   e->head = index;
+   e->tail = index;
+   L->lists[index] = LIST_TAIL_MARK;
+   L->where[index] = LIST_HEAD_MARK;
+   return e;
+}
+
+/* amg_linklist.c:168 hypre_enter_on_lists */
+static void lol_enter(lol_t *L, int measure, int index)
+{
+   lol_elt *p = L->head, *e;
+   if (!p)
+   {
+      e = lol_new(measure, index, L);
+      L->head = L->tail = e;
+      return;
+   }
+   for (; p; p = p->next)
+   {
+      if (measure > p->data)
+      {
+         /* a new list in front of p */
+         e = lol_new(measure, index, L);
+         e->next = p;
+         e->prev = p->prev;
+         if (p->prev) { p->prev->next = e; } else { L->head = e; }
+         p->prev = e;
+         return;
+      }
+      if (measure == p->data)
+      {
+         /* append at the tail of p */
+         L->lists[p->tail] = index;
+         L->where[index] = p->tail;
+         L->lists[index] = LIST_TAIL_MARK;
+         p->tail = index;
+         return;
+      }
+   }
+   /* smaller than every list: a new last list */
+   e = lol_new(measure, index, L);
+   e->prev = L->tail;
+   L->tail->next = e;
+   L->tail = e;
+}
+
+/* amg_linklist.c:40 hypre_remove_point */
+static void lol_remove(lol_t *L, int measure, int index)
+{
+   lol_elt *p;
+   for (p = L->head; p; p = p->next)
+   {
+      if (p->data != measure) { continue; }
+      if (p->head == index && p->tail == index)
+      {
+         /* the point is alone on its list: drop the list */
+         if (p->prev) { p->prev->next = p->next; } else { L->head = p->next; }
+         if (p->next) { p->next->prev = p->prev; } else { L->tail = p->prev; }
+         free(p);
+      }
+      else if (p->head == index)
+      {
+         p->head = L->lists[index];
+         L->where[L->lists[index]] = LIST_HEAD_MARK;
+      }
+      else if (p->tail == index)
+      {
+         p->tail = L->where[index];
+         L->lists[L->where[index]] = LIST_TAIL_MARK;
+      }
+      else
+      {
+         L->lists[L->where[index]] = L->lists[index];
+         L->where[L->lists[index]] = L->where[index];
+      }
+      return;
+   }
+   abort();   /* "No such list!" */
+}
+
+int orc_rs_first_pass(const orc_csr *S, int *CF)
+{
+   const int n = S->nrows;
+   const int *Si = S->ia, *Sj = S->ja;
+   const int nnz = Si[n];
+   int *STi = (int *) xcalloc((size_t) n + 1, sizeof(int));
+   int *STj = (int *) xmalloc(sizeof(int) * (size_t) (nnz > 0 ? nnz : 1));
+   int *measure = (int *) xmalloc(sizeof(int) * (size_t) (n > 0 ? n : 1));
+   lol_t L;
+   int i, j, k, num_left = 0;
+   L.head = L.tail = NULL;
+   L.lists = (int *) xcalloc((size_t) (n > 0 ? n : 1), sizeof(int));
+   L.where = (int *) xcalloc((size_t) (n > 0 ? n : 1), sizeof(int));
+
+   /* S^T by a stable counting sort (par_coarsen.c:1066-1095) */
+   for (i = 0; i < nnz; i++) { STi[Sj[i] + 1]++; }
+   for (i = 0; i < n; i++) { STi[i + 1] += STi[i]; }
+   {
+      int *pos = (int *) xmalloc(sizeof(int) * (size_t) (n > 0 ? n : 1));
+      memcpy(pos, STi, sizeof(int) * (size_t) n);
+      for (i = 0; i < n; i++)
+      {
+         for (j = Si[i]; j < Si[i + 1]; j++) { STj[pos[Sj[j]]++] = i; }
+      }
+      free(pos);
+   }
+   /* measures: the number of points that depend on i (:1107-1112) */
+   for (i = 0; i < n; i++) { measure[i] = STi[i + 1] - STi[i]; }
+
+   /* :1166-1190 */
+   for (j = 0; j < n; j++) { CF[j] = 0; }
+   for (j = 0; j < n; j++)
+   {
+      if (Si[j + 1] - Si[j] == 0)
+      {
+         CF[j] = SF_PT;
+         measure[j] = 0;
+      }
+      else
+      {
+         num_left++;
+      }
+   }
+   /* :1212-1256 lists, and points nobody depends on become Z_PT */
+   for (j = 0; j < n; j++)
+   {
+      const int m = measure[j];
+      if (CF[j] == SF_PT) { continue; }
+      if (m > 0)
+      {
+         lol_enter(&L, m, j);
+         continue;
+      }
+      CF[j] = Z_PT;
+      for (k = Si[j]; k < Si[j + 1]; k++)
+      {
+         const int nb = Sj[k];
+         if (CF[nb] == SF_PT) { continue; }
+         if (nb < j)
+         {
+            if (measure[nb] > 0) { lol_remove(&L, measure[nb], nb); }
+            lol_enter(&L, ++measure[nb], nb);
+         }
+         else
+         {
+            ++measure[nb];
+         }
+      }
+      --num_left;
+   }
+   /* :1278-1358 the first pass proper */
+   while (num_left > 0)
+   {
+      const int index = L.head->head;
+      const int m = measure[index];
+      CF[index] = C_PT;
+      measure[index] = 0;
+      --num_left;
+      lol_remove(&L, m, index);
+      for (j = STi[index]; j < STi[index + 1]; j++)
+      {
+         const int nb = STj[j];
+         if (CF[nb] != 0) { continue; }
+         CF[nb] = F_PT;
+         lol_remove(&L, measure[nb], nb);
+         --num_left;
+         for (k = Si[nb]; k < Si[nb + 1]; k++)
+         {
+            const int nb2 = Sj[k];
+            if (CF[nb2] != 0) { continue; }
+            lol_remove(&L, measure[nb2], nb2);
+            lol_enter(&L, ++measure[nb2], nb2);
+         }
+      }
+      for (j = Si[index]; j < Si[index + 1]; j++)
+      {
+         const int nb = Sj[j];
+         int mm;
+         if (CF[nb] != 0) { continue; }
+         mm = measure[nb];
+         lol_remove(&L, mm, nb);
+         measure[nb] = --mm;
+         if (mm > 0)
+         {
+            lol_enter(&L, mm, nb);
+         }
+         else
+         {
+            CF[nb] = F_PT;
+            --num_left;
+            for (k = Si[nb]; k < Si[nb + 1]; k++)
+            {
+               const int nb2 = Sj[k];
+               if (CF[nb2] != 0) { continue; }
+               lol_remove(&L, measure[nb2], nb2);
+               lol_enter(&L, ++measure[nb2], nb2);
+            }
+         }
+      }
+   }
+   while (L.head)
+   {
+      lol_elt *nx = L.head->next;
+      free(L.head);
+      L.head = nx;
+   }
+   free(STi); free(STj); free(measure); free(L.lists); free(L.where);
+   return 0;
+}
+
+int orc_hmis_with_measure(const orc_csr *S, const double *measure_in, int *CF)
+{
+   orc_rs_first_pass(S, CF);
+   return pmis_core(S, measure_in, CF, 1);
+}
+
+int orc_hmis(const orc_csr *S, int *CF)
+{
+   double *m = (double *) xmalloc(sizeof(double) * (size_t) (S->nrows > 0 ? S->nrows : 1));
+   int it;
+   orc_pmis_measure(S, 2747, m);
+   it = orc_hmis_with_measure(S, m, CF);
+   free(m);
+   return it;
 }
 
 int orc_pmis(const orc_csr *S, int *CF)
@@ -1245,6 +1550,7 @@ void orc_amg_default_params(orc_amg_params *p)
    p->outer_weight = 1.0;
    p->hgs_parts = 0;
    p->hgs_rows = 4;
+   p->coarsen_type = 8;
 }
 
 static void csr_copy(const orc_csr *A, orc_csr *B)
@@ -1276,7 +1582,8 @@ orc_amg *orc_amg_setup(const orc_csr *A0, const orc_amg_params *prm)
       if (prm->max_levels == 1) { break; }
       orc_strength(A, prm->strong_threshold, prm->max_row_sum, &H->S[level]);
       H->CF[level] = (int *) xmalloc(sizeof(int) * (size_t) fine_size);
-      H->rounds[level] = orc_pmis(&H->S[level], H->CF[level]);
+      H->rounds[level] = (prm->coarsen_type == 10) ? orc_hmis(&H->S[level], H->CF[level])
+                                                   : orc_pmis(&H->S[level], H->CF[level]);
       for (i = 0; i < fine_size; i++) { if (H->CF[level][i] == 1) { coarse_size++; } }
       /* par_amg_setup.c:1690-1725: no coarse grid -> stop, coarsest solve is
        * one sweep of the down smoother */

@@ -72,6 +72,14 @@ int orc_pmis_measure(const orc_csr *S, int seed, double *measure);
 int orc_pmis_with_measure(const orc_csr *S, const double *measure_in, int *CF);
 /* Full PMIS with the hypre seed (2747 on rank 0, CF_init 0) */
 int orc_pmis(const orc_csr *S, int *CF);
+/* Ruge-Stueben first pass as HMIS runs it (par_coarsen.c:911 with
+ * coarsen_type 10): CF out 1 = C, -1 = F, -2 = Z (nobody depends on it),
+ * -3 = SF (no strong connections) */
+int orc_rs_first_pass(const orc_csr *S, int *CF);
+/* HMIS (par_coarsen.c:2842): the first pass, then PMIS with CF_init 1 for a
+ * given measure / the hypre seed.  Returns the PMIS rounds. */
+int orc_hmis_with_measure(const orc_csr *S, const double *measure_in, int *CF);
+int orc_hmis(const orc_csr *S, int *CF);
 /* src/parcsr_ls/par_lr_interp.c:1024 hypre_BoomerAMGBuildExtPIInterpHost,
  * followed by par_interp.c:2622 truncation when max_elmts > 0 or trunc > 0.
  * Converts CF -3 -> -1 at the end, as the reference does. */
@@ -130,6 +138,7 @@ typedef struct
    double outer_weight;       /* 1.0 (hypre's omega) */
    int    hgs_parts;          /* hybrid GS partitions; 0: about hgs_rows rows each */
    int    hgs_rows;           /* 4 */
+   int    coarsen_type;       /* 8 PMIS, 10 HMIS */
 } orc_amg_params;
 
 void orc_amg_default_params(orc_amg_params *p);

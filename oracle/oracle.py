@@ -31,7 +31,7 @@ class _Params(C.Structure):
                 ("relax_type", C.c_int), ("coarse_relax_type", C.c_int),
                 ("relax_weight", C.c_double), ("num_sweeps", C.c_int),
                 ("relax_up_type", C.c_int), ("outer_weight", C.c_double),
-                ("hgs_parts", C.c_int), ("hgs_rows", C.c_int)]
+                ("hgs_parts", C.c_int), ("hgs_rows", C.c_int), ("coarsen_type", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -62,6 +62,9 @@ def lib():
         L.orc_pmis_measure.argtypes = [P(_Csr), C.c_int, P(C.c_double)]
         L.orc_pmis_with_measure.argtypes = [P(_Csr), P(C.c_double), P(C.c_int)]
         L.orc_pmis.argtypes = [P(_Csr), P(C.c_int)]
+        L.orc_rs_first_pass.argtypes = [P(_Csr), P(C.c_int)]
+        L.orc_hmis_with_measure.argtypes = [P(_Csr), P(C.c_double), P(C.c_int)]
+        L.orc_hmis.argtypes = [P(_Csr), P(C.c_int)]
         L.orc_extpi_interp.argtypes = [P(_Csr), P(_Csr), P(C.c_int), C.c_double, C.c_int,
                                        P(_Csr)]
         L.orc_extpi_interp_raw.argtypes = [P(_Csr), P(_Csr), P(C.c_int), P(_Csr)]
@@ -228,6 +231,26 @@ def pmis(S: Csr, measure: np.ndarray | None = None):
     else:
         m = np.ascontiguousarray(measure, dtype=np.float64)
         r = lib().orc_pmis_with_measure(C.byref(cs), _dp(m), _ip(cf))
+    return cf, r
+
+
+def rs_first_pass(S: Csr) -> np.ndarray:
+    """The Ruge-Stueben first pass of HMIS: 1 = C, -1 = F, -2 = Z, -3 = SF."""
+    cf = np.empty(S.nrows, dtype=np.int32)
+    cs = S.to_c()
+    lib().orc_rs_first_pass(C.byref(cs), _ip(cf))
+    return cf
+
+
+def hmis(S: Csr, measure: np.ndarray | None = None):
+    """HMIS (first pass + PMIS with CF_init 1). Returns (CF, PMIS rounds)."""
+    cf = np.empty(S.nrows, dtype=np.int32)
+    cs = S.to_c()
+    if measure is None:
+        r = lib().orc_hmis(C.byref(cs), _ip(cf))
+    else:
+        m = np.ascontiguousarray(measure, dtype=np.float64)
+        r = lib().orc_hmis_with_measure(C.byref(cs), _dp(m), _ip(cf))
     return cf, r
 
 
