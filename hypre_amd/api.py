@@ -555,6 +555,25 @@ def pmis(S: HostCsr, measure: np.ndarray | None = None):
     return cf, rounds.value
 
 
+def hmis(S: HostCsr, measure: np.ndarray | None = None):
+    n = S.nrows
+    cf = np.empty(n, np.int32)
+    rounds = C.c_int()
+    m = None if measure is None else np.ascontiguousarray(measure, dtype=np.float64)
+    ja = S.ja if len(S.ja) else np.zeros(1, np.int32)
+    call("HYPRE_MI355X_HMIS", n, _ip(S.ia), _ip(ja), None if m is None else _dp(m), _ip(cf),
+         C.byref(rounds))
+    return cf, rounds.value
+
+
+def rs_first_pass(S: HostCsr) -> np.ndarray:
+    n = S.nrows
+    cf = np.empty(n, np.int32)
+    ja = S.ja if len(S.ja) else np.zeros(1, np.int32)
+    call("HYPRE_MI355X_RSFirstPass", n, _ip(S.ia), _ip(ja), _ip(cf))
+    return cf
+
+
 def extpi_interp(A: HostCsr, S: HostCsr, cf: np.ndarray, trunc_factor=0.0, max_elmts=4):
     n = A.nrows
     cf = np.ascontiguousarray(cf, dtype=np.int32).copy()

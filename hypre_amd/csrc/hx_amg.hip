@@ -87,7 +87,7 @@ static void make_fp32_view(const Csr &src, Csr &dst)
 }
 
 // src/parcsr_ls/par_amg_setup.c:1002-3185 (the level loop) for coarsen_type
-// 8/9, interp_type 6, relax 18/0 with Gaussian elimination on the coarsest
+// 8/9/10, interp_type 6, relax 18/0 with Gaussian elimination on the coarsest
 // grid.
 // device word that the strength pass of `level` sets on a zero l1 norm
 static int *l1_zero_flag(int level) { return device_flags() + 768 + std::min(level, 255); }
@@ -121,7 +121,8 @@ void AMGSolver::setup(const Csr &A0)
       create_s(A, theta(level), p.max_row_sum, lv.S, stc, lv.l1, l1_zero_flag(level), s_thr, s_flg);
       lv.CF = dalloc_n<int>((size_t) fine_size);
       if (p.keep_setup_data) { lv.measure = dalloc_n<double>((size_t) fine_size); }
-      lv.pmis_rounds = pmis(lv.S, nullptr, lv.CF, lv.measure, stc);
+      lv.pmis_rounds = (p.coarsen_type == 10) ? hmis(lv.S, nullptr, lv.CF, lv.measure, stc)
+                                              : pmis(lv.S, nullptr, lv.CF, lv.measure, stc);
       dfree(stc);
       int *f2c = dalloc_n<int>((size_t) fine_size + 1);
       const int coarse_size = coarse_map(fine_size, lv.CF, f2c);
@@ -687,7 +688,10 @@ HYPRE_Int HYPRE_BoomerAMGSetInterpType(HYPRE_Solver solver, HYPRE_Int v)
 HYPRE_Int HYPRE_BoomerAMGSetCoarsenType(HYPRE_Solver solver, HYPRE_Int v)
 {
    return guard([&] {
-      if (v != 8 && v != 9) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "only PMIS coarsening (8, 9) is supported"); }
+      if (v != 8 && v != 9 && v != 10) {
+         g_error_arg = 2;
+         fail(HYPRE_ERROR_ARG, "only PMIS (8, 9) and HMIS (10) coarsening are supported");
+      }
       as_amg(solver)->prm.coarsen_type = v;
    });
 }

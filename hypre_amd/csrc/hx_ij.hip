@@ -1430,6 +1430,40 @@ HYPRE_Int HYPRE_MI355X_PMIS(HYPRE_Int n, const HYPRE_Int *S_ia, const HYPRE_Int 
    });
 }
 
+HYPRE_Int HYPRE_MI355X_HMIS(HYPRE_Int n, const HYPRE_Int *S_ia, const HYPRE_Int *S_ja,
+                            const HYPRE_Real *measure, HYPRE_Int *cf, HYPRE_Int *rounds)
+{
+   return guard([&] {
+      ensure_init();
+      Csr S;
+      csr_from_host(S, n, n, S_ia, S_ja, nullptr);
+      double *m = nullptr;
+      if (measure) {
+         m = dalloc_n<double>((size_t) n);
+         h2d(m, measure, sizeof(double) * (size_t) n);
+      }
+      int *dcf = dalloc_n<int>((size_t) n);
+      int r = hmis(S, m, dcf, nullptr);
+      d2h(cf, dcf, sizeof(int) * (size_t) n);
+      if (rounds) { *rounds = r; }
+      dfree(m);
+      dfree(dcf);
+      S.free_all();
+   });
+}
+
+HYPRE_Int HYPRE_MI355X_RSFirstPass(HYPRE_Int n, const HYPRE_Int *S_ia, const HYPRE_Int *S_ja,
+                                   HYPRE_Int *cf)
+{
+   return guard([&] {
+      if (n < 0 || !S_ia || !cf || (n > 0 && S_ia[n] > 0 && !S_ja)) {
+         g_error_arg = 1;
+         fail(HYPRE_ERROR_ARG, "bad strength pattern");
+      }
+      rs_first_pass_host(n, S_ia, S_ja, cf);
+   });
+}
+
 HYPRE_Int HYPRE_MI355X_ExtPIInterp(HYPRE_Int n, const HYPRE_Int *ia, const HYPRE_Int *ja, const HYPRE_Real *a,
                                    const HYPRE_Int *S_ia, const HYPRE_Int *S_ja, HYPRE_Int *cf,
                                    HYPRE_Real trunc_factor, HYPRE_Int max_elmts, HYPRE_Int *P_ncols,

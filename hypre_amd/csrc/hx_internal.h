@@ -210,8 +210,15 @@ void csr_reverse_rows(const Csr &A, Csr &R);
 // PMIS: measure (optional input: given measure on device, else Park-Miller
 // stream seed 2747), CF out; returns number of rounds
 // st_count: the column counts of S when already known (else counted here)
+// cf_init 1 (HMIS): CF holds the Ruge-Stueben first pass on entry
 int  pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_keep_dev,
+          const int *st_count = nullptr, int cf_init = 0);
+// HMIS (coarsen_type 10): the Ruge-Stueben first pass on the host, then PMIS
+// with CF_init 1 on the device; returns the PMIS rounds
+int  hmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_keep_dev,
           const int *st_count = nullptr);
+// the first pass alone (host arrays; CF: 1 C, -1 F, -2 Z, -3 SF)
+void rs_first_pass_host(int n, const int *S_ia, const int *S_ja, int *CF);
 // fine->coarse map; returns n_coarse
 int  coarse_map(int n, const int *CF, int *f2c);
 // ext+i interpolation + truncation (CF: -3 -> -1 on exit)

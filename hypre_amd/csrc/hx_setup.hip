@@ -906,6 +906,37 @@ __global__ void k_pmis_init(int n, const int *__restrict__ S_ia, const int *__re
    }
 }
 
+// CF_init 1 (HMIS, par_coarsen.c:2346-2374): CF holds the Ruge-Stueben first
+// pass's split (1 C, -1 F, -2 Z, -3 SF); F points become undecided, Z points
+// stay in the graph only when their measure is >= 1 or they depend on
+// someone, and no candidate flags are set (the first pass's C points are the
+// first independent set).  Rows outside the graph get measure 0.
+__global__ void k_pmis_init_cf1(int n, const int *__restrict__ S_ia, const int *__restrict__ cnt,
+                                const double *__restrict__ given, int seed,
+                                double *__restrict__ m, int *__restrict__ CF)
+{
+   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+      const double mi = given ? given[i] : (double) cnt[i] + pm_draw(seed, i);
+      int c = CF[i];
+      double mo = mi;
+      if (c == -3) {
+         mo = 0.0;
+      } else {
+         if (c == -1) { c = 0; }
+         if (c == -2) {
+            if (mi >= 1.0 || S_ia[i + 1] > S_ia[i]) {
+               c = 0;
+            } else {
+               c = -1;
+               mo = 0.0;
+            }
+         }
+      }
+      m[i] = mo;
+      CF[i] = c;
+   }
+}
+
 constexpr int PM_U = 4;   // strided steps of a row whose loads are issued together
 
 // par_coarsen.c:2506-2550: drop candidates with a larger-measure S-neighbour
@@ -1052,7 +1083,7 @@ __global__ __launch_bounds__(256) void k_pmis_update(int nw, const int *__restri
 }
 
 int pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_keep_dev,
-         const int *st_count)
+         const int *st_count, int cf_init)
 {
    int n = S.nrows;
    if (n == 0) { return 0; }
@@ -1070,7 +1101,11 @@ int pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_ke
       HX_LAUNCH_CHECK();
       stc = cnt;
    }
-   k_pmis_init<<<nb, 256, 0, st>>>(n, S.ia, stc, measure_in_dev, 2747, m, CF);
+   if (cf_init == 1) {
+      k_pmis_init_cf1<<<nb, 256, 0, st>>>(n, S.ia, stc, measure_in_dev, 2747, m, CF);
+   } else {
+      k_pmis_init<<<nb, 256, 0, st>>>(n, S.ia, stc, measure_in_dev, 2747, m, CF);
+   }
    HX_LAUNCH_CHECK();
    if (measure_keep_dev) { d2d(measure_keep_dev, m, sizeof(double) * n); }
    int *remaining = device_flags() + 44;
@@ -1085,8 +1120,11 @@ int pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_ke
       dzero(remaining, sizeof(int));
       const int nbk = (int) std::max<long long>(1, std::min<long long>(((long long) nw * K + 255) / 256, 16384));
       const int nbu = std::max(1, std::min((nw + 255) / 256, 8192));
+      // par_coarsen.c:2487: with CF_init 1 the first round keeps the given
+      // C points as its independent set
+      const bool pick = !(cf_init == 1 && rounds == 0);
       dispatch_k(K, [&](auto kc) {
-         k_pmis_remove<decltype(kc)::value><<<nbk, 256, 0, st>>>(nw, list, S.ia, S.ja, m, CF);
+         if (pick) { k_pmis_remove<decltype(kc)::value><<<nbk, 256, 0, st>>>(nw, list, S.ia, S.ja, m, CF); }
          k_pmis_cf<decltype(kc)::value><<<nbk, 256, 0, st>>>(nw, list, S.ia, S.ja, m, CF);
       });
       int *next = lists[rounds & 1];

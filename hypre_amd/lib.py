@@ -129,6 +129,8 @@ SIGNATURES = {
     "HYPRE_MI355X_PCGGetTimes": (I, [P, DP, DP]),
     "HYPRE_MI355X_CreateS": (I, [I, IP, IP, DP, D, D, IP, IP]),
     "HYPRE_MI355X_PMIS": (I, [I, IP, IP, DP, IP, IP]),
+    "HYPRE_MI355X_HMIS": (I, [I, IP, IP, DP, IP, IP]),
+    "HYPRE_MI355X_RSFirstPass": (I, [I, IP, IP, IP]),
     "HYPRE_MI355X_ExtPIInterp": (I, [I, IP, IP, DP, IP, IP, IP, D, I, IP, IP, IP, IP, DP]),
     "HYPRE_MI355X_RAP": (I, [I, I, IP, IP, DP, IP, IP, DP, IP, IP, IP, DP]),
     "HYPRE_MI355X_SpMV": (I, [I, I, IP, IP, DP, D, DP, D, DP]),

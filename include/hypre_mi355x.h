@@ -90,6 +90,15 @@ HYPRE_Int HYPRE_MI355X_CreateS(HYPRE_Int n, const HYPRE_Int *ia, const HYPRE_Int
  * Park-Miller stream (seed 2747), else the given measure is used */
 HYPRE_Int HYPRE_MI355X_PMIS(HYPRE_Int n, const HYPRE_Int *S_ia, const HYPRE_Int *S_ja,
                             const HYPRE_Real *measure, HYPRE_Int *cf, HYPRE_Int *rounds);
+/* HMIS (coarsen_type 10, src/parcsr_ls/par_coarsen.c:2842): the
+ * Ruge-Stueben first pass on the host, then PMIS with CF_init 1 on the device;
+ * measure == NULL draws the reference's Park-Miller stream (seed 2747) */
+HYPRE_Int HYPRE_MI355X_HMIS(HYPRE_Int n, const HYPRE_Int *S_ia, const HYPRE_Int *S_ja,
+                            const HYPRE_Real *measure, HYPRE_Int *cf, HYPRE_Int *rounds);
+/* the first pass alone (par_coarsen.c:911 with coarsen_type 10, host only):
+ * cf 1 = C, -1 = F, -2 = Z (nobody depends on it), -3 = no strong connections */
+HYPRE_Int HYPRE_MI355X_RSFirstPass(HYPRE_Int n, const HYPRE_Int *S_ia, const HYPRE_Int *S_ja,
+                                   HYPRE_Int *cf);
 /* ext+i interpolation + truncation; P arrays sized by a first call with
  * P_ia == NULL that returns *P_nnz; cf is updated (-3 -> -1) */
 HYPRE_Int HYPRE_MI355X_ExtPIInterp(HYPRE_Int n, const HYPRE_Int *ia, const HYPRE_Int *ja,

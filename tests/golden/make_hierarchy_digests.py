@@ -3,8 +3,8 @@ to commit as arrays, as per-level SHA-256 digests (hierarchy_digest.py).
 
     python tests/golden/make_hierarchy_digests.py          # about 3 minutes
 
-Settings are the benchmark's (PMIS with the Park-Miller measure stream seed
-2747, ext+i with P_max_elmts 4, theta 0.25, max_row_sum 0.9, l1-Jacobi norms,
+Settings are the benchmark's (PMIS, or HMIS for the *_hmis cases, with the
+Park-Miller measure stream seed 2747, ext+i with P_max_elmts 4, theta 0.25, max_row_sum 0.9, l1-Jacobi norms,
 max_coarse_size 9).  The oracle (oracle/amg_oracle.c) is pinned by the
 reference's recorded runs; these digests carry its exact arrays to the GPU
 parity test tests/test_gpu_full_size_digests.py.
@@ -27,7 +27,11 @@ CASES = {
     "jump27_96": lambda: O.gen_vardiff27(96, 16, -4.0, 4.0, 1),
     "aniso7_96": lambda: O.gen_aniso7(96, 0.01, 0.6),
     "lap27_200": lambda: O.gen_laplace27(200, 200, 200),
+    "lap27_64_hmis": lambda: O.gen_laplace27(64, 64, 64),
+    "jump27_64_hmis": lambda: O.gen_vardiff27(64, 16, -4.0, 4.0, 1),
 }
+# settings beyond the benchmark's, per case
+PARAMS = {"lap27_64_hmis": {"coarsen_type": 10}, "jump27_64_hmis": {"coarsen_type": 10}}
 
 
 def main(names):
@@ -39,12 +43,14 @@ def main(names):
     for name in names:
         t0 = time.time()
         A = CASES[name]()
-        H = O.Hierarchy(A, max_row_sum=0.9)
+        extra = PARAMS.get(name, {})
+        H = O.Hierarchy(A, max_row_sum=0.9, **extra)
 
         def get(kind, lev):
             return {"A": H.A, "P": H.P, "R": H.R, "S": H.S, "CF": H.CF, "l1": H.l1}[kind](lev)
 
-        res[name] = {"settings": "PMIS, ext+i Pmx 4, theta 0.25, max_row_sum 0.9, l1 norms, "
+        coarsen = "HMIS" if extra.get("coarsen_type") == 10 else "PMIS"
+        res[name] = {"settings": f"{coarsen}, ext+i Pmx 4, theta 0.25, max_row_sum 0.9, l1 norms, "
                                  "max_coarse_size 9",
                      "levels": hierarchy_digests(H.num_levels, get)}
         print(f"{name}: {H.num_levels} levels, {time.time() - t0:.1f} s", flush=True)

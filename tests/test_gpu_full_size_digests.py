@@ -28,7 +28,10 @@ MAKE = {
     "jump27_96": lambda hx: hx.ParCSRMatrix.var_diffusion(96, 27, 16, -4.0, 4.0, 1),
     "aniso7_96": lambda hx: hx.ParCSRMatrix.aniso7(96, 0.01, 0.6),
     "lap27_200": lambda hx: hx.ParCSRMatrix.laplacian27(200, 200, 200),
+    "lap27_64_hmis": lambda hx: hx.ParCSRMatrix.laplacian27(64, 64, 64),
+    "jump27_64_hmis": lambda hx: hx.ParCSRMatrix.var_diffusion(64, 27, 16, -4.0, 4.0, 1),
 }
+COARSEN = {"lap27_64_hmis": 10, "jump27_64_hmis": 10}
 
 
 def first_difference(got, ref):
@@ -43,7 +46,7 @@ def first_difference(got, ref):
 def test_hierarchy_digests_match_oracle(hx, name):
     print(f"[gpu-digests] {name}", flush=True)
     A = MAKE[name](hx)
-    amg = hx.BoomerAMG(max_row_sum=0.9, keep_setup_data=1)
+    amg = hx.BoomerAMG(max_row_sum=0.9, keep_setup_data=1, coarsen_type=COARSEN.get(name, 8))
     amg.setup(A)
     nl = amg.num_levels()
 

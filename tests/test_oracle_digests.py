@@ -19,17 +19,17 @@ SCRIPT = r'''
 import json, os, sys
 sys.path.insert(0, {root!r}); sys.path.insert(0, os.path.join({root!r}, "tests", "golden"))
 from hierarchy_digest import hierarchy_digests
-from make_hierarchy_digests import CASES
+from make_hierarchy_digests import CASES, PARAMS
 from oracle import oracle as O
 A = CASES[{name!r}]()
-H = O.Hierarchy(A, max_row_sum=0.9)
+H = O.Hierarchy(A, max_row_sum=0.9, **PARAMS.get({name!r}, {{}}))
 get = lambda kind, lev: {{"A": H.A, "P": H.P, "R": H.R, "S": H.S, "CF": H.CF, "l1": H.l1}}[kind](lev)
 print(json.dumps(hierarchy_digests(H.num_levels, get)))
 '''
 
 
 @pytest.mark.parametrize("name,threads", [("lap27_64", 1), ("lap27_64", 5), ("jump27_96", 8),
-                                          ("aniso7_96", 3)])
+                                          ("aniso7_96", 3), ("jump27_64_hmis", 4)])
 def test_oracle_matches_committed_digests(oracle, name, threads):
     with open(os.path.join(HERE, "golden", "hierarchy_digests.json")) as f:
         ref = json.load(f)[name]["levels"]

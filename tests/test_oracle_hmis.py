@@ -184,3 +184,21 @@ def test_hmis_invariants(oracle, gen):
         if cf[i] == F and rows[i]:
             assert any(cf[j] == C for j in rows[i])
     assert rounds >= 1
+
+
+@pytest.mark.parametrize("gen", ["l7", "l27", "aniso", "jump", "rand"])
+def test_library_first_pass_matches_oracle(oracle, gen):
+    """The library's host first pass (bucket FIFOs, no GPU involved) gives the
+    oracle's split (the reference's list of lists) bit for bit."""
+    from hypre_amd import api as hx
+    if gen == "rand":
+        S = random_strength(oracle, 300, 0.03, 99, symmetric=False)
+    else:
+        A = {"l7": lambda: oracle.gen_laplace7(14, 13, 12),
+             "l27": lambda: oracle.gen_laplace27(12, 12, 12),
+             "aniso": lambda: oracle.gen_aniso7(14, 0.02, 0.4),
+             "jump": lambda: oracle.gen_vardiff27(13, 3, -3.0, 3.0, 4)}[gen]()
+        S = oracle.strength(A)
+    hs = hx.HostCsr(S.nrows, S.ncols, np.ascontiguousarray(S.ia, np.int32),
+                    np.ascontiguousarray(S.ja, np.int32), None)
+    assert np.array_equal(hx.rs_first_pass(hs), oracle.rs_first_pass(S))
