@@ -211,7 +211,12 @@ static void backward(hx_param_model_struct *m, int B)
 // Block partial sums (rows in order per thread, fixed tree), summed on the host.
 constexpr int NQ = 8;
 
+// Per-row sums of the features (one thread per row, block partials).  The
+// coefficient jump of row i is tested against its neighbours' diagonals by
+// products (|a_jj| >= 10 |a_ii| or |a_ii| >= 10 |a_jj|), which every IEEE
+// implementation rounds alike, instead of comparing logarithms.
 __global__ __launch_bounds__(256) void k_row_features(int n, const int *__restrict__ ia,
+                                                      const int *__restrict__ ja,
                                                       const double *__restrict__ a,
                                                       double *__restrict__ part)
 {
@@ -221,13 +226,17 @@ __global__ __launch_bounds__(256) void k_row_features(int n, const int *__restri
       const int s = ia[i], e = ia[i + 1];
       const double d = a[s];
       const double ad = fabs(d) > 0 ? fabs(d) : 1.0;
-      double mx = 0.0, mx2 = 0.0, mn = 1e300, sum = d, scale = 0.0;
+      double mx = 0.0, mx2 = 0.0, mn = 1e300, scale = 0.0;
+      bool j1 = false, j3 = false;
       for (int k = s + 1; k < e; k++) {
          const double v = a[k], av = fabs(v);
-         sum += v;
          if (av > mx) { mx2 = mx; mx = av; } else if (av > mx2) { mx2 = av; }
          mn = av < mn ? av : mn;
          scale = (d < 0) ? ((scale < v) ? v : scale) : ((scale < v) ? scale : v);
+         const double dj = fabs(a[ia[ja[k]]]);
+         const double aj = dj > 0 ? dj : 1.0;
+         j1 = j1 || aj >= 10.0 * ad || ad >= 10.0 * aj;
+         j3 = j3 || aj >= 1000.0 * ad || ad >= 1000.0 * aj;
       }
       int strong = 0;
       const double thr = 0.25 * scale;
@@ -236,15 +245,14 @@ __global__ __launch_bounds__(256) void k_row_features(int n, const int *__restri
          strong += (d < 0) ? !(v <= thr) : !(v >= thr);
       }
       const int no = e - s - 1;
-      const double ld = log10(ad);
-      q[0] += mx / ad;
-      q[1] += (no > 0 && mx > 0) ? mn / mx : 0.0;
-      q[2] += (no > 1 && mx > 0) ? mx2 / mx : 0.0;
-      q[3] += no > 0 ? (double) strong / no : 0.0;
-      q[4] += fabs(sum) / ad;
-      q[5] += ld;
-      q[6] += ld * ld;
-      q[7] += (double) (e - s);
+      const double len = (double) (e - s);
+      q[0] += len;
+      q[1] += len * len;
+      q[2] += (no > 0 && mx > 0) ? mn / mx : 0.0;
+      q[3] += (no > 1 && mx > 0) ? mx2 / mx : 0.0;
+      q[4] += no > 0 ? (double) strong / no : 0.0;
+      q[5] += j1 ? 1.0 : 0.0;
+      q[6] += j3 ? 1.0 : 0.0;
    }
    for (int c = 0; c < NQ; c++) { sh[c][threadIdx.x] = q[c]; }
    __syncthreads();
@@ -257,12 +265,16 @@ __global__ __launch_bounds__(256) void k_row_features(int n, const int *__restri
    if (threadIdx.x < NQ) { part[(size_t) blockIdx.x * NQ + threadIdx.x] = sh[threadIdx.x][0]; }
 }
 
+// f = [log10(n)/8, mean row length/27, std(row length)/27, mean min/max
+// |a_ij|, mean 2nd/1st largest |a_ij| (off-diagonal anisotropy), mean strong
+// fraction at theta 0.25, fraction of rows with a coefficient jump of at
+// least one decade to a neighbour, the same for three decades]
 void matrix_features(const Csr &A, double *f)
 {
    const int n = A.nrows;
    const int nb = std::max(1, std::min((n + 255) / 256, 1024));
    double *part = dalloc_n<double>((size_t) nb * NQ);
-   k_row_features<<<nb, 256, 0, stream()>>>(n, A.ia, A.a, part);
+   k_row_features<<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, part);
    HX_LAUNCH_CHECK();
    std::vector<double> h((size_t) nb * NQ);
    d2h(h.data(), part, sizeof(double) * h.size());
@@ -272,16 +284,16 @@ void matrix_features(const Csr &A, double *f)
       for (int c = 0; c < NQ; c++) { q[c] += h[(size_t) b * NQ + c]; }
    }
    const double inv = n > 0 ? 1.0 / n : 0.0;
-   const double mean_ld = q[5] * inv;
-   const double var_ld = std::max(0.0, q[6] * inv - mean_ld * mean_ld);
+   const double mean_len = q[0] * inv;
+   const double var_len = std::max(0.0, q[1] * inv - mean_len * mean_len);
    f[0] = n > 0 ? log10((double) n) / 8.0 : 0.0;
-   f[1] = q[7] * inv / 27.0;
-   f[2] = q[0] * inv;
-   f[3] = q[1] * inv;
-   f[4] = q[2] * inv;
-   f[5] = q[3] * inv;
-   f[6] = q[4] * inv;
-   f[7] = std::sqrt(var_ld) / 4.0;
+   f[1] = mean_len / 27.0;
+   f[2] = std::sqrt(var_len) / 27.0;
+   f[3] = q[2] * inv;
+   f[4] = q[3] * inv;
+   f[5] = q[4] * inv;
+   f[6] = q[5] * inv;
+   f[7] = q[6] * inv;
 }
 
 } // namespace hx

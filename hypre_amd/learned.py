@@ -3,15 +3,18 @@
 A small perceptron on the device (8 matrix features -> 64 -> 64 -> 3 x 25
 outputs, FP32, see hypre_amd/csrc/hx_mlp.hip) predicts, for every level, the
 strong threshold theta, the l1-Jacobi weight omega and the number of sweeps.
-It is trained on sweeps of plain settings: for each training matrix every
-(theta, omega, sweeps) of a small grid is run through AMG-PCG and the setting
-with the least work to 1e-8 becomes the target.
+The features (HYPRE_MI355X_MatrixFeatures) are the row-length distribution,
+off-diagonal anisotropy ratios, the strong fraction and a coefficient-jump
+histogram.
 
-Work of one solve (deterministic, in fine-matrix nonzero passes):
-    cycle = sum_l [ (2 s_l + 1) nnz(A_l) + 2 nnz(P_l) ] / nnz(A_0)
-    work  = iterations * (1 + cycle) + setup_weight * operator_complexity
-The guard `apply_with_fallback` keeps the defaults whenever the predicted
-settings need more work per digit of residual reduction on the matrix at hand.
+Training data come from sweeps of the plain method itself: for each training
+matrix, AMG-PCG is timed (setup + solve to 1e-8, best of two runs) for every
+setting of a small grid on the fine level with the coarse levels at the
+defaults, then for every setting on the coarse levels with the fine level at
+its best; the fastest pair is the target (level 0 from the first, levels 1-24
+from the second).  The guard `apply_with_fallback` keeps the defaults whenever
+the predicted settings reach the tolerance more slowly (less convergence per
+second) on the matrix at hand.
 """
 from __future__ import annotations
 
@@ -29,13 +32,11 @@ from .lib import lib
 
 NIN, NH1, NH2, LEVELS = 8, 64, 64, 25
 NOUT = 3 * LEVELS
-SETUP_WEIGHT = 4.0
 WEIGHTS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned_params.json")
 DEFAULT = (0.25, 1.0, 1)
 
-THETAS = (0.25, 0.4, 0.55, 0.7)
-OMEGAS = (0.8, 1.0, 1.2)
-SWEEPS = (1, 2)
+# settings of one level group (the fine level, or all coarse levels)
+GRID = tuple((th, om, sw) for th in (0.25, 0.4, 0.55) for om in (0.8, 1.0, 1.2) for sw in (1, 2))
 
 
 def _fp(a: np.ndarray):
@@ -130,12 +131,18 @@ def matrix_features(A: hx.ParCSRMatrix) -> np.ndarray:
     return f
 
 
-def encode(theta, omega, sweeps, levels=LEVELS) -> np.ndarray:
-    """Targets in sigmoid space for one setting used on every level."""
-    t = np.empty(3 * levels, np.float32)
-    t[0::3] = (theta - 0.1) / 0.8
-    t[1::3] = (omega - 0.6) / 0.8
-    t[2::3] = (sweeps - 1) / 2.0
+def per_level(fine, coarse, levels=LEVELS):
+    """Per-level settings: `fine` on level 0, `coarse` on levels 1.."""
+    return [tuple(fine)] + [tuple(coarse)] * (levels - 1)
+
+
+def encode(level_params) -> np.ndarray:
+    """Targets in sigmoid space for per-level (theta, omega, sweeps)."""
+    t = np.empty(3 * len(level_params), np.float32)
+    for l, (th, om, sw) in enumerate(level_params):
+        t[3 * l] = (th - 0.1) / 0.8
+        t[3 * l + 1] = (om - 0.6) / 0.8
+        t[3 * l + 2] = (sw - 1) / 2.0
     return t
 
 
@@ -148,23 +155,26 @@ def decode(y, levels=LEVELS):
 class SolveStats:
     iterations: int
     rel_res: float
-    cycle: float
-    op_complexity: float
+    seconds: float          # setup + solve, best of the timed runs
 
     @property
-    def work(self) -> float:
-        return self.iterations * (1.0 + self.cycle) + SETUP_WEIGHT * self.op_complexity
+    def converged(self) -> bool:
+        return self.rel_res < 1e-8
 
     @property
-    def work_per_digit(self) -> float:
+    def seconds_per_digit(self) -> float:
+        """Inverse convergence speed: seconds per decimal digit of residual
+        reduction (the measured convergence factor per second)."""
         digits = -math.log10(max(self.rel_res, 1e-300))
-        return self.work / max(digits, 1e-3)
+        return self.seconds / max(digits, 1e-3)
 
 
-def solve_stats(A, b, level_params=None, model=None, tol=1e-8, max_iter=200) -> SolveStats:
-    """AMG-PCG with uniform defaults, explicit per-level (theta, omega,
-    sweeps), or a model's prediction; deterministic work count."""
-    amg = hx.BoomerAMG()
+def timed_solve(A, b, level_params=None, model=None, fp32=False, tol=1e-8, max_iter=200,
+                repeats=2) -> SolveStats:
+    """AMG-PCG from x0 = 0 with the defaults, explicit per-level settings or a
+    model's prediction; HYPRE_ParCSRPCGSetup + Solve timed by the library,
+    best of `repeats` runs."""
+    amg = hx.BoomerAMG(fp32_cycle=1 if fp32 else 0)
     if model is not None:
         model.apply(amg, A)
     elif level_params is not None:
@@ -172,114 +182,164 @@ def solve_stats(A, b, level_params=None, model=None, tol=1e-8, max_iter=200) -> 
         amg.set_level_params(theta=th, omega=om, sweeps=sw)
     pcg = hx.PCG(amg, tol=tol, max_iter=max_iter)
     x = hx.ParVector(b.n)
-    x.fill(0.0)
-    pcg.setup(A, b, x)
-    pcg.solve(A, b, x)
-    its, rr = pcg.num_iterations(), pcg.final_rel_res()
-    nlev = amg.num_levels()
-    nnzA = [amg.level_dims(l, "A")[2] for l in range(nlev)]
-    nnzP = [amg.level_dims(l, "P")[2] for l in range(nlev - 1)]
-    if model is not None:
-        lp = decode(model.forward(matrix_features(A)[None, :])[0])
-    elif level_params is not None:
-        lp = list(level_params) + [DEFAULT] * LEVELS
-    else:
-        lp = [DEFAULT] * LEVELS
-    cyc = sum((2 * lp[l][2] + 1) * nnzA[l] + (2 * nnzP[l] if l < nlev - 1 else 0)
-              for l in range(nlev)) / nnzA[0]
+    best = None
+    for _ in range(repeats):
+        x.fill(0.0)
+        pcg.setup(A, b, x)
+        pcg.solve(A, b, x)
+        t = sum(pcg.times())
+        best = t if best is None else min(best, t)
+    st = SolveStats(pcg.num_iterations(), pcg.final_rel_res(), best)
     pcg.destroy()
     amg.destroy()
-    return SolveStats(its, rr, cyc, sum(nnzA) / nnzA[0])
+    x.destroy()
+    return st
 
 
-def best_setting(A, b):
-    """Sweep the grid of uniform settings; return (best setting, its stats, default stats)."""
-    base = solve_stats(A, b)
-    best, best_stats = DEFAULT, base
-    for th in THETAS:
-        for om in OMEGAS:
-            for sw in SWEEPS:
-                if (th, om, sw) == DEFAULT:
-                    continue
-                st = solve_stats(A, b, level_params=[(th, om, sw)] * LEVELS)
-                if st.rel_res < 1e-8 and st.work < best_stats.work:
-                    best, best_stats = (th, om, sw), st
-    return best, best_stats, base
+def best_per_level(A, b, repeats=2):
+    """Coordinate search over GRID: the fine level with coarse defaults, then
+    the coarse levels with the best fine setting.  Returns (fine, coarse),
+    their stats, the default stats and every timing."""
+    base = timed_solve(A, b, repeats=repeats)
+    fine, fine_st = DEFAULT, base
+    table = {"fine": {}, "coarse": {}}
+    for g in GRID:
+        st = base if g == DEFAULT else timed_solve(A, b, per_level(g, DEFAULT), repeats=repeats)
+        table["fine"][str(g)] = round(st.seconds, 6) if st.converged else None
+        if st.converged and st.seconds < fine_st.seconds:
+            fine, fine_st = g, st
+    coarse, best_st = DEFAULT, fine_st
+    for g in GRID:
+        st = fine_st if g == DEFAULT else timed_solve(A, b, per_level(fine, g), repeats=repeats)
+        table["coarse"][str(g)] = round(st.seconds, 6) if st.converged else None
+        if st.converged and st.seconds < best_st.seconds:
+            coarse, best_st = g, st
+    return (fine, coarse), best_st, base, table
 
 
 def aniso_problem(N, seed):
-    """Anisotropic 7-point diffusion of the training family: eps = 10^(-3U),
-    rotation pi U about z, from a seeded generator."""
+    """Anisotropic 7-point diffusion (the family of the 64-solve batch):
+    eps = 10^(-3U), rotation pi U about z, from a seeded generator."""
     rng = np.random.default_rng(seed)
     eps = 10.0 ** (-3.0 * rng.uniform())
     theta = math.pi * rng.uniform()
     A = hx.ParCSRMatrix.aniso7(N, eps, theta)
     b = hx.ParVector(N ** 3)
     b.set_random(seed + 1)
-    return A, b, eps, theta
+    return A, b, {"family": "aniso7", "eps": eps, "rotation": theta}
 
 
-def choose(learned: SolveStats, default: SolveStats, tol: float = 1e-8) -> str:
-    """"learned" only if the predicted settings converged and need less work
-    per digit of residual reduction than the defaults."""
-    if learned.rel_res < tol and learned.work_per_digit < default.work_per_digit:
+def jump_problem(N, seed):
+    """Jumping-coefficient diffusion: 7- or 27-point, kappa piecewise constant
+    on cells of 4, 8 or 16 nodes, log-uniform over 2 to 8 decades."""
+    rng = np.random.default_rng(10_000 + seed)
+    stencil = 27 if rng.uniform() < 0.5 else 7
+    cell = int(rng.choice([4, 8, 16]))
+    dec = float(rng.choice([2.0, 4.0, 6.0, 8.0]))
+    A = hx.ParCSRMatrix.var_diffusion(N, stencil, cell, -dec / 2, dec / 2, seed)
+    b = hx.ParVector(N ** 3)
+    b.set_random(seed + 1)
+    return A, b, {"family": f"jump{stencil}", "cell": cell, "decades": dec}
+
+
+def choose(learned: SolveStats, default: SolveStats) -> str:
+    """"learned" only if the predicted settings converged and reduce the
+    residual faster (fewer seconds per digit) than the defaults."""
+    if learned.converged and learned.seconds_per_digit < default.seconds_per_digit:
         return "learned"
     return "default"
 
 
-def apply_with_fallback(amg: hx.BoomerAMG, model: ParamModel, A, b):
+def apply_with_fallback(amg: hx.BoomerAMG, model: ParamModel, A, b, fp32=False):
     """Predict per-level parameters for A; keep them only when AMG-PCG with
-    them needs less work per digit than the defaults.  Returns
-    ("learned" | "default", learned stats, default stats)."""
-    learned = solve_stats(A, b, model=model)
-    default = solve_stats(A, b)
+    them converges faster than the defaults (one timed solve each).  Returns
+    ("learned" | "default", learned stats, default stats, seconds spent
+    deciding)."""
+    import time
+    t0 = time.perf_counter()
+    learned = timed_solve(A, b, model=model, fp32=fp32, repeats=1)
+    default = timed_solve(A, b, fp32=fp32, repeats=1)
     which = choose(learned, default)
     if which == "learned":
         model.apply(amg, A)
     else:
         call("HYPRE_MI355X_BoomerAMGClearLevelParams", amg.h)
-    return which, learned, default
+    return which, learned, default, time.perf_counter() - t0
+
+
+def constant_rules(rows, k=4):
+    """Candidate constant rules (one (fine, coarse) pair for every matrix):
+    the k most frequent training targets."""
+    from collections import Counter
+    cnt = Counter((tuple(r["fine"]), tuple(r["coarse"])) for r in rows)
+    return [list(p) for p, _ in cnt.most_common(k)]
 
 
 def train_main(argv=None):
-    """python -m hypre_amd.learned [--n 64] [--count 64] [--steps 3000]
+    """python -m hypre_amd.learned [--aniso-n 128] [--aniso-count 64]
+                                   [--jump-n 96] [--jump-count 32] [--steps 4000]
 
-    Sweeps the settings grid on `count` anisotropic problems (seeds 1..count),
-    trains the device model on the best settings and writes WEIGHTS_FILE."""
+    Sweeps the per-level grid on the training problems (anisotropic seeds
+    1..aniso-count, jumping-coefficient seeds 1..jump-count), trains the
+    device model on the fastest settings, picks the best constant rule on the
+    same problems, and writes WEIGHTS_FILE."""
     import argparse
     import time
     ap = argparse.ArgumentParser()
-    ap.add_argument("--n", type=int, default=64)
-    ap.add_argument("--count", type=int, default=64)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--aniso-n", type=int, default=128)
+    ap.add_argument("--aniso-count", type=int, default=64)
+    ap.add_argument("--jump-n", type=int, default=96)
+    ap.add_argument("--jump-count", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--out", default=WEIGHTS_FILE)
     args = ap.parse_args(argv)
     hx.initialize()
     X, T, rows = [], [], []
+    problems = ([("aniso", args.aniso_n, s) for s in range(1, args.aniso_count + 1)] +
+                [("jump", args.jump_n, s) for s in range(1, args.jump_count + 1)])
     t0 = time.perf_counter()
-    for s in range(1, args.count + 1):
-        A, b, eps, th = aniso_problem(args.n, s)
+    for kind, N, s in problems:
+        A, b, info = (aniso_problem if kind == "aniso" else jump_problem)(N, s)
         f = matrix_features(A)
-        best, bst, base = best_setting(A, b)
+        (fine, coarse), bst, base, table = best_per_level(A, b)
         X.append(f.astype(np.float32))
-        T.append(encode(*best))
-        rows.append({"seed": s, "eps": eps, "rotation": th, "best": best,
-                     "work_best": round(bst.work, 2), "work_default": round(base.work, 2),
-                     "its_best": bst.iterations, "its_default": base.iterations})
-        print(json.dumps(rows[-1]), flush=True)
+        T.append(encode(per_level(fine, coarse)))
+        rows.append({"kind": kind, "n": N, "seed": s, **info, "features": [round(v, 6) for v in f],
+                     "fine": fine, "coarse": coarse, "seconds_best": round(bst.seconds, 6),
+                     "seconds_default": round(base.seconds, 6), "its_best": bst.iterations,
+                     "its_default": base.iterations, "timings": table})
+        print(json.dumps({k: v for k, v in rows[-1].items() if k != "timings"}), flush=True)
         A.destroy()
         b.destroy()
     X = np.stack(X)
     T = np.stack(T)
     model = ParamModel(seed=1)
     hist = model.train(X, T, args.steps, args.lr)
-    meta = {"family": f"aniso7 {args.n}^3, eps=10^(-3U), rotation pi*U, seeds 1..{args.count}",
-            "grid": {"theta": THETAS, "omega": OMEGAS, "sweeps": SWEEPS},
-            "steps": args.steps, "lr": args.lr, "loss_first": float(hist[0]),
-            "loss_last": float(hist[-1]), "sweep_seconds": round(time.perf_counter() - t0, 1),
-            "mean_work_ratio_best_over_default":
-                float(np.mean([r["work_best"] / r["work_default"] for r in rows])),
+    # the best constant rule on the training problems (mean time over default)
+    cands = constant_rules(rows)
+    scores = []
+    for fine, coarse in cands:
+        r = []
+        for (kind, N, s), row in zip(problems, rows):
+            A, b, _ = (aniso_problem if kind == "aniso" else jump_problem)(N, s)
+            st = timed_solve(A, b, per_level(fine, coarse))
+            r.append(st.seconds / row["seconds_default"] if st.converged else 10.0)
+            A.destroy()
+            b.destroy()
+        scores.append(float(np.mean(r)))
+        print(json.dumps({"constant_rule": [fine, coarse], "mean_time_over_default": scores[-1]}),
+              flush=True)
+    best_rule = cands[int(np.argmin(scores))]
+    meta = {"families": {"aniso7": f"{args.aniso_n}^3, eps=10^(-3U), rotation pi*U, seeds 1..{args.aniso_count}",
+                         "jump": f"{args.jump_n}^3, 7/27-point, cells 4/8/16, 2-8 decades, seeds 1..{args.jump_count}"},
+            "grid": [list(g) for g in GRID], "steps": args.steps, "lr": args.lr,
+            "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+            "sweep_seconds": round(time.perf_counter() - t0, 1),
+            "mean_time_best_over_default":
+                float(np.mean([r["seconds_best"] / r["seconds_default"] for r in rows])),
+            "constant_rule": best_rule, "constant_rule_candidates":
+                [{"rule": c, "mean_time_over_default": sc} for c, sc in zip(cands, scores)],
             "rows": rows}
     model.save(args.out, meta)
     print(json.dumps({k: v for k, v in meta.items() if k != "rows"}), flush=True)

@@ -182,10 +182,11 @@ HYPRE_Int HYPRE_MI355X_ParamModelGradient(HYPRE_MI355X_ParamModel model, HYPRE_I
 HYPRE_Int HYPRE_MI355X_ParamModelTrain(HYPRE_MI355X_ParamModel model, HYPRE_Int batch,
                                        const float *X, const float *T, HYPRE_Int steps, float lr,
                                        float *loss_hist);
-/* the 8 matrix features the model reads: log10(n)/8, mean row length/27,
- * mean max|a_ij|/|a_ii|, mean min/max |a_ij|, mean 2nd/1st largest |a_ij|,
- * mean strong fraction at theta 0.25, mean |row sum|/|a_ii|,
- * std(log10|a_ii|)/4 */
+/* the 8 matrix features the model reads: log10(n)/8; the row-length
+ * distribution (mean/27, standard deviation/27); the off-diagonal anisotropy
+ * ratios (mean min/max |a_ij|, mean 2nd/1st largest |a_ij|); the mean strong
+ * fraction at theta 0.25; a coefficient-jump histogram (fractions of rows
+ * with a neighbour whose |diagonal| differs by a factor >= 10 and >= 1000) */
 HYPRE_Int HYPRE_MI355X_MatrixFeatures(HYPRE_ParCSRMatrix A, HYPRE_Real *features);
 /* features of A -> model -> per-level theta, omega, sweeps of the solver */
 HYPRE_Int HYPRE_MI355X_BoomerAMGSetParamModel(HYPRE_Solver solver, HYPRE_MI355X_ParamModel model,

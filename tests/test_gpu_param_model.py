@@ -116,24 +116,43 @@ def test_model_sets_per_level_parameters(hx, L, PM):
     m.destroy()
 
 
-def test_learned_parameters_reduce_work_on_held_out_problems(hx, L):
-    """The committed model (trained on seeds 1..64 of the anisotropic family)
-    on held-out seeds: with the fallback guard the chosen settings never need
-    more work per digit than the defaults, and on average they need less."""
+HELD_OUT = [("aniso", 128, s) for s in (1001, 1002, 1003, 1004, 1005, 1006)] + \
+           [("jump", 96, s) for s in (2001, 2002, 2003)]
+
+
+def test_learned_parameters_on_held_out_problems(hx, L):
+    """The committed model (trained on anisotropic 128^3 seeds 1..64 and
+    jumping-coefficient 96^3 seeds 1..32) on held-out seeds of both families.
+    After the fallback decision, each candidate is timed afresh (best of two
+    setup+solve runs): the settings kept never converge more slowly than the
+    defaults beyond timing noise, are faster on average, and are compared
+    with the best constant rule the training sweep found."""
     say("held-out gain")
     model = L.ParamModel.load()
-    ratios, picks = [], []
-    for seed in (1001, 1002, 1003, 1004, 1005, 1006):
-        A, b, eps, th = L.aniso_problem(64, seed)
+    import json
+    with open(L.WEIGHTS_FILE) as f:
+        meta = json.load(f)["meta"]
+    rule = L.per_level(*meta["constant_rule"])
+    r_learned, r_const, picks = [], [], []
+    for kind, N, seed in HELD_OUT:
+        A, b, info = (L.aniso_problem if kind == "aniso" else L.jump_problem)(N, seed)
         amg = hx.BoomerAMG()
-        which, learned, default = L.apply_with_fallback(amg, model, A, b)
-        chosen = learned if which == "learned" else default
-        assert chosen.rel_res < 1e-8
-        assert chosen.work_per_digit <= default.work_per_digit
-        ratios.append(chosen.work_per_digit / default.work_per_digit)
-        picks.append(which)
-        say(seed, f"eps={eps:.3g} rot={th:.3f}", which, f"its {learned.iterations} vs {default.iterations}",
-            f"ratio {ratios[-1]:.3f}")
+        which, _, _, _ = L.apply_with_fallback(amg, model, A, b)
         amg.destroy()
-    assert np.mean(ratios) < 0.97, ratios
+        default = L.timed_solve(A, b)
+        chosen = L.timed_solve(A, b, model=model) if which == "learned" else default
+        const = L.timed_solve(A, b, rule)
+        assert chosen.converged and default.converged and const.converged
+        r_learned.append(chosen.seconds_per_digit / default.seconds_per_digit)
+        r_const.append(const.seconds_per_digit / default.seconds_per_digit)
+        picks.append(which)
+        say(kind, seed, info, which, f"its {chosen.iterations}/{default.iterations}/{const.iterations}",
+            f"learned {r_learned[-1]:.3f} constant {r_const[-1]:.3f} (x default)")
+        A.destroy()
+        b.destroy()
+    say("mean learned", np.mean(r_learned), "mean constant rule", np.mean(r_const))
+    assert max(r_learned) < 1.05, r_learned          # timing noise only, never a real loss
+    assert np.mean(r_learned) < 0.97, r_learned
     assert "learned" in picks
+    # the per-problem prediction against one setting for every problem
+    assert np.mean(r_learned) <= np.mean(r_const) + 0.02, (r_learned, r_const)

@@ -62,12 +62,14 @@ def test_level_parameter_mapping():
     assert abs(lp[2][0] - 0.25) < 1e-15 and abs(lp[2][1] - 1.0) < 1e-15 and lp[2][2] == 2
 
 
-def test_fallback_rule_keeps_defaults_unless_cheaper():
+def test_fallback_rule_keeps_defaults_unless_faster():
     """The guard of hypre_amd.learned: learned settings only when converged
-    and cheaper per digit of residual reduction."""
+    and faster per digit of residual reduction (seconds per digit)."""
     from hypre_amd.learned import SolveStats, choose
-    d = SolveStats(iterations=20, rel_res=5e-9, cycle=4.0, op_complexity=1.3)
-    assert choose(SolveStats(15, 6e-9, 4.0, 1.3), d) == "learned"
-    assert choose(SolveStats(25, 6e-9, 4.0, 1.3), d) == "default"
-    assert choose(SolveStats(10, 2e-7, 4.0, 1.3), d) == "default"        # did not converge
-    assert choose(SolveStats(12, 6e-9, 9.0, 1.3), d) == "default"        # fewer its, more work
+    d = SolveStats(iterations=20, rel_res=1e-9, seconds=0.9)          # 0.1 s per digit
+    assert choose(SolveStats(15, 1e-9, 0.8), d) == "learned"
+    assert choose(SolveStats(25, 1e-9, 1.0), d) == "default"
+    assert choose(SolveStats(10, 2e-7, 0.3), d) == "default"           # did not converge
+    # fewer seconds but fewer digits too: 0.85 s for 8 digits is slower per digit
+    assert choose(SolveStats(12, 9.9e-9, 0.85), d) == "default"
+    assert SolveStats(10, 1e-9, 0.9).seconds_per_digit == 0.1
