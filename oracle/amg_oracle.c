@@ -388,8 +388,8 @@ int orc_pmis_measure(const orc_csr *S, int seed, double *measure)
    return 0;
 }
 
-/* CF_init 0: CF is set here; CF_init 1 (HMIS): CF holds the first pass's
- * split on entry and its C points are the first independent set */
+/* CF_init 0 or 3: CF is set here; CF_init 1 (HMIS): CF holds the first
+ * pass's split on entry and its C points are the first independent set */
 static int pmis_core(const orc_csr *S, const double *measure_in, int *CF, int cf_init)
 {
    int n = S->nrows, i, j, jS, ig, cnt, graph_size, iter = 0;
@@ -433,13 +433,14 @@ static int pmis_core(const orc_csr *S, const double *measure_in, int *CF, int cf
    }
    else
    {
-      /* par_coarsen.c:2386-2407 (CF_init == 0) */
+      /* par_coarsen.c:2386-2407 (CF_init 0, or 3 for the second coarsening
+       * of aggressive levels: rows without strong connections become C) */
       for (i = 0; i < n; i++)
       {
          CF[i] = 0;
          if (Si[i + 1] - Si[i] == 0)
          {
-            CF[i] = SF_PT;
+            CF[i] = (cf_init == 3) ? C_PT : SF_PT;
             measure[i] = 0;
          }
          else
@@ -808,6 +809,346 @@ int orc_pmis(const orc_csr *S, int *CF)
    it = orc_pmis_with_measure(S, m, CF);
    free(m);
    return it;
+}
+
+/* ======================================================================
+ * Aggressive coarsening (agg_num_levels), one rank, num_paths 1:
+ * src/parcsr_ls/par_amg_setup.c:1311-1352 (second coarsening of the C
+ * points over the distance-two strength graph) and :1799-1824 (C/F
+ * correction, modified multipass interpolation, agg_interp_type 8).
+ * ====================================================================== */
+
+/* src/parcsr_ls/par_strength.c:1795 hypre_BoomerAMGCreate2ndSHost with
+ * num_paths 1: row ic (the ic-th C point i1 of CF, row order) holds, in
+ * first-seen order, the coarse indices of the C points reached from i1 by one
+ * strong connection (:2440-2449) or by two through any point (:2451-2465),
+ * except ic itself.  Pattern only.  C points whose row is empty are marked 2
+ * in CF (:2700-2722); the second pass writes the same order as the counting
+ * pass (:2740-2790). */
+int orc_create_2nd_s(const orc_csr *S, int *CF, orc_csr *S2)
+{
+   const int n = S->nrows;
+   int i, nc = 0, ic;
+   int *f2c = (int *) xmalloc(sizeof(int) * (size_t) (n > 0 ? n : 1));
+   int *c2f = (int *) xmalloc(sizeof(int) * (size_t) (n > 0 ? n : 1));
+   int *marker;
+   long cap = 1024, cnt = 0;
+   for (i = 0; i < n; i++)
+   {
+      if (CF[i] > 0) { f2c[i] = nc; c2f[nc++] = i; }
+      else { f2c[i] = -1; }
+   }
+   marker = (int *) xmalloc(sizeof(int) * (size_t) (nc > 0 ? nc : 1));
+   for (i = 0; i < nc; i++) { marker[i] = -1; }
+   S2->nrows = nc; S2->ncols = nc; S2->a = NULL;
+   S2->ia = (int *) xmalloc(sizeof(int) * (size_t) (nc + 1));
+   S2->ja = (int *) xmalloc(sizeof(int) * (size_t) cap);
+   for (ic = 0; ic < nc; ic++)
+   {
+      const long row_begin = cnt;
+      const int i1 = c2f[ic];
+      int jj1, jj2;
+      S2->ia[ic] = (int) cnt;
+      for (jj1 = S->ia[i1]; jj1 < S->ia[i1 + 1]; jj1++)
+      {
+         const int i2 = S->ja[jj1];
+         if (cnt + 1 + (S->ia[i2 + 1] - S->ia[i2]) > cap)
+         {
+            cap = 2 * cap + (S->ia[i2 + 1] - S->ia[i2]) + 1;
+            S2->ja = (int *) realloc(S2->ja, sizeof(int) * (size_t) cap);
+         }
+         if (CF[i2] > 0)
+         {
+            const int idx = f2c[i2];
+            if (marker[idx] < row_begin) { marker[idx] = (int) cnt; S2->ja[cnt++] = idx; }
+         }
+         for (jj2 = S->ia[i2]; jj2 < S->ia[i2 + 1]; jj2++)
+         {
+            const int i3 = S->ja[jj2];
+            if (CF[i3] > 0)
+            {
+               const int idx = f2c[i3];
+               if (idx != ic && marker[idx] < row_begin) { marker[idx] = (int) cnt; S2->ja[cnt++] = idx; }
+            }
+         }
+      }
+   }
+   S2->ia[nc] = (int) cnt;
+   for (ic = 0; ic < nc; ic++)
+   {
+      if (S2->ia[ic + 1] == S2->ia[ic]) { CF[c2f[ic]] = 2; }
+   }
+   free(f2c); free(c2f); free(marker);
+   return nc;
+}
+
+/* the second coarsening of the aggressive branch for coarsen_type 8:
+ * hypre_BoomerAMGCoarsenPMIS(S2, S2, 3, ...) (par_amg_setup.c:1324),
+ * i.e. CF_init 3 (par_coarsen.c:2386-2407: rows without strong connections
+ * become C points; :2487: no independent set is picked in the first round) */
+int orc_pmis_cf3_with_measure(const orc_csr *S, const double *measure_in, int *CF)
+{
+   return pmis_core(S, measure_in, CF, 3);
+}
+
+int orc_pmis_cf3(const orc_csr *S, int *CF)
+{
+   double *m = (double *) xmalloc(sizeof(double) * (size_t) (S->nrows > 0 ? S->nrows : 1));
+   int it;
+   orc_pmis_measure(S, 2747, m);
+   it = pmis_core(S, m, CF, 3);
+   free(m);
+   return it;
+}
+
+/* src/parcsr_ls/par_strength.c:3068 hypre_BoomerAMGCorrectCFMarkerHost:
+ * the C points of CF (in row order) take the second coarsening's marks;
+ * points marked 2 (empty second-strength row) become plain C points. */
+void orc_correct_cf_marker(int n, int *CF, const int *CFN)
+{
+   int i, cnt = 0;
+   for (i = 0; i < n; i++)
+   {
+      if (CF[i] > 0)
+      {
+         if (CF[i] == 1) { CF[i] = CFN[cnt++]; }
+         else { CF[i] = 1; cnt++; }
+      }
+   }
+}
+
+/* src/parcsr_mv/par_csr_matop.c:348 hypre_ParMatmul on one rank: row i of
+ * C = A B in first-seen column order; the first product of a column is
+ * stored, later ones added in traversal order (:1107-1140).  When C is
+ * square, each row first gets its diagonal position with value 0
+ * (:476-479, :691-697). */
+int orc_matmul(const orc_csr *A, const orc_csr *B, orc_csr *C)
+{
+   const int n = A->nrows, m = B->ncols;
+   const int allsquare = (n == m);
+   int i, *marker = (int *) xmalloc(sizeof(int) * (size_t) (m > 0 ? m : 1));
+   long cap = 1024, cnt = 0;
+   for (i = 0; i < m; i++) { marker[i] = -1; }
+   C->nrows = n; C->ncols = m;
+   C->ia = (int *) xmalloc(sizeof(int) * (size_t) (n + 1));
+   C->ja = (int *) xmalloc(sizeof(int) * (size_t) cap);
+   C->a = (double *) xmalloc(sizeof(double) * (size_t) cap);
+   for (i = 0; i < n; i++)
+   {
+      const long row_begin = cnt;
+      int jj2, jj3;
+      C->ia[i] = (int) cnt;
+      if (cnt + 1 > cap)
+      {
+         cap = 2 * cap + 1;
+         C->ja = (int *) realloc(C->ja, sizeof(int) * (size_t) cap);
+         C->a = (double *) realloc(C->a, sizeof(double) * (size_t) cap);
+      }
+      if (allsquare) { marker[i] = (int) cnt; C->ja[cnt] = i; C->a[cnt++] = 0.0; }
+      for (jj2 = A->ia[i]; jj2 < A->ia[i + 1]; jj2++)
+      {
+         const int i2 = A->ja[jj2];
+         const double a_entry = A->a[jj2];
+         if (cnt + (B->ia[i2 + 1] - B->ia[i2]) > cap)
+         {
+            cap = 2 * cap + (B->ia[i2 + 1] - B->ia[i2]);
+            C->ja = (int *) realloc(C->ja, sizeof(int) * (size_t) cap);
+            C->a = (double *) realloc(C->a, sizeof(double) * (size_t) cap);
+         }
+         for (jj3 = B->ia[i2]; jj3 < B->ia[i2 + 1]; jj3++)
+         {
+            const int i3 = B->ja[jj3];
+            if (marker[i3] < row_begin)
+            {
+               marker[i3] = (int) cnt;
+               C->ja[cnt] = i3;
+               C->a[cnt++] = a_entry * B->a[jj3];
+            }
+            else
+            {
+               C->a[marker[i3]] += a_entry * B->a[jj3];
+            }
+         }
+      }
+   }
+   C->ia[n] = (int) cnt;
+   free(marker);
+   return 0;
+}
+
+/* src/parcsr_ls/par_mod_multi_interp.c:530 hypre_GenerateMultipassPi, one
+ * rank: for the `num_points` rows of `pass_order` (pass color+1), the
+ * entries a_ij of the strong connections j of pass `color` (S order; the
+ * entry of A found by a forward search, :717-726), scaled to
+ * -a_ij * row_sums[i] / (sum_j a_ij * a_ii) (:729-765; row_sums[i] is
+ * divided in place only when the product is nonzero).  Columns number the
+ * points of pass `color` in row order. */
+static void gen_multipass_pi(const orc_csr *A, const orc_csr *S, const int *pass_order,
+                             const int *pass_marker, int num_points, int color,
+                             double *row_sums, orc_csr *Pi)
+{
+   const int n = A->nrows;
+   int i, j, nct = 0, cnt = 0;
+   int *f2c = (int *) xmalloc(sizeof(int) * (size_t) (n > 0 ? n : 1));
+   for (i = 0; i < n; i++) { f2c[i] = (pass_marker[i] == color) ? nct++ : -1; }
+   Pi->nrows = num_points; Pi->ncols = nct;
+   Pi->ia = (int *) xcalloc((size_t) num_points + 1, sizeof(int));
+   for (i = 0; i < num_points; i++)
+   {
+      const int i1 = pass_order[i];
+      for (j = S->ia[i1]; j < S->ia[i1 + 1]; j++)
+      {
+         if (pass_marker[S->ja[j]] == color) { Pi->ia[i + 1]++; }
+      }
+   }
+   for (i = 0; i < num_points; i++) { Pi->ia[i + 1] += Pi->ia[i]; }
+   Pi->ja = (int *) xmalloc(sizeof(int) * (size_t) (Pi->ia[num_points] > 0 ? Pi->ia[num_points] : 1));
+   Pi->a = (double *) xmalloc(sizeof(double) * (size_t) (Pi->ia[num_points] > 0 ? Pi->ia[num_points] : 1));
+   for (i = 0; i < num_points; i++)
+   {
+      const int i1 = pass_order[i];
+      int j2 = A->ia[i1];
+      for (j = S->ia[i1]; j < S->ia[i1 + 1]; j++)
+      {
+         const int j1 = S->ja[j];
+         while (A->ja[j2] != j1) { j2++; }
+         if (pass_marker[j1] == color)
+         {
+            Pi->ja[cnt] = f2c[j1];
+            Pi->a[cnt++] = A->a[j2];
+         }
+      }
+   }
+   for (i = 0; i < num_points; i++)
+   {
+      const int i1 = pass_order[i];
+      const double diagonal = A->a[A->ia[i1]];
+      double row_sum_C = 0.0, value;
+      for (j = Pi->ia[i]; j < Pi->ia[i + 1]; j++) { row_sum_C += Pi->a[j]; }
+      value = row_sum_C * diagonal;
+      if (value != 0) { row_sums[i1] /= value; }
+      for (j = Pi->ia[i]; j < Pi->ia[i + 1]; j++) { Pi->a[j] = -Pi->a[j] * row_sums[i1]; }
+   }
+   free(f2c);
+}
+
+/* src/parcsr_ls/par_mod_multi_interp.c:17 hypre_BoomerAMGBuildModMultipassHost
+ * with interp_type 8, one rank, num_functions 1.  Points are colored by
+ * passes (C points pass 1; a remaining point joins pass p+1 when one of its
+ * strong connections is in pass p; :151-219, at most 9 passes), the weights
+ * of pass 2 come from gen_multipass_pi and those of pass p+1 are the product
+ * of its own gen_multipass_pi matrix with the weights of pass p (:247-259),
+ * and P is truncated by (trunc_factor, P_max_elmts) when either is set
+ * (:410-419).  CF -3 becomes -1 at the end (:503-506).  Returns 0, or -1 when
+ * the passes exceed the reference's limit. */
+int orc_mod_multipass(const orc_csr *A, const orc_csr *S, int *CF, double trunc_factor,
+                      int P_max_elmts, orc_csr *P)
+{
+   const int n = A->nrows;
+   int i, j, cnt = 0, remaining = 0, current_pass = 1, num_passes = 1, nc = 0, p;
+   int *pass_marker = (int *) xcalloc((size_t) (n > 0 ? n : 1), sizeof(int));
+   int *pass_order = (int *) xcalloc((size_t) (n > 0 ? n : 1), sizeof(int));
+   int *f2c = (int *) xcalloc((size_t) (n > 0 ? n : 1), sizeof(int));
+   int *left = (int *) xcalloc((size_t) (n > 0 ? n : 1), sizeof(int));
+   int pass_starts[11] = {0};
+   double *row_sums;
+   orc_csr Pi[10];
+   memset(Pi, 0, sizeof(Pi));
+   for (i = 0; i < n; i++)
+   {
+      if (CF[i] == 1) { pass_marker[i] = 1; f2c[i] = cnt; pass_order[cnt++] = i; }
+      else { left[remaining++] = i; }
+   }
+   nc = cnt;
+   pass_starts[1] = cnt;
+   while (remaining > 0)
+   {
+      const int old_remaining = remaining;
+      int cnt_rem = 0;
+      for (i = 0; i < old_remaining; i++)
+      {
+         const int i1 = left[i];
+         int hit = 0;
+         for (j = S->ia[i1]; j < S->ia[i1 + 1]; j++)
+         {
+            if (pass_marker[S->ja[j]] == current_pass) { hit = 1; break; }
+         }
+         if (hit) { pass_marker[i1] = current_pass + 1; pass_order[cnt++] = i1; remaining--; }
+         else { left[cnt_rem++] = i1; }
+      }
+      remaining = cnt_rem;
+      current_pass++;
+      num_passes++;
+      if (num_passes > 9)
+      {
+         free(pass_marker); free(pass_order); free(f2c); free(left);
+         return -1;
+      }
+      pass_starts[num_passes] = cnt;
+      if (old_remaining == remaining) { break; }
+   }
+   /* :221-265 row sums of the off-diagonal entries of the F rows */
+   row_sums = (double *) xcalloc((size_t) (n > 0 ? n : 1), sizeof(double));
+   for (i = 0; i < n; i++)
+   {
+      if (CF[i] < 0)
+      {
+         for (j = A->ia[i] + 1; j < A->ia[i + 1]; j++) { row_sums[i] += A->a[j]; }
+      }
+   }
+   if (num_passes > 1)
+   {
+      gen_multipass_pi(A, S, &pass_order[pass_starts[1]], pass_marker,
+                       pass_starts[2] - pass_starts[1], 1, row_sums, &Pi[0]);
+   }
+   for (i = 1; i < num_passes - 1; i++)
+   {
+      orc_csr Q;
+      gen_multipass_pi(A, S, &pass_order[pass_starts[i + 1]], pass_marker,
+                       pass_starts[i + 2] - pass_starts[i + 1], i + 1, row_sums, &Q);
+      orc_matmul(&Q, &Pi[i - 1], &Pi[i]);
+      orc_csr_free(&Q);
+   }
+   /* :266-330 assemble P: C rows are unit rows, pass rows copy their Pi row */
+   P->nrows = n; P->ncols = nc;
+   P->ia = (int *) xcalloc((size_t) n + 1, sizeof(int));
+   for (i = 0; i < pass_starts[1]; i++) { P->ia[pass_order[i] + 1] = 1; }
+   for (p = 0; p < num_passes - 1; p++)
+   {
+      int j1 = 0;
+      for (i = pass_starts[p + 1]; i < pass_starts[p + 2]; i++, j1++)
+      {
+         P->ia[pass_order[i] + 1] = Pi[p].ia[j1 + 1] - Pi[p].ia[j1];
+      }
+   }
+   for (i = 0; i < n; i++) { P->ia[i + 1] += P->ia[i]; }
+   P->ja = (int *) xmalloc(sizeof(int) * (size_t) (P->ia[n] > 0 ? P->ia[n] : 1));
+   P->a = (double *) xmalloc(sizeof(double) * (size_t) (P->ia[n] > 0 ? P->ia[n] : 1));
+   for (i = 0; i < pass_starts[1]; i++)
+   {
+      const int i1 = pass_order[i];
+      P->ja[P->ia[i1]] = f2c[i1];
+      P->a[P->ia[i1]] = 1.0;
+   }
+   for (p = 0; p < num_passes - 1; p++)
+   {
+      int j1 = 0;
+      for (i = pass_starts[p + 1]; i < pass_starts[p + 2]; i++, j1++)
+      {
+         const int i1 = pass_order[i];
+         int i2 = Pi[p].ia[j1];
+         for (j = P->ia[i1]; j < P->ia[i1 + 1]; j++, i2++)
+         {
+            P->ja[j] = Pi[p].ja[i2];
+            P->a[j] = Pi[p].a[i2];
+         }
+      }
+   }
+   for (p = 0; p < num_passes - 1; p++) { orc_csr_free(&Pi[p]); }
+   if (trunc_factor != 0.0 || P_max_elmts > 0) { orc_truncate(P, trunc_factor, P_max_elmts); }
+   for (i = 0; i < n; i++) { if (CF[i] == SF_PT) { CF[i] = F_PT; } }
+   free(pass_marker); free(pass_order); free(f2c); free(left); free(row_sums);
+   return 0;
 }
 
 /* ======================================================================
@@ -1551,6 +1892,10 @@ void orc_amg_default_params(orc_amg_params *p)
    p->hgs_parts = 0;
    p->hgs_rows = 4;
    p->coarsen_type = 8;
+   p->agg_num_levels = 0;
+   p->agg_interp_type = 8;
+   p->agg_trunc_factor = 0.0;
+   p->agg_P_max_elmts = 0;
 }
 
 static void csr_copy(const orc_csr *A, orc_csr *B)
@@ -1580,10 +1925,25 @@ orc_amg *orc_amg_setup(const orc_csr *A0, const orc_amg_params *prm)
       int fine_size = H->A[level].nrows, coarse_size = 0, i;
       orc_csr *A = &H->A[level];
       if (prm->max_levels == 1) { break; }
+      const int aggressive = level < prm->agg_num_levels;
       orc_strength(A, prm->strong_threshold, prm->max_row_sum, &H->S[level]);
       H->CF[level] = (int *) xmalloc(sizeof(int) * (size_t) fine_size);
       H->rounds[level] = (prm->coarsen_type == 10) ? orc_hmis(&H->S[level], H->CF[level])
                                                    : orc_pmis(&H->S[level], H->CF[level]);
+      if (aggressive)
+      {
+         /* par_amg_setup.c:1311-1327: the C points are coarsened again over
+          * the distance-two strength graph; :1799-1806 the split is
+          * corrected before the interpolation */
+         orc_csr S2;
+         int *CFN;
+         const int nc1 = orc_create_2nd_s(&H->S[level], H->CF[level], &S2);
+         CFN = (int *) xmalloc(sizeof(int) * (size_t) (nc1 > 0 ? nc1 : 1));
+         orc_pmis_cf3(&S2, CFN);
+         orc_correct_cf_marker(fine_size, H->CF[level], CFN);
+         free(CFN);
+         orc_csr_free(&S2);
+      }
       for (i = 0; i < fine_size; i++) { if (H->CF[level][i] == 1) { coarse_size++; } }
       /* par_amg_setup.c:1690-1725: no coarse grid -> stop, coarsest solve is
        * one sweep of the down smoother */
@@ -1594,8 +1954,16 @@ orc_amg *orc_amg_setup(const orc_csr *A0, const orc_amg_params *prm)
          if (level > 0) { free(H->CF[level]); H->CF[level] = NULL; }
          break;
       }
-      orc_extpi_interp(A, &H->S[level], H->CF[level], prm->trunc_factor,
-                       prm->P_max_elmts, &H->P[level]);
+      if (aggressive)
+      {
+         orc_mod_multipass(A, &H->S[level], H->CF[level], prm->agg_trunc_factor,
+                           prm->agg_P_max_elmts, &H->P[level]);
+      }
+      else
+      {
+         orc_extpi_interp(A, &H->S[level], H->CF[level], prm->trunc_factor,
+                          prm->P_max_elmts, &H->P[level]);
+      }
       orc_transpose(&H->P[level], &H->R[level]);
       orc_rap(&H->R[level], A, &H->P[level], &H->A[level + 1]);
       level++;

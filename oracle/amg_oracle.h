@@ -80,6 +80,24 @@ int orc_rs_first_pass(const orc_csr *S, int *CF);
  * given measure / the hypre seed.  Returns the PMIS rounds. */
 int orc_hmis_with_measure(const orc_csr *S, const double *measure_in, int *CF);
 int orc_hmis(const orc_csr *S, int *CF);
+/* ---------------- aggressive coarsening (one rank, num_paths 1) -------- */
+/* src/parcsr_ls/par_strength.c:1795 hypre_BoomerAMGCreate2ndSHost: strength
+ * over the C points of CF (distance one or two in S), pattern only, rows and
+ * columns in coarse numbering; C points with an empty row get CF 2.
+ * Returns the number of C points. */
+int  orc_create_2nd_s(const orc_csr *S, int *CF, orc_csr *S2);
+/* PMIS with CF_init 3 (the second coarsening, par_amg_setup.c:1324) */
+int  orc_pmis_cf3_with_measure(const orc_csr *S, const double *measure_in, int *CF);
+int  orc_pmis_cf3(const orc_csr *S, int *CF);
+/* src/parcsr_ls/par_strength.c:3068 hypre_BoomerAMGCorrectCFMarkerHost */
+void orc_correct_cf_marker(int n, int *CF, const int *CFN);
+/* src/parcsr_mv/par_csr_matop.c:348 hypre_ParMatmul (one rank) */
+int  orc_matmul(const orc_csr *A, const orc_csr *B, orc_csr *C);
+/* src/parcsr_ls/par_mod_multi_interp.c:17 hypre_BoomerAMGBuildModMultipassHost,
+ * interp_type 8; CF -3 -> -1 at the end.  -1 when passes exceed 9. */
+int  orc_mod_multipass(const orc_csr *A, const orc_csr *S, int *CF, double trunc_factor,
+                       int P_max_elmts, orc_csr *P);
+
 /* src/parcsr_ls/par_lr_interp.c:1024 hypre_BoomerAMGBuildExtPIInterpHost,
  * followed by par_interp.c:2622 truncation when max_elmts > 0 or trunc > 0.
  * Converts CF -3 -> -1 at the end, as the reference does. */
@@ -139,6 +157,10 @@ typedef struct
    int    hgs_parts;          /* hybrid GS partitions; 0: about hgs_rows rows each */
    int    hgs_rows;           /* 4 */
    int    coarsen_type;       /* 8 PMIS, 10 HMIS */
+   int    agg_num_levels;     /* 0: levels below this one are coarsened aggressively */
+   int    agg_interp_type;    /* 8: modified multipass */
+   double agg_trunc_factor;   /* 0 */
+   int    agg_P_max_elmts;    /* 0 */
 } orc_amg_params;
 
 void orc_amg_default_params(orc_amg_params *p);

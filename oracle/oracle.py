@@ -31,7 +31,9 @@ class _Params(C.Structure):
                 ("relax_type", C.c_int), ("coarse_relax_type", C.c_int),
                 ("relax_weight", C.c_double), ("num_sweeps", C.c_int),
                 ("relax_up_type", C.c_int), ("outer_weight", C.c_double),
-                ("hgs_parts", C.c_int), ("hgs_rows", C.c_int), ("coarsen_type", C.c_int)]
+                ("hgs_parts", C.c_int), ("hgs_rows", C.c_int), ("coarsen_type", C.c_int),
+                ("agg_num_levels", C.c_int), ("agg_interp_type", C.c_int),
+                ("agg_trunc_factor", C.c_double), ("agg_P_max_elmts", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -65,6 +67,14 @@ def lib():
         L.orc_rs_first_pass.argtypes = [P(_Csr), P(C.c_int)]
         L.orc_hmis_with_measure.argtypes = [P(_Csr), P(C.c_double), P(C.c_int)]
         L.orc_hmis.argtypes = [P(_Csr), P(C.c_int)]
+        L.orc_create_2nd_s.argtypes = [P(_Csr), P(C.c_int), P(_Csr)]
+        L.orc_pmis_cf3_with_measure.argtypes = [P(_Csr), P(C.c_double), P(C.c_int)]
+        L.orc_pmis_cf3.argtypes = [P(_Csr), P(C.c_int)]
+        L.orc_correct_cf_marker.argtypes = [C.c_int, P(C.c_int), P(C.c_int)]
+        L.orc_correct_cf_marker.restype = None
+        L.orc_matmul.argtypes = [P(_Csr), P(_Csr), P(_Csr)]
+        L.orc_mod_multipass.argtypes = [P(_Csr), P(_Csr), P(C.c_int), C.c_double, C.c_int,
+                                        P(_Csr)]
         L.orc_extpi_interp.argtypes = [P(_Csr), P(_Csr), P(C.c_int), C.c_double, C.c_int,
                                        P(_Csr)]
         L.orc_extpi_interp_raw.argtypes = [P(_Csr), P(_Csr), P(C.c_int), P(_Csr)]
@@ -254,6 +264,53 @@ def hmis(S: Csr, measure: np.ndarray | None = None):
     return cf, r
 
 
+def create_2nd_s(S: Csr, CF: np.ndarray):
+    """Second strength matrix of aggressive coarsening over the C points of
+    CF.  Returns (S2 pattern, CF with 2 on C points whose S2 row is empty)."""
+    cf = np.ascontiguousarray(CF, dtype=np.int32).copy()
+    cs, out = S.to_c(), _Csr()
+    lib().orc_create_2nd_s(C.byref(cs), _ip(cf), C.byref(out))
+    return _from_c(out, with_values=False), cf
+
+
+def pmis_cf3(S: Csr, measure: np.ndarray | None = None):
+    """PMIS with CF_init 3 (rows without strong connections become C).
+    Returns (CF, rounds)."""
+    cf = np.empty(S.nrows, dtype=np.int32)
+    cs = S.to_c()
+    if measure is None:
+        r = lib().orc_pmis_cf3(C.byref(cs), _ip(cf))
+    else:
+        m = np.ascontiguousarray(measure, dtype=np.float64)
+        r = lib().orc_pmis_cf3_with_measure(C.byref(cs), _dp(m), _ip(cf))
+    return cf, r
+
+
+def correct_cf_marker(CF: np.ndarray, CFN: np.ndarray) -> np.ndarray:
+    cf = np.ascontiguousarray(CF, dtype=np.int32).copy()
+    cfn = np.ascontiguousarray(CFN, dtype=np.int32)
+    lib().orc_correct_cf_marker(len(cf), _ip(cf), _ip(cfn))
+    return cf
+
+
+def matmul(A: Csr, B: Csr) -> Csr:
+    ca, cb, out = A.to_c(), B.to_c(), _Csr()
+    lib().orc_matmul(C.byref(ca), C.byref(cb), C.byref(out))
+    return _from_c(out)
+
+
+def mod_multipass(A: Csr, S: Csr, CF: np.ndarray, trunc_factor=0.0, max_elmts=0):
+    """Modified multipass interpolation (agg_interp_type 8).  Returns
+    (P, CF_after) with -3 mapped to -1."""
+    cf = np.ascontiguousarray(CF, dtype=np.int32).copy()
+    ca, cs, p = A.to_c(), S.to_c(), _Csr()
+    rc = lib().orc_mod_multipass(C.byref(ca), C.byref(cs), _ip(cf), trunc_factor, max_elmts,
+                                 C.byref(p))
+    if rc != 0:
+        raise RuntimeError("multipass interpolation: more than 9 passes")
+    return _from_c(p), cf
+
+
 def extpi_interp(A: Csr, S: Csr, CF: np.ndarray, trunc_factor=0.0, max_elmts=4):
     """Returns (P, CF_after) -- CF_after has -3 mapped to -1 like the reference."""
     cf = np.ascontiguousarray(CF, dtype=np.int32).copy()
@@ -268,6 +325,29 @@ def extpi_interp_raw(A: Csr, S: Csr, CF: np.ndarray) -> Csr:
     ca, cs, p = A.to_c(), S.to_c(), _Csr()
     lib().orc_extpi_interp_raw(C.byref(ca), C.byref(cs), _ip(cf), C.byref(p))
     return _from_c(p)
+
+
+def truncate(P: Csr, trunc_factor=0.0, max_elmts=4) -> Csr:
+    """hypre_ParCSRMatrixTruncate (rescale 1) of P, in place; returns P."""
+    libc = C.CDLL(None)
+    libc.malloc.restype = C.c_void_p
+    libc.malloc.argtypes = [C.c_size_t]
+    cp = _Csr()
+    cp.nrows, cp.ncols = P.nrows, P.ncols
+    # orc_truncate replaces the arrays with malloc'd ones and frees the old:
+    # hand it malloc'd copies
+    bufs = []
+    for arr, ty in ((np.ascontiguousarray(P.ia, np.int32), C.c_int),
+                    (np.ascontiguousarray(P.ja, np.int32), C.c_int),
+                    (np.ascontiguousarray(P.a, np.float64), C.c_double)):
+        ptr = libc.malloc(max(arr.nbytes, 8))
+        C.memmove(ptr, arr.ctypes.data, arr.nbytes)
+        bufs.append(C.cast(ptr, C.POINTER(ty)))
+    cp.ia, cp.ja, cp.a = bufs
+    lib().orc_truncate(C.byref(cp), trunc_factor, max_elmts)
+    out = _from_c(cp, free=True)
+    P.ia, P.ja, P.a = out.ia, out.ja, out.a
+    return P
 
 
 def qsort2_abs(v: np.ndarray, w: np.ndarray):
