@@ -365,6 +365,11 @@ class BoomerAMG:
         "outer_wt": "HYPRE_BoomerAMGSetOuterWt",
         "print_level": "HYPRE_BoomerAMGSetPrintLevel",
         "cycle_type": "HYPRE_BoomerAMGSetCycleType",
+        "agg_num_levels": "HYPRE_BoomerAMGSetAggNumLevels",
+        "agg_interp_type": "HYPRE_BoomerAMGSetAggInterpType",
+        "agg_trunc_factor": "HYPRE_BoomerAMGSetAggTruncFactor",
+        "agg_P_max_elmts": "HYPRE_BoomerAMGSetAggPMaxElmts",
+        "num_paths": "HYPRE_BoomerAMGSetNumPaths",
         "fp32_cycle": "HYPRE_MI355X_BoomerAMGSetCyclePrecision",
         "keep_setup_data": "HYPRE_MI355X_BoomerAMGSetKeepSetupData",
         "solve_format": "HYPRE_MI355X_BoomerAMGSetSolveFormat",

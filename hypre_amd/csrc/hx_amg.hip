@@ -95,9 +95,14 @@ static int *l1_zero_flag(int level) { return device_flags() + 768 + std::min(lev
 void AMGSolver::setup(const Csr &A0)
 {
    double t0 = wall_seconds();
+   const AMGParams &p = prm;
+   if (p.agg_num_levels > 0 && (p.agg_interp_type != 8 || p.coarsen_type != 8)) {
+      g_error_arg = 1;
+      fail(HYPRE_ERROR_ARG, "aggressive coarsening is built for PMIS (coarsen_type 8) with "
+                            "modified multipass interpolation (agg_interp_type 8) only");
+   }
    free_hierarchy();
    TimedLaunch tl("setup", 0.0);
-   const AMGParams &p = prm;
    L.reserve(std::max(p.max_levels, 1) + 1);
    L.emplace_back();
    L[0].A = &A0;
@@ -125,6 +130,22 @@ void AMGSolver::setup(const Csr &A0)
                                               : pmis(lv.S, nullptr, lv.CF, lv.measure, stc);
       dfree(stc);
       int *f2c = dalloc_n<int>((size_t) fine_size + 1);
+      const bool aggressive = level < p.agg_num_levels;
+      if (aggressive) {
+         // par_amg_setup.c:1311-1327: the C points are coarsened again over
+         // the distance-two strength graph (PMIS, CF_init 3), and
+         // par_amg_setup.c:1799-1806 corrects the split with the result
+         const int nc1 = coarse_map(fine_size, lv.CF, f2c);
+         Csr S2;
+         create_2nd_s(lv.S, lv.CF, f2c, nc1, S2);
+         if (nc1 > 0) {
+            int *CFN = dalloc_n<int>((size_t) nc1);
+            pmis(S2, nullptr, CFN, nullptr, nullptr, 3);
+            correct_cf_marker(fine_size, lv.CF, f2c, CFN);
+            dfree(CFN);
+         }
+         S2.free_all();
+      }
       const int coarse_size = coarse_map(fine_size, lv.CF, f2c);
       // par_amg_setup.c:1690-1725
       if (coarse_size == 0 || coarse_size == fine_size) {
@@ -147,8 +168,13 @@ void AMGSolver::setup(const Csr &A0)
          dfree(lv.CF); lv.CF = nullptr;
          break;
       }
-      extpi_interp(A, lv.S, lv.CF, f2c, coarse_size, p.trunc_factor, p.P_max_elmts, lv.P, s_thr,
-                   s_flg);
+      if (aggressive) {
+         mod_multipass_interp(A, lv.S, lv.CF, coarse_size, p.agg_trunc_factor, p.agg_P_max_elmts,
+                              lv.P);
+      } else {
+         extpi_interp(A, lv.S, lv.CF, f2c, coarse_size, p.trunc_factor, p.P_max_elmts, lv.P, s_thr,
+                      s_flg);
+      }
       dfree(s_thr);
       dfree(s_flg);
       dfree(f2c);
@@ -779,8 +805,46 @@ HYPRE_Int HYPRE_BoomerAMGSetNumFunctions(HYPRE_Solver solver, HYPRE_Int v)
 HYPRE_Int HYPRE_BoomerAMGSetAggNumLevels(HYPRE_Solver solver, HYPRE_Int v)
 {
    return guard([&] {
-      if (v != 0) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "aggressive coarsening is not supported"); }
+      if (v < 0) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "agg_num_levels must be >= 0"); }
       as_amg(solver)->prm.agg_num_levels = v;
+   });
+}
+
+// src/parcsr_ls/HYPRE_parcsr_amg.c (hypre_BoomerAMGSetAggInterpType,
+// par_amg.c): this build interpolates aggressive levels with the modified
+// multipass method (8) only; hypre's default 4 is accepted here and refused
+// at setup when aggressive levels are requested
+HYPRE_Int HYPRE_BoomerAMGSetAggInterpType(HYPRE_Solver solver, HYPRE_Int v)
+{
+   return guard([&] {
+      if (v < 0 || v > 9) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "agg_interp_type must be 0..9"); }
+      as_amg(solver)->prm.agg_interp_type = v;
+   });
+}
+
+HYPRE_Int HYPRE_BoomerAMGSetAggPMaxElmts(HYPRE_Solver solver, HYPRE_Int v)
+{
+   return guard([&] {
+      if (v < 0) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "agg_P_max_elmts must be >= 0"); }
+      as_amg(solver)->prm.agg_P_max_elmts = v;
+   });
+}
+
+HYPRE_Int HYPRE_BoomerAMGSetAggTruncFactor(HYPRE_Solver solver, HYPRE_Real v)
+{
+   return guard([&] {
+      if (v < 0.0 || v >= 1.0) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "agg_trunc_factor must be in [0, 1)"); }
+      as_amg(solver)->prm.agg_trunc_factor = v;
+   });
+}
+
+// num_paths 1 is the only value of this build (the second strength matrix
+// keeps every distance-two connection)
+HYPRE_Int HYPRE_BoomerAMGSetNumPaths(HYPRE_Solver solver, HYPRE_Int v)
+{
+   return guard([&] {
+      if (v != 1) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "only num_paths 1 is supported"); }
+      (void) as_amg(solver);
    });
 }
 

@@ -210,7 +210,9 @@ void csr_reverse_rows(const Csr &A, Csr &R);
 // PMIS: measure (optional input: given measure on device, else Park-Miller
 // stream seed 2747), CF out; returns number of rounds
 // st_count: the column counts of S when already known (else counted here)
-// cf_init 1 (HMIS): CF holds the Ruge-Stueben first pass on entry
+// cf_init 1 (HMIS): CF holds the Ruge-Stueben first pass on entry; cf_init 3
+// (second coarsening of aggressive levels): rows without strong connections
+// become C points and the first round picks no independent set
 int  pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_keep_dev,
           const int *st_count = nullptr, int cf_init = 0);
 // HMIS (coarsen_type 10): the Ruge-Stueben first pass on the host, then PMIS
@@ -227,6 +229,17 @@ int  coarse_map(int n, const int *CF, int *f2c);
 void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
                   double trunc_factor, int max_elmts, Csr &P, const double *s_thr = nullptr,
                   const int *s_flg = nullptr);
+// hypre_BoomerAMGInterpTruncation of P in place (rows of <= 64 entries)
+void truncate_rows(Csr &P, double trunc_factor, int max_elmts);
+// aggressive coarsening: the second strength matrix over the nc C points of
+// CF (f2c their coarse indices), distance one or two in S, first-seen order
+void create_2nd_s(const Csr &S, const int *CF, const int *f2c, int nc, Csr &S2);
+// C points of CF take the second split CFN (f2c1: coarse index of the first split)
+void correct_cf_marker(int n, int *CF, const int *f2c1, const int *CFN);
+// modified multipass interpolation (agg_interp_type 8) for nc C points,
+// truncated by (trunc_factor, max_elmts) when either is set; CF -3 -> -1
+void mod_multipass_interp(const Csr &A, const Csr &S, int *CF, int nc, double trunc_factor,
+                          int max_elmts, Csr &P);
 void transpose(const Csr &P, Csr &R);
 void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac);
 // dense column-major copy of a small CSR matrix (n <= 64)

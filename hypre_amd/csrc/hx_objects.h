@@ -73,6 +73,9 @@ struct AMGParams
    int    relax_order = 0;
    int    measure_type = 0;
    int    agg_num_levels = 0;
+   int    agg_interp_type = 4;   // hypre's default; this build runs 8 (modified multipass)
+   double agg_trunc_factor = 0.0;
+   int    agg_P_max_elmts = 0;
    int    num_functions = 1;
    int    print_level = 0;
    int    logging = 0;

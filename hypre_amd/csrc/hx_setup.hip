@@ -890,18 +890,21 @@ __global__ void k_col_count(long long nnz, const int *__restrict__ ja, int *__re
 }
 
 // par_coarsen.c:2248-2407 + par_indepset.c:50-75, first-round candidate flags
+// CF_init 3 (par_coarsen.c:2396-2399): rows without strong connections
+// become C points, and no candidate is flagged (no independent set is picked
+// in the first round, par_coarsen.c:2487)
 __global__ void k_pmis_init(int n, const int *__restrict__ S_ia, const int *__restrict__ cnt,
                             const double *__restrict__ given, int seed, double *__restrict__ m,
-                            int *__restrict__ CF)
+                            int *__restrict__ CF, int cf_init)
 {
    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
       double mi = given ? given[i] : (double) cnt[i] + pm_draw(seed, i);
       if (S_ia[i + 1] == S_ia[i]) {
-         CF[i] = -3;                 // SF_PT
+         CF[i] = (cf_init == 3) ? 1 : -3;   // C_PT : SF_PT
          m[i] = 0.0;
       } else {
          m[i] = mi;
-         CF[i] = (mi > 1) ? 1 : 0;
+         CF[i] = (cf_init == 0 && mi > 1) ? 1 : 0;
       }
    }
 }
@@ -1104,7 +1107,7 @@ int pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_ke
    if (cf_init == 1) {
       k_pmis_init_cf1<<<nb, 256, 0, st>>>(n, S.ia, stc, measure_in_dev, 2747, m, CF);
    } else {
-      k_pmis_init<<<nb, 256, 0, st>>>(n, S.ia, stc, measure_in_dev, 2747, m, CF);
+      k_pmis_init<<<nb, 256, 0, st>>>(n, S.ia, stc, measure_in_dev, 2747, m, CF, cf_init);
    }
    HX_LAUNCH_CHECK();
    if (measure_keep_dev) { d2d(measure_keep_dev, m, sizeof(double) * n); }
@@ -1121,8 +1124,8 @@ int pmis(const Csr &S, const double *measure_in_dev, int *CF, double *measure_ke
       const int nbk = (int) std::max<long long>(1, std::min<long long>(((long long) nw * K + 255) / 256, 16384));
       const int nbu = std::max(1, std::min((nw + 255) / 256, 8192));
       // par_coarsen.c:2487: with CF_init 1 the first round keeps the given
-      // C points as its independent set
-      const bool pick = !(cf_init == 1 && rounds == 0);
+      // C points as its independent set; with CF_init 3 it picks none
+      const bool pick = !(cf_init != 0 && rounds == 0);
       dispatch_k(K, [&](auto kc) {
          if (pick) { k_pmis_remove<decltype(kc)::value><<<nbk, 256, 0, st>>>(nw, list, S.ia, S.ja, m, CF); }
          k_pmis_cf<decltype(kc)::value><<<nbk, 256, 0, st>>>(nw, list, S.ia, S.ja, m, CF);
@@ -3092,6 +3095,69 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
    dfree(tmp_a);
    dfree(base);
    dfree(cnt);
+}
+
+// rows of P staged at a fixed stride for k_trunc_rows
+__global__ void k_stage_rows(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                             const double *__restrict__ a, int mc, int *__restrict__ cnt,
+                             int *__restrict__ rj, double *__restrict__ ra)
+{
+   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+      const int s = ia[i], c = ia[i + 1] - s;
+      cnt[i] = c;
+      for (int k = 0; k < c; k++) {
+         rj[(size_t) i * mc + k] = ja[s + k];
+         ra[(size_t) i * mc + k] = a[s + k];
+      }
+   }
+}
+
+void truncate_rows(Csr &P, double trunc_factor, int max_elmts)
+{
+   const int n = P.nrows;
+   if (n == 0) { return; }
+   const int mx = max_row_len(P);
+   if (mx > 2 * XF_MC) {
+      fail(HYPRE_ERROR_GENERIC, "interpolation rows longer than 64 entries cannot be truncated");
+   }
+   const int mc = mx <= XF_MC / 2 ? XF_MC / 2 : (mx <= XF_MC ? XF_MC : 2 * XF_MC);
+   const int stride = max_elmts > 0 ? std::min(max_elmts, mc) : mc;
+   int *cnt = dalloc_n<int>((size_t) n);
+   int *rj = dalloc_n<int>((size_t) n * mc);
+   double *ra = dalloc_n<double>((size_t) n * mc);
+   int *ocnt = dalloc_n<int>((size_t) n + 1);
+   int *oj = dalloc_n<int>((size_t) n * stride);
+   double *oa = dalloc_n<double>((size_t) n * stride);
+   const int nb = std::max(1, std::min((n + 255) / 256, 8192));
+   k_stage_rows<<<nb, 256, 0, stream()>>>(n, P.ia, P.ja, P.a, mc, cnt, rj, ra);
+   const int nbt = (int) std::max<long long>(1, std::min<long long>((n + TR_ROWS - 1) / TR_ROWS, 16384));
+   if (mc == XF_MC / 2) {
+      k_trunc_rows<XF_MC / 2><<<nbt, TR_ROWS, 0, stream()>>>(n, nullptr, cnt, rj, ra, trunc_factor,
+                                                             max_elmts, stride, ocnt, oj, oa);
+   } else if (mc == XF_MC) {
+      k_trunc_rows<XF_MC><<<nbt, TR_ROWS, 0, stream()>>>(n, nullptr, cnt, rj, ra, trunc_factor,
+                                                         max_elmts, stride, ocnt, oj, oa);
+   } else {
+      k_trunc_rows<2 * XF_MC><<<nbt, TR_ROWS, 0, stream()>>>(n, nullptr, cnt, rj, ra, trunc_factor,
+                                                             max_elmts, stride, ocnt, oj, oa);
+   }
+   HX_LAUNCH_CHECK();
+   dfree(rj);
+   dfree(ra);
+   dfree(cnt);
+   const int nc = P.ncols;
+   P.free_all();
+   P.nrows = n;
+   P.ncols = nc;
+   P.ia = dalloc_n<int>((size_t) n + 1);
+   P.nnz = exclusive_scan(ocnt, P.ia, n);
+   P.ja = dalloc_n<int>((size_t) std::max<long long>(P.nnz, 1));
+   P.a = dalloc_n<double>((size_t) std::max<long long>(P.nnz, 1));
+   k_compact_rows<<<nb, 256, 0, stream()>>>(n, nullptr, ocnt, P.ia, oj, oa, stride, P.ja, P.a);
+   HX_LAUNCH_CHECK();
+   dfree(oj);
+   dfree(oa);
+   dfree(ocnt);
 }
 
 // ================================================================== transpose

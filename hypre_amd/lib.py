@@ -95,6 +95,11 @@ SIGNATURES = {
     "HYPRE_BoomerAMGSetRelaxWt": (I, [P, D]),
     "HYPRE_BoomerAMGSetPrintLevel": (I, [P, I]),
     "HYPRE_BoomerAMGSetCycleType": (I, [P, I]),
+    "HYPRE_BoomerAMGSetAggNumLevels": (I, [P, I]),
+    "HYPRE_BoomerAMGSetAggInterpType": (I, [P, I]),
+    "HYPRE_BoomerAMGSetAggTruncFactor": (I, [P, D]),
+    "HYPRE_BoomerAMGSetAggPMaxElmts": (I, [P, I]),
+    "HYPRE_BoomerAMGSetNumPaths": (I, [P, I]),
     "HYPRE_BoomerAMGGetNumIterations": (I, [P, IP]),
     "HYPRE_BoomerAMGGetFinalRelativeResidualNorm": (I, [P, DP]),
     "HYPRE_MI355X_BoomerAMGGetNumLevels": (I, [P, IP]),

@@ -20,6 +20,7 @@
 #include "amg_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -1956,8 +1957,12 @@ orc_amg *orc_amg_setup(const orc_csr *A0, const orc_amg_params *prm)
       }
       if (aggressive)
       {
-         orc_mod_multipass(A, &H->S[level], H->CF[level], prm->agg_trunc_factor,
-                           prm->agg_P_max_elmts, &H->P[level]);
+         if (orc_mod_multipass(A, &H->S[level], H->CF[level], prm->agg_trunc_factor,
+                               prm->agg_P_max_elmts, &H->P[level]) != 0)
+         {
+            fprintf(stderr, "oracle: multipass interpolation needs more than 9 passes\n");
+            abort();
+         }
       }
       else
       {
