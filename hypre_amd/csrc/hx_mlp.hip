@@ -9,13 +9,19 @@
 // 16x16 output tile); the bias/column sums and the loss are reduced in a
 // fixed order, so a training run is bit-reproducible.
 //
-// Output y of level l: theta = 0.1 + 0.8 y[3l], omega = 0.6 + 0.8 y[3l+1],
-// sweeps = 1 + floor(2 y[3l+2] + 0.5).
+// Two readings of the output, chosen by the caller:
+// * per-level settings (nout = 3 L): theta = 0.1 + 0.8 y[3l],
+//   omega = 0.6 + 0.8 y[3l+1], sweeps = 1 + floor(2 y[3l+2] + 0.5);
+// * a cost model over a grid of G settings (nout = 2 G, grid set with
+//   HYPRE_MI355X_ParamModelSetGrid): y[g] predicts the time of setting g on
+//   the fine level relative to the defaults, y[G + g] the same for the coarse
+//   levels; the predicted fastest setting of each group is applied.
 #include "hx_internal.h"
 #include "hx_objects.h"
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 struct hx_param_model_struct
 {
@@ -31,9 +37,14 @@ struct hx_param_model_struct
    float *Y = nullptr, *T = nullptr, *D3 = nullptr, *D2 = nullptr, *D1 = nullptr, *DH = nullptr;
    float *loss = nullptr;        // device scalar
    int    oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0, oW3 = 0, ob3 = 0;
+   // settings of the cost-model reading (empty: per-level reading)
+   std::vector<double> g_theta, g_omega;
+   std::vector<int>    g_sweeps;
 };
 
 namespace hx {
+
+constexpr int kModelLevels = 25;   // levels a cost model sets (hypre's max_levels default)
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -441,13 +452,41 @@ HYPRE_Int HYPRE_MI355X_MatrixFeatures(HYPRE_ParCSRMatrix A, HYPRE_Real *features
    });
 }
 
+HYPRE_Int HYPRE_MI355X_ParamModelSetGrid(HYPRE_MI355X_ParamModel model, HYPRE_Int ng,
+                                         const HYPRE_Real *theta, const HYPRE_Real *omega,
+                                         const HYPRE_Int *sweeps)
+{
+   return guard([&] {
+      auto *m = as_model(model);
+      if (ng == 0) {
+         m->g_theta.clear();
+         m->g_omega.clear();
+         m->g_sweeps.clear();
+         return;
+      }
+      if (ng < 0 || 2 * ng != m->nout || !theta || !omega || !sweeps) {
+         fail(HYPRE_ERROR_ARG, "grid size must be half the model's outputs");
+      }
+      for (int g = 0; g < ng; g++) {
+         if (!(theta[g] >= 0.0 && theta[g] < 1.0) || !(omega[g] > 0.0) || sweeps[g] < 1) {
+            fail(HYPRE_ERROR_ARG, "grid setting out of range");
+         }
+      }
+      m->g_theta.assign(theta, theta + ng);
+      m->g_omega.assign(omega, omega + ng);
+      m->g_sweeps.assign(sweeps, sweeps + ng);
+   });
+}
+
 HYPRE_Int HYPRE_MI355X_BoomerAMGSetParamModel(HYPRE_Solver solver, HYPRE_MI355X_ParamModel model,
                                               HYPRE_ParCSRMatrix A)
 {
    return guard([&] {
       auto *m = as_model(model);
       if (!solver || !A) { fail(HYPRE_ERROR_ARG, "null argument"); }
-      if (m->nin != 8 || m->nout % 3 != 0) { fail(HYPRE_ERROR_ARG, "model shape is not 8 -> 3L"); }
+      if (m->nin != 8 || (m->g_theta.empty() && m->nout % 3 != 0)) {
+         fail(HYPRE_ERROR_ARG, "model shape is not 8 -> 3L or 8 -> 2G with a grid");
+      }
       double f[8];
       matrix_features(A->A,f);
       float xf[8];
@@ -459,6 +498,19 @@ HYPRE_Int HYPRE_MI355X_BoomerAMGSetParamModel(HYPRE_Solver solver, HYPRE_MI355X_
       k_sigmoid<<<1, 256, 0, stream()>>>(m->nout, m->Z3, m->Y);
       HX_LAUNCH_CHECK();
       d2h(y.data(), m->Y, sizeof(float) * m->nout);
+      if (!m->g_theta.empty()) {
+         // cost model: the first predicted minimum of each group
+         const int G = (int) m->g_theta.size();
+         const int gf = (int) (std::min_element(y.begin(), y.begin() + G) - y.begin());
+         const int gc = (int) (std::min_element(y.begin() + G, y.begin() + 2 * G) - y.begin()) - G;
+         for (int l = 0; l < kModelLevels; l++) {
+            const int g = (l == 0) ? gf : gc;
+            HYPRE_MI355X_BoomerAMGSetLevelStrongThreshold(solver, l, m->g_theta[g]);
+            HYPRE_MI355X_BoomerAMGSetLevelRelaxWt(solver, l, m->g_omega[g]);
+            HYPRE_MI355X_BoomerAMGSetLevelNumSweeps(solver, l, m->g_sweeps[g]);
+         }
+         return;
+      }
       const int L = m->nout / 3;
       for (int l = 0; l < L; l++) {
          const double theta = 0.1 + 0.8 * (double) y[3 * l];

@@ -325,7 +325,10 @@ static int grid_for_groups(int n, int rw)
 // row's products in storage order, t = ((0 + a_0 x_0) + a_1 x_1) + ..., the
 // order of hypre's host CSR matvec (src/seq_mv/csr_matvec.c:22) and of the
 // windowed CSR kernels above, so both formats give the same bits.
-constexpr int ELL_U = 9;
+#ifndef HX_ELL_U
+#define HX_ELL_U 8
+#endif
+constexpr int ELL_U = HX_ELL_U;
 
 template <int U, typename V, typename X, class Epi>
 __device__ __forceinline__ void ell_slice(int sl, int lane, int n, const int *__restrict__ ia,
@@ -334,7 +337,8 @@ __device__ __forceinline__ void ell_slice(int sl, int lane, int n, const int *__
 {
    const int r = sl * 64 + lane;
    const int w0 = sw[sl], w = sw[sl + 1] - w0;
-   const size_t base = (size_t) w0 * 64 + lane;
+   const int *__restrict__ ejs = ej + (size_t) w0 * 64 + lane;
+   const V *__restrict__ eas = ea + (size_t) w0 * 64 + lane;
    int len = 0;
    if (r < n) { len = ia[r + 1] - ia[r]; }
    X t = 0;
@@ -345,15 +349,18 @@ __device__ __forceinline__ void ell_slice(int sl, int lane, int n, const int *__
 #pragma unroll
       for (int u = 0; u < U; u++) {
          // steps past the slice width re-read its last step (in bounds, masked)
-         const size_t q = base + (size_t) min(k0 + u, w - 1) * 64;
-         jj[u] = __builtin_nontemporal_load(ej + q);
-         vv[u] = __builtin_nontemporal_load(ea + q);
+         const int k = min(k0 + u, w - 1) * 64;
+         jj[u] = __builtin_nontemporal_load(ejs + k);
+         vv[u] = __builtin_nontemporal_load(eas + k);
       }
 #pragma unroll
       for (int u = 0; u < U; u++) { xx[u] = x[(k0 + u < len) ? jj[u] : 0]; }
+      // every step's product is formed and selected, so no load is moved
+      // into a branch (and waited for on its own)
 #pragma unroll
       for (int u = 0; u < U; u++) {
-         if (k0 + u < len) { t += (X) vv[u] * xx[u]; }
+         const X s = t + (X) vv[u] * xx[u];
+         t = (k0 + u < len) ? s : t;
       }
    }
    if (r < n) { epi(r, t); }
@@ -451,29 +458,68 @@ __global__ void k_ell_width(int n, int ns, const int *__restrict__ ia, int *__re
 }
 
 // one wave per slice, lane = row: entry k of row r goes to sw[s]*64 + k*64 +
-// lane; padding holds the row's own index and a zero value (never used)
-__global__ __launch_bounds__(BS) void k_ell_fill(int n, int ns, const int *__restrict__ ia,
+// lane; padding holds the row's own index and a zero value (never used).
+// The slice's CSR entries are one contiguous range; it is staged through LDS
+// in chunks with coalesced loads, and each lane then writes its row's entries
+// of the chunk (for rows that start inside the chunk, entry j of every row is
+// written by the same instruction: one contiguous 64-word step).
+constexpr int EF_CH = 1024;
+constexpr int EF_U = 4;
+
+__global__ __launch_bounds__(64) void k_ell_fill(int n, int ns, const int *__restrict__ ia,
                                                  const int *__restrict__ ja,
                                                  const double *__restrict__ a,
                                                  const int *__restrict__ sw, int *__restrict__ ej,
                                                  double *__restrict__ ea)
 {
-   const int sl = (int) ((blockIdx.x * (size_t) BS + threadIdx.x) >> 6);
-   const int lane = threadIdx.x & 63;
-   if (sl >= ns) { return; }
-   const int r = sl * 64 + lane;
-   const int w0 = sw[sl], w = sw[sl + 1] - w0;
-   const size_t base = (size_t) w0 * 64 + lane;
-   int rs = 0, len = 0;
-   if (r < n) {
-      rs = ia[r];
-      len = ia[r + 1] - rs;
-   }
-   const int pad_col = (r < n) ? r : 0;
-   for (int k = 0; k < w; k++) {
-      const bool in = k < len;
-      ej[base + (size_t) k * 64] = in ? ja[rs + k] : pad_col;
-      ea[base + (size_t) k * 64] = in ? a[rs + k] : 0.0;
+   __shared__ int sj[EF_CH];
+   __shared__ double sa[EF_CH];
+   const int lane = threadIdx.x;
+   for (int sl = blockIdx.x; sl < ns; sl += gridDim.x) {
+      const int r = sl * 64 + lane;
+      const int w0 = sw[sl], w = sw[sl + 1] - w0;
+      int *ejs = ej + (size_t) w0 * 64 + lane;
+      double *eas = ea + (size_t) w0 * 64 + lane;
+      int rs = 0, re = 0;
+      if (r < n) {
+         rs = ia[r];
+         re = ia[r + 1];
+      }
+      const int pad_col = (r < n) ? r : 0;
+      for (int k = re - rs; k < w; k++) {
+         ejs[(size_t) k * 64] = pad_col;
+         eas[(size_t) k * 64] = 0.0;
+      }
+      const int s0 = ia[sl * 64], s1 = ia[min(n, sl * 64 + 64)];
+      for (int c0 = s0; c0 < s1; c0 += EF_CH) {
+         const int c1 = min(s1, c0 + EF_CH);
+         for (int q0 = c0; q0 < c1; q0 += 64 * EF_U) {
+            int jv[EF_U];
+            double av[EF_U];
+#pragma unroll
+            for (int u = 0; u < EF_U; u++) {
+               const int q = min(q0 + u * 64 + lane, c1 - 1);
+               jv[u] = __builtin_nontemporal_load(ja + q);
+               av[u] = __builtin_nontemporal_load(a + q);
+            }
+#pragma unroll
+            for (int u = 0; u < EF_U; u++) {
+               const int q = q0 + u * 64 + lane;
+               if (q < c1) {
+                  sj[q - c0] = jv[u];
+                  sa[q - c0] = av[u];
+               }
+            }
+         }
+         __syncthreads();
+         const int q1 = min(re, c1);
+         for (int q = max(rs, c0); q < q1; q++) {
+            const size_t k = (size_t) (q - rs) * 64;
+            ejs[k] = sj[q - c0];
+            eas[k] = sa[q - c0];
+         }
+         __syncthreads();
+      }
    }
 }
 
@@ -496,9 +542,8 @@ bool ell_build(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32)
    E.sw = w;
    E.j = dalloc_n<int>((size_t) stored);
    E.a = dalloc_n<double>((size_t) stored);
-   const long long threads = 64LL * ns;
-   k_ell_fill<<<(unsigned) ((threads + BS - 1) / BS), BS, 0, stream()>>>(A.nrows, ns, A.ia, A.ja, A.a,
-                                                                         E.sw, E.j, E.a);
+   k_ell_fill<<<std::min(ns, 65536), 64, 0, stream()>>>(A.nrows, ns, A.ia, A.ja, A.a, E.sw, E.j,
+                                                         E.a);
    HX_LAUNCH_CHECK();
    if (fp32) {
       E.a32 = dalloc_n<float>((size_t) stored);

@@ -1,20 +1,26 @@
 """Learned per-level BoomerAMG parameters (no counterpart in hypre).
 
-A small perceptron on the device (8 matrix features -> 64 -> 64 -> 3 x 25
-outputs, FP32, see hypre_amd/csrc/hx_mlp.hip) predicts, for every level, the
-strong threshold theta, the l1-Jacobi weight omega and the number of sweeps.
-The features (HYPRE_MI355X_MatrixFeatures) are the row-length distribution,
-off-diagonal anisotropy ratios, the strong fraction and a coefficient-jump
-histogram.
+A small perceptron on the device (8 matrix features -> 64 -> 64 -> 48
+outputs, FP32, see hypre_amd/csrc/hx_mlp.hip) is a cost model: for each of
+the 24 settings (strong threshold theta, l1-Jacobi weight omega, sweeps) of
+GRID it predicts the setup+solve time relative to the defaults when the
+setting is used on the fine level, and when it is used on the coarse
+levels.  The solver takes the predicted fastest setting of each group
+(level 0, levels 1-24).  The features (HYPRE_MI355X_MatrixFeatures) are the
+row-length distribution, off-diagonal anisotropy ratios, the strong fraction
+and a coefficient-jump histogram.
 
 Training data come from sweeps of the plain method itself: for each training
 matrix, AMG-PCG is timed (setup + solve to 1e-8, best of two runs) for every
-setting of a small grid on the fine level with the coarse levels at the
-defaults, then for every setting on the coarse levels with the fine level at
-its best; the fastest pair is the target (level 0 from the first, levels 1-24
-from the second).  The guard `apply_with_fallback` keeps the defaults whenever
-the predicted settings reach the tolerance more slowly (less convergence per
-second) on the matrix at hand.
+setting of GRID on the fine level with the coarse levels at the defaults,
+then for every setting on the coarse levels with the fine level at its best;
+the targets are those times over the defaults' (the measured convergence per
+second of each setting).  Learning the whole table, rather than only its
+fastest entry, averages the timing noise over similar matrices: where the
+features carry no information the model reduces to the best constant rule,
+and where they do it can beat it.  The guard `apply_with_fallback` keeps the
+defaults whenever the predicted settings reach the tolerance more slowly
+(less convergence per second) on the matrix at hand.
 """
 from __future__ import annotations
 
@@ -31,12 +37,17 @@ from .api import call
 from .lib import lib
 
 NIN, NH1, NH2, LEVELS = 8, 64, 64, 25
-NOUT = 3 * LEVELS
 WEIGHTS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned_params.json")
 DEFAULT = (0.25, 1.0, 1)
 
 # settings of one level group (the fine level, or all coarse levels)
-GRID = tuple((th, om, sw) for th in (0.25, 0.4, 0.55) for om in (0.8, 1.0, 1.2) for sw in (1, 2))
+GRID = tuple((th, om, sw) for th in (0.25, 0.4, 0.55) for om in (1.0, 1.2, 1.4, 1.6)
+             for sw in (1, 2))
+NOUT = 2 * len(GRID)          # cost model: fine-level group, then coarse-level group
+NOUT_LEVELS = 3 * LEVELS      # per-level reading (theta, omega, sweeps of each level)
+# relative times r map to model targets (r - 0.5), clipped to (0, 1); a
+# setting that did not converge is the top of the range
+RATIO_LO, RATIO_HI = 0.5, 1.5
 
 
 def _fp(a: np.ndarray):
@@ -46,7 +57,8 @@ def _fp(a: np.ndarray):
 class ParamModel:
     """HYPRE_MI355X_ParamModel* through the C ABI."""
 
-    def __init__(self, nin=NIN, nh1=NH1, nh2=NH2, nout=NOUT, params=None, seed=1):
+    def __init__(self, nin=NIN, nh1=NH1, nh2=NH2, nout=NOUT_LEVELS, params=None, seed=1,
+                 grid=None):
         h = C.c_void_p()
         call("HYPRE_MI355X_ParamModelCreate", nin, nh1, nh2, nout, C.byref(h))
         self.h = h
@@ -55,6 +67,28 @@ class ParamModel:
         call("HYPRE_MI355X_ParamModelNumParams", h, C.byref(npar))
         self.np = npar.value
         self.set_params(init_params(self.dims, seed) if params is None else params)
+        self.grid = None
+        if grid is not None:
+            self.set_grid(grid)
+
+    def set_grid(self, grid):
+        """Cost-model reading over `grid` (nout = 2 len(grid))."""
+        grid = [tuple(g) for g in grid]
+        th = np.array([g[0] for g in grid], np.float64)
+        om = np.array([g[1] for g in grid], np.float64)
+        sw = np.array([g[2] for g in grid], np.int32)
+        call("HYPRE_MI355X_ParamModelSetGrid", self.h, len(grid),
+             th.ctypes.data_as(C.POINTER(C.c_double)), om.ctypes.data_as(C.POINTER(C.c_double)),
+             sw.ctypes.data_as(C.POINTER(C.c_int)))
+        self.grid = grid
+
+    def predict(self, A: hx.ParCSRMatrix):
+        """(fine setting, coarse setting) the cost model picks for A."""
+        return self.predict_from_features(matrix_features(A))
+
+    def predict_from_features(self, f):
+        y = self.forward(np.asarray(f, np.float32)[None, :])[0]
+        return decode_costs(y, self.grid)
 
     def set_params(self, p):
         p = np.ascontiguousarray(p, dtype=np.float32)
@@ -94,13 +128,14 @@ class ParamModel:
     def save(self, path=WEIGHTS_FILE, meta=None):
         with open(path, "w") as f:
             json.dump({"dims": list(self.dims), "params": [float(v) for v in self.get_params()],
+                       "grid": [list(g) for g in self.grid] if self.grid else None,
                        "meta": meta or {}}, f)
 
     @classmethod
     def load(cls, path=WEIGHTS_FILE) -> "ParamModel":
         with open(path) as f:
             d = json.load(f)
-        return cls(*d["dims"], params=np.asarray(d["params"], np.float32))
+        return cls(*d["dims"], params=np.asarray(d["params"], np.float32), grid=d.get("grid"))
 
     def destroy(self):
         if self.h:
@@ -134,6 +169,20 @@ def matrix_features(A: hx.ParCSRMatrix) -> np.ndarray:
 def per_level(fine, coarse, levels=LEVELS):
     """Per-level settings: `fine` on level 0, `coarse` on levels 1.."""
     return [tuple(fine)] + [tuple(coarse)] * (levels - 1)
+
+
+def encode_costs(fine_ratios, coarse_ratios) -> np.ndarray:
+    """Cost-model targets from the relative times of the GRID settings on the
+    fine level and on the coarse levels (None: did not converge)."""
+    r = [RATIO_HI if v is None else v for v in list(fine_ratios) + list(coarse_ratios)]
+    return np.clip(np.asarray(r, np.float64) - RATIO_LO, 1e-3, 1 - 1e-3).astype(np.float32)
+
+
+def decode_costs(y, grid=GRID):
+    """The first predicted minimum of each group: (fine setting, coarse setting)."""
+    G = len(grid)
+    y = np.asarray(y)
+    return tuple(grid[int(np.argmin(y[:G]))]), tuple(grid[int(np.argmin(y[G:2 * G]))])
 
 
 def encode(level_params) -> np.ndarray:
@@ -279,9 +328,9 @@ def train_main(argv=None):
     """python -m hypre_amd.learned [--aniso-n 128] [--aniso-count 64]
                                    [--jump-n 96] [--jump-count 32] [--steps 4000]
 
-    Sweeps the per-level grid on the training problems (anisotropic seeds
-    1..aniso-count, jumping-coefficient seeds 1..jump-count), trains the
-    device model on the fastest settings, picks the best constant rule on the
+    Sweeps GRID on the training problems (anisotropic seeds 1..aniso-count,
+    jumping-coefficient seeds 1..jump-count), trains the device cost model on
+    the relative times of every setting, picks the best constant rule on the
     same problems, and writes WEIGHTS_FILE."""
     import argparse
     import time
@@ -304,7 +353,11 @@ def train_main(argv=None):
         f = matrix_features(A)
         (fine, coarse), bst, base, table = best_per_level(A, b)
         X.append(f.astype(np.float32))
-        T.append(encode(per_level(fine, coarse)))
+        fr = [None if table["fine"][str(g)] is None else table["fine"][str(g)] / base.seconds
+              for g in GRID]
+        cr = [None if table["coarse"][str(g)] is None else table["coarse"][str(g)] / base.seconds
+              for g in GRID]
+        T.append(encode_costs(fr, cr))
         rows.append({"kind": kind, "n": N, "seed": s, **info, "features": [round(v, 6) for v in f],
                      "fine": fine, "coarse": coarse, "seconds_best": round(bst.seconds, 6),
                      "seconds_default": round(base.seconds, 6), "its_best": bst.iterations,
@@ -314,8 +367,9 @@ def train_main(argv=None):
         b.destroy()
     X = np.stack(X)
     T = np.stack(T)
-    model = ParamModel(seed=1)
+    model = ParamModel(nout=NOUT, seed=1, grid=GRID)
     hist = model.train(X, T, args.steps, args.lr)
+    picks = [model.predict_from_features(x) for x in X]
     # the best constant rule on the training problems (mean time over default)
     cands = constant_rules(rows)
     scores = []
@@ -338,6 +392,9 @@ def train_main(argv=None):
             "sweep_seconds": round(time.perf_counter() - t0, 1),
             "mean_time_best_over_default":
                 float(np.mean([r["seconds_best"] / r["seconds_default"] for r in rows])),
+            "reading": "cost model: relative time of each GRID setting on the fine level "
+                       "and on the coarse levels; the predicted fastest setting of each is used",
+            "training_picks": [[list(a), list(b)] for a, b in picks],
             "constant_rule": best_rule, "constant_rule_candidates":
                 [{"rule": c, "mean_time_over_default": sc} for c, sc in zip(cands, scores)],
             "rows": rows}

@@ -158,6 +158,7 @@ SIGNATURES = {
     "HYPRE_MI355X_ParamModelTrain": (I, [P, I, FP, FP, I, C.c_float, FP]),
     "HYPRE_MI355X_MatrixFeatures": (I, [P, DP]),
     "HYPRE_MI355X_BoomerAMGSetParamModel": (I, [P, P, P]),
+    "HYPRE_MI355X_ParamModelSetGrid": (I, [P, I, DP, DP, IP]),
     "HYPRE_MI355X_SetKernelTiming": (I, [I]),
     "HYPRE_MI355X_SetKernelTimingClass": (I, [C.c_char_p]),
     "HYPRE_MI355X_ResetKernelTiming": (I, []),

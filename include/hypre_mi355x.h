@@ -188,7 +188,16 @@ HYPRE_Int HYPRE_MI355X_ParamModelTrain(HYPRE_MI355X_ParamModel model, HYPRE_Int 
  * fraction at theta 0.25; a coefficient-jump histogram (fractions of rows
  * with a neighbour whose |diagonal| differs by a factor >= 10 and >= 1000) */
 HYPRE_Int HYPRE_MI355X_MatrixFeatures(HYPRE_ParCSRMatrix A, HYPRE_Real *features);
-/* features of A -> model -> per-level theta, omega, sweeps of the solver */
+/* cost-model reading of a model with nout = 2 ng: output g predicts the
+ * time of setting g (theta[g], omega[g], sweeps[g]) on the fine level
+ * relative to the defaults, output ng + g the same on the coarse levels;
+ * ng = 0 returns to the per-level reading */
+HYPRE_Int HYPRE_MI355X_ParamModelSetGrid(HYPRE_MI355X_ParamModel model, HYPRE_Int ng,
+                                         const HYPRE_Real *theta, const HYPRE_Real *omega,
+                                         const HYPRE_Int *sweeps);
+/* features of A -> model -> per-level theta, omega, sweeps of the solver
+ * (with a grid: the predicted fastest setting on level 0 and on levels
+ * 1..24) */
 HYPRE_Int HYPRE_MI355X_BoomerAMGSetParamModel(HYPRE_Solver solver, HYPRE_MI355X_ParamModel model,
                                               HYPRE_ParCSRMatrix A);
 

@@ -116,6 +116,36 @@ def test_model_sets_per_level_parameters(hx, L, PM):
     m.destroy()
 
 
+def test_cost_model_applies_its_predicted_fastest_settings(hx, L, PM):
+    """With a grid, BoomerAMGSetParamModel applies the first predicted
+    minimum of each output group (level 0, levels 1..24): the V-cycle equals,
+    bit for bit, the one built from those settings set level by level."""
+    say("cost model apply")
+    dims = (8, 64, 64, L.NOUT)
+    A = hx.ParCSRMatrix.aniso7(20, 0.05, 0.4)
+    n, _ = A.dims()
+    f = 2 * np.random.default_rng(4).uniform(size=n) - 1
+    for seed in (11, 12, 13):
+        p = PM.init_params(dims, seed=seed).astype(np.float32)
+        m = L.ParamModel(*dims, params=p, grid=L.GRID)
+        y = m.forward(L.matrix_features(A)[None, :].astype(np.float32))[0]
+        fine, coarse = L.decode_costs(y)
+        assert (fine, coarse) == m.predict(A)
+        a1 = hx.BoomerAMG(keep_setup_data=1, max_iter=1, tol=0.0)
+        m.apply(a1, A)
+        a1.setup(A)
+        a2 = hx.BoomerAMG(keep_setup_data=1, max_iter=1, tol=0.0)
+        th, om, sw = zip(*L.per_level(fine, coarse))
+        a2.set_level_params(theta=th, omega=om, sweeps=sw)
+        a2.setup(A)
+        assert a1.num_levels() == a2.num_levels()
+        u1, u2 = a1.cycle_host(f), a2.cycle_host(f)
+        assert np.array_equal(u1.view(np.int64), u2.view(np.int64)), (fine, coarse)
+        for a in (a1, a2):
+            a.destroy()
+        m.destroy()
+
+
 HELD_OUT = [("aniso", 128, s) for s in (1001, 1002, 1003, 1004, 1005, 1006)] + \
            [("jump", 96, s) for s in (2001, 2002, 2003)]
 
@@ -152,7 +182,10 @@ def test_learned_parameters_on_held_out_problems(hx, L):
         b.destroy()
     say("mean learned", np.mean(r_learned), "mean constant rule", np.mean(r_const))
     assert max(r_learned) < 1.05, r_learned          # timing noise only, never a real loss
-    assert np.mean(r_learned) < 0.97, r_learned
+    # a gain over the defaults well beyond the ~1% noise of a best-of-two timing
+    assert np.mean(r_learned) < 0.985, r_learned
     assert "learned" in picks
-    # the per-problem prediction against one setting for every problem
-    assert np.mean(r_learned) <= np.mean(r_const) + 0.02, (r_learned, r_const)
+    # the per-problem prediction against the best single setting of the
+    # training sweep: on these families the fastest setting hardly varies
+    # between problems, so the network has to match that rule (within noise)
+    assert np.mean(r_learned) <= np.mean(r_const) + 0.015, (r_learned, r_const)
