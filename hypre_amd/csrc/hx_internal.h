@@ -65,6 +65,9 @@ inline int guard(F &&f)
 
 // ------------------------------------------------------------------ runtime
 void        ensure_init();
+// takes the address of the fixed scratch array of the allocation-free entry
+// points on the current device (called once per device by ensure_init)
+void        noalloc_attach_device();
 hipStream_t stream();
 void        sync();                       // hipStreamSynchronize(stream())
 void       *dalloc(size_t bytes);         // caching device allocator

@@ -37,6 +37,7 @@ void ensure_init()
    HX_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
    HX_HIP(hipMalloc(&g_flags, 1024 * sizeof(int)));
    HX_HIP(hipMemset(g_flags, 0, 1024 * sizeof(int)));
+   noalloc_attach_device();
    g_inited = true;
 }
 

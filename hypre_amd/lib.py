@@ -68,6 +68,7 @@ SIGNATURES = {
     "HYPRE_ParVectorSetConstantValues": (I, [P, D]),
     "HYPRE_ParVectorScale": (I, [D, P]),
     "HYPRE_ParVectorInnerProd": (I, [P, P, DP]),
+    "HYPRE_MI355X_ParVectorInnerProdNoAlloc": (I, [P, P, DP]),
     "HYPRE_ParVectorCopy": (I, [P, P]),
     "HYPRE_ParVectorAxpy": (I, [D, P, P]),
     "GenerateLaplacian": (P, [I, I, I, I, I, I, I, I, I, I, DP]),

@@ -212,6 +212,20 @@ HYPRE_Int HYPRE_MI355X_ResetKernelTiming(void);
 HYPRE_Int HYPRE_MI355X_GetKernelTiming(const char *name, HYPRE_Real *total_ms,
                                        HYPRE_Int *launches, HYPRE_Real *bytes);
 
+/* ---- allocation-free variant (no counterpart in hypre) ------------------
+ * For callers that cannot allocate device or pinned memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * HYPRE_ParVectorInnerProd (src/parcsr_mv/HYPRE_parcsr_vector.c) with its
+ * per-block partial sums and its result kept in a fixed 16.6 KB device array
+ * of the library (one per device, taken at HYPRE_Initialize) instead of
+ * memory allocated on first use, and the result copied straight into *prod.
+ * Same bits as HYPRE_ParVectorInnerProd.  Any length a HYPRE_Int holds is
+ * accepted (a call never needs more than 2048 partial sums); vectors of
+ * different lengths return HYPRE_ERROR_ARG.  Calls that overlap on one device
+ * share that array and race; calls from one host thread never overlap. */
+HYPRE_Int HYPRE_MI355X_ParVectorInnerProdNoAlloc(HYPRE_ParVector x, HYPRE_ParVector y,
+                                                 HYPRE_Real *prod);
+
 #ifdef __cplusplus
 }
 #endif
