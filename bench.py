@@ -195,6 +195,11 @@ def main():
     ap.add_argument("--hgs-parts", type=int, default=0,
                     help="fixed number of hybrid Gauss-Seidel partitions on every level "
                          "(the reference's thread count); 0 = automatic")
+    ap.add_argument("--coarsen", type=int, default=8, choices=[8, 10],
+                    help="coarsening: 8 PMIS (default), 10 HMIS (host first pass)")
+    ap.add_argument("--agg-levels", type=int, default=0,
+                    help="aggressive coarsening on this many levels (PMIS, modified "
+                         "multipass interpolation)")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: one setup, 3 PCG iterations, "
                          "then 3 copies of a 1 GiB vector (the bytes reference of the counters)")
@@ -216,9 +221,11 @@ def main():
     b.set_random(prob["rhs_seed"])        # 2*hypre_Rand()-1 from hypre_SeedRand(seed)
     x = hx.ParVector(n)
     down, up = (int(args.relax.split("/")[0]), int(args.relax.split("/")[-1]))
-    amg = hx.BoomerAMG(coarsen_type=8, interp_type=6, strong_threshold=0.25, max_row_sum=0.9,
-                       P_max_elmts=4, relax_type=down, num_sweeps=1, max_coarse_size=9,
-                       fp32_cycle=1 if args.fp32_cycle else 0)
+    amg = hx.BoomerAMG(coarsen_type=args.coarsen, interp_type=6, strong_threshold=0.25,
+                       max_row_sum=0.9, P_max_elmts=4, relax_type=down, num_sweeps=1,
+                       max_coarse_size=9, fp32_cycle=1 if args.fp32_cycle else 0)
+    if args.agg_levels:
+        amg.set(agg_num_levels=args.agg_levels, agg_interp_type=8)
     amg.set(relax_up=up, hybrid_partitions=(args.hgs_parts, args.hgs_rows))
     pcg = hx.PCG(amg, tol=1e-8, max_iter=3 if args.probe else 1000, two_norm=1)
     learned_info = None
@@ -307,7 +314,10 @@ def main():
         "dtype": "f32-cycle/f64" if args.fp32_cycle else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{prob['desc']} {N}^3 ({n} rows, {nnz} nnz), Dirichlet; PMIS, "
+            "workload": f"{prob['desc']} {N}^3 ({n} rows, {nnz} nnz), Dirichlet; "
+                        f"{'HMIS' if args.coarsen == 10 else 'PMIS'}, "
+                        + (f"{args.agg_levels} aggressive level(s) with modified multipass, "
+                           if args.agg_levels else "") +
                         f"ext+i (Pmx 4), theta 0.25, max_row_sum 0.9, {SMOOTHERS[args.relax]} V(1,1), "
                         "GE coarsest, "
                         f"two-norm PCG to 1e-8, x0 = 0, RHS uniform[-1,1] seed {prob['rhs_seed']}",
@@ -338,19 +348,25 @@ def main():
                                 "gbs": round(bytes_rlz / (ms_rlz * 1e-3) / 1e9, 1) if ms_rlz > 0 else None},
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_spmv_w<RW,double,double,DOT> (PCG s = A p on the fine level)",
+            "kernel": "k_spmv_ell<double,double,DOT> (PCG s = A p on the fine level; sliced ELL "
+                      "with 16-bit column offsets where a slice allows them)",
             "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "traffic": measured_traffic(N, args.problem),
+            # measured HBM bytes over the measured launch time (the packed
+            # column offsets move fewer bytes than the 12 B per nonzero that
+            # the algorithmic count assumes)
+            "traffic_gbs": (round(measured_traffic(N, args.problem) / (avg_ms * 1e-3) / 1e9, 1)
+                            if launches and measured_traffic(N, args.problem) else None),
             "algorithmic_bytes_per_launch": bytes_per,
             "avg_launch_ms": round(avg_ms, 5),
             "launches": launches,
         },
     }
     default_run = args.problem == "lap27" and N == 200 and args.relax == "18" and not args.learned \
-        and not args.fp32_cycle
+        and not args.fp32_cycle and args.coarsen == 8 and not args.agg_levels
     if default_run and not args.no_north_star:
         for h in (r, b, x):
             h.destroy()

@@ -104,12 +104,20 @@ struct TimedLaunch
 // consecutive rows; a slice of width w (its longest row) stores 64*w entries
 // column-major (entry k of its 64 rows are 64 consecutive words), each row's
 // entries in CSR storage order, padding masked by the CSR row length.
+// Columns of a slice whose step k spans at most 65535 in col - row (every
+// step of a stencil operator's interior and of most boundary slices) are
+// kept as 16-bit offsets: col = row + jb[k] + j16, jb[k] the step's least
+// col - row; the other slices read the 32-bit columns.
 struct Ell
 {
    int       nslices = 0;
    long long stored = 0;          // 64 * sum of slice widths
+   long long packed = 0;          // entries of slices with 16-bit columns
    int      *sw = nullptr;        // exclusive prefix of slice widths (nslices + 1)
-   int      *j = nullptr;
+   int      *j = nullptr;         // 32-bit columns (all slices)
+   unsigned short *j16 = nullptr; // 16-bit column offsets (slices with mode 1)
+   int      *jb = nullptr;        // per slice step: least col - row (sum of widths)
+   int      *mode = nullptr;      // per slice: 1 = 16-bit columns, 0 = 32-bit
    double   *a = nullptr;
    float    *a32 = nullptr;
    void      free_all();

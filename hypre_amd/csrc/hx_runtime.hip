@@ -426,9 +426,9 @@ void Csr::free_all()
 
 void Ell::free_all()
 {
-   dfree(sw); dfree(j); dfree(a); dfree(a32);
-   sw = j = nullptr; a = nullptr; a32 = nullptr;
-   nslices = 0; stored = 0;
+   dfree(sw); dfree(j); dfree(j16); dfree(jb); dfree(mode); dfree(a); dfree(a32);
+   sw = j = nullptr; j16 = nullptr; jb = mode = nullptr; a = nullptr; a32 = nullptr;
+   nslices = 0; stored = 0; packed = 0;
 }
 
 void Csr::alloc(int n, int nc, long long nz, bool values)

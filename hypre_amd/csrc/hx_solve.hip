@@ -12,6 +12,7 @@
 #include "hx_internal.h"
 
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
 
 namespace hx {
@@ -317,30 +318,34 @@ static int grid_for_groups(int n, int rw)
 // ---------------------------------------------------------------- sliced ELL
 // A wave owns a contiguous range of 64-row slices and takes them one at a
 // time, lane = row.  Step k of a slice loads entry k of its 64 rows: one
-// contiguous 256-byte index load and one 512-byte value load per wave
-// (non-temporal: the matrix is read once, x stays in L2), then the 64 x
-// gathers, which for a stencil operator are neighbours of 64 consecutive rows.
-// U steps are issued together (index and value loads, then the gathers, then
-// the adds), so each round trip to memory carries U steps.  Each lane adds its
-// row's products in storage order, t = ((0 + a_0 x_0) + a_1 x_1) + ..., the
-// order of hypre's host CSR matvec (src/seq_mv/csr_matvec.c:22) and of the
-// windowed CSR kernels above, so both formats give the same bits.
+// contiguous index load (128 bytes of 16-bit offsets, or 256 bytes of 32-bit
+// columns) and one 512-byte value load per wave (non-temporal: the matrix is
+// read once, x stays in L2), then the 64 x gathers, which for a stencil
+// operator are neighbours of 64 consecutive rows.  U steps are issued
+// together (index and value loads, then the gathers, then the adds), so each
+// round trip to memory carries U steps.  Each lane adds its row's products in
+// storage order, t = ((0 + a_0 x_0) + a_1 x_1) + ..., the order of hypre's
+// host CSR matvec (src/seq_mv/csr_matvec.c:22) and of the windowed CSR
+// kernels above, so both formats give the same bits.
 #ifndef HX_ELL_U
-#define HX_ELL_U 8
+#define HX_ELL_U 9
 #endif
 constexpr int ELL_U = HX_ELL_U;
 
-template <int U, typename V, typename X, class Epi>
-__device__ __forceinline__ void ell_slice(int sl, int lane, int n, const int *__restrict__ ia,
-                                          const int *__restrict__ sw, const int *__restrict__ ej,
-                                          const V *__restrict__ ea, const X *__restrict__ x, Epi epi)
+struct EllView
 {
-   const int r = sl * 64 + lane;
-   const int w0 = sw[sl], w = sw[sl + 1] - w0;
-   const int *__restrict__ ejs = ej + (size_t) w0 * 64 + lane;
-   const V *__restrict__ eas = ea + (size_t) w0 * 64 + lane;
-   int len = 0;
-   if (r < n) { len = ia[r + 1] - ia[r]; }
+   const int *sw, *mode, *jb, *j;
+   const unsigned short *j16;
+};
+
+static EllView ell_view(const Ell &E) { return EllView{E.sw, E.mode, E.jb, E.j, E.j16}; }
+
+template <int U, bool PACKED, typename V, typename X>
+__device__ __forceinline__ X ell_steps(int r, int w, int len, const int *__restrict__ ejs,
+                                       const unsigned short *__restrict__ e16s,
+                                       const int *__restrict__ jbs, const V *__restrict__ eas,
+                                       const X *__restrict__ x)
+{
    X t = 0;
    for (int k0 = 0; k0 < w; k0 += U) {
       int jj[U];
@@ -349,9 +354,13 @@ __device__ __forceinline__ void ell_slice(int sl, int lane, int n, const int *__
 #pragma unroll
       for (int u = 0; u < U; u++) {
          // steps past the slice width re-read its last step (in bounds, masked)
-         const int k = min(k0 + u, w - 1) * 64;
-         jj[u] = __builtin_nontemporal_load(ejs + k);
-         vv[u] = __builtin_nontemporal_load(eas + k);
+         const int ks = min(k0 + u, w - 1);
+         if (PACKED) {
+            jj[u] = r + jbs[ks] + (int) __builtin_nontemporal_load(e16s + ks * 64);
+         } else {
+            jj[u] = __builtin_nontemporal_load(ejs + ks * 64);
+         }
+         vv[u] = __builtin_nontemporal_load(eas + ks * 64);
       }
 #pragma unroll
       for (int u = 0; u < U; u++) { xx[u] = x[(k0 + u < len) ? jj[u] : 0]; }
@@ -362,6 +371,25 @@ __device__ __forceinline__ void ell_slice(int sl, int lane, int n, const int *__
          const X s = t + (X) vv[u] * xx[u];
          t = (k0 + u < len) ? s : t;
       }
+   }
+   return t;
+}
+
+template <int U, typename V, typename X, class Epi>
+__device__ __forceinline__ void ell_slice(int sl, int lane, int n, const int *__restrict__ ia,
+                                          const EllView &E, const V *__restrict__ ea,
+                                          const X *__restrict__ x, Epi epi)
+{
+   const int r = sl * 64 + lane;
+   const int w0 = E.sw[sl], w = E.sw[sl + 1] - w0;
+   const size_t base = (size_t) w0 * 64 + lane;
+   int len = 0;
+   if (r < n) { len = ia[r + 1] - ia[r]; }
+   X t;
+   if (E.mode[sl]) {
+      t = ell_steps<U, true, V, X>(r, w, len, nullptr, E.j16 + base, E.jb + w0, ea + base, x);
+   } else {
+      t = ell_steps<U, false, V, X>(r, w, len, E.j + base, nullptr, nullptr, ea + base, x);
    }
    if (r < n) { epi(r, t); }
 }
@@ -378,8 +406,7 @@ __device__ __forceinline__ void wave_slices(int ns, int blk, int wv, int nb, int
 
 template <typename V, typename X, bool DOT, bool ZR = false>
 __global__ __launch_bounds__(BS) void k_spmv_ell(int n, int ns, const int *__restrict__ ia,
-                                                 const int *__restrict__ sw,
-                                                 const int *__restrict__ ej,
+                                                 EllView E,
                                                  const V *__restrict__ ea, const X *__restrict__ x,
                                                  X alpha, X beta, const X *b, X *y,
                                                  double *__restrict__ part,
@@ -393,7 +420,7 @@ __global__ __launch_bounds__(BS) void k_spmv_ell(int n, int ns, const int *__res
    wave_slices(ns, blk, wv, nb, S0, S1);
    double acc = 0.0;
    for (int sl = S0; sl < S1; sl++) {
-      ell_slice<ELL_U, V, X>(sl, lane, n, ia, sw, ej, ea, x, [&](int row, X t) {
+      ell_slice<ELL_U, V, X>(sl, lane, n, ia, E, ea, x, [&](int row, X t) {
          const X v = (beta == (X) 0) ? alpha * t : alpha * t + beta * b[row];
          y[row] = v;
          if (DOT) { acc += (double) v * (double) x[row]; }
@@ -414,8 +441,7 @@ __global__ __launch_bounds__(BS) void k_spmv_ell(int n, int ns, const int *__res
 
 template <typename V, typename X>
 __global__ __launch_bounds__(BS) void k_relax_l1_ell(int n, int ns, const int *__restrict__ ia,
-                                                     const int *__restrict__ sw,
-                                                     const int *__restrict__ ej,
+                                                     EllView E,
                                                      const V *__restrict__ ea,
                                                      const X *__restrict__ f,
                                                      const X *__restrict__ u,
@@ -428,7 +454,7 @@ __global__ __launch_bounds__(BS) void k_relax_l1_ell(int n, int ns, const int *_
    int S0, S1;
    wave_slices(ns, blk, wv, nb, S0, S1);
    for (int sl = S0; sl < S1; sl++) {
-      ell_slice<ELL_U, V, X>(sl, lane, n, ia, sw, ej, ea, u, [&](int row, X t) {
+      ell_slice<ELL_U, V, X>(sl, lane, n, ia, E, ea, u, [&](int row, X t) {
          const X v = w * f[row] - w * t;
          uo[row] = u[row] + v / l1[row];
       });
@@ -523,6 +549,41 @@ __global__ __launch_bounds__(64) void k_ell_fill(int n, int ns, const int *__res
    }
 }
 
+// per slice: the least and the largest col - row of each step over the rows
+// that have an entry there; a slice whose steps all span at most 65535 stores
+// 16-bit offsets from its steps' least values (mode 1)
+__global__ __launch_bounds__(64) void k_ell_pack(int n, int ns, const int *__restrict__ ia,
+                                                 const int *__restrict__ sw,
+                                                 const int *__restrict__ ej,
+                                                 unsigned short *__restrict__ e16,
+                                                 int *__restrict__ jb, int *__restrict__ mode)
+{
+   const int lane = threadIdx.x;
+   for (int sl = blockIdx.x; sl < ns; sl += gridDim.x) {
+      const int r = sl * 64 + lane;
+      const int w0 = sw[sl], w = sw[sl + 1] - w0;
+      int len = 0;
+      if (r < n) { len = ia[r + 1] - ia[r]; }
+      const int *ejs = ej + (size_t) w0 * 64 + lane;
+      unsigned short *e16s = e16 + (size_t) w0 * 64 + lane;
+      bool ok = true;
+      for (int k = 0; k < w; k++) {
+         const bool in = k < len;
+         const int off = ejs[(size_t) k * 64] - r;   // both in [0, 2^31)
+         int lo = in ? off : INT_MAX, hi = in ? off : INT_MIN;
+         for (int d = 32; d > 0; d >>= 1) {
+            lo = min(lo, __shfl_xor(lo, d, 64));
+            hi = max(hi, __shfl_xor(hi, d, 64));
+         }
+         // some row of the slice has step k (w is its longest row)
+         if ((long long) hi - (long long) lo > 65535) { ok = false; }
+         if (lane == 0) { jb[w0 + k] = lo; }
+         e16s[(size_t) k * 64] = in ? (unsigned short) (unsigned) (off - lo) : (unsigned short) 0;
+      }
+      if (lane == 0) { mode[sl] = ok ? 1 : 0; }
+   }
+}
+
 bool ell_build(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32)
 {
    E.free_all();
@@ -544,6 +605,12 @@ bool ell_build(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32)
    E.a = dalloc_n<double>((size_t) stored);
    k_ell_fill<<<std::min(ns, 65536), 64, 0, stream()>>>(A.nrows, ns, A.ia, A.ja, A.a, E.sw, E.j,
                                                          E.a);
+   HX_LAUNCH_CHECK();
+   E.j16 = dalloc_n<unsigned short>((size_t) stored);
+   E.jb = dalloc_n<int>((size_t) (stored / 64) + 1);
+   E.mode = dalloc_n<int>((size_t) ns);
+   k_ell_pack<<<std::min(ns, 65536), 64, 0, stream()>>>(A.nrows, ns, A.ia, E.sw, E.j, E.j16, E.jb,
+                                                         E.mode);
    HX_LAUNCH_CHECK();
    if (fp32) {
       E.a32 = dalloc_n<float>((size_t) stored);
@@ -580,7 +647,7 @@ static int launch_spmv(const Csr &A, const V *vals, X alpha, const X *x, X beta,
       const Ell &E = *A.ell;
       const int nb = grid_for_groups(A.nrows, 64);
       TimedLaunch tl(cls, bytes);
-      k_spmv_ell<V, X, DOT, ZR><<<nb, BS, 0, stream()>>>(A.nrows, E.nslices, A.ia, E.sw, E.j,
+      k_spmv_ell<V, X, DOT, ZR><<<nb, BS, 0, stream()>>>(A.nrows, E.nslices, A.ia, ell_view(E),
                                                         ell_values<V>(E), x, alpha, beta, b, y,
                                                         part, zl1, zw, zu);
       HX_LAUNCH_CHECK();
@@ -724,8 +791,8 @@ static void launch_relax_l1(const Csr &A, const V *vals, const X *f, const X *u,
    } else if (A.ell) {
       const Ell &E = *A.ell;
       const int nb = grid_for_groups(A.nrows, 64);
-      k_relax_l1_ell<V, X><<<nb, BS, 0, s>>>(A.nrows, E.nslices, A.ia, E.sw, E.j, ell_values<V>(E),
-                                             f, u, l1, w, uo);
+      k_relax_l1_ell<V, X><<<nb, BS, 0, s>>>(A.nrows, E.nslices, A.ia, ell_view(E),
+                                             ell_values<V>(E), f, u, l1, w, uo);
    } else {
       const int rw = pick_rw(A.nnz, A.nrows);
       const int nb = grid_for_groups(A.nrows, rw);
