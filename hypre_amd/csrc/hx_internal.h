@@ -104,10 +104,11 @@ struct TimedLaunch
 // consecutive rows; a slice of width w (its longest row) stores 64*w entries
 // column-major (entry k of its 64 rows are 64 consecutive words), each row's
 // entries in CSR storage order, padding masked by the CSR row length.
-// Columns of a slice whose step k spans at most 65535 in col - row (every
-// step of a stencil operator's interior and of most boundary slices) are
-// kept as 16-bit offsets: col = row + jb[k] + j16, jb[k] the step's least
-// col - row; the other slices read the 32-bit columns.
+// Columns of a slice whose every step k spans at most 255 (at most 65535)
+// in col - row are kept as 8-bit (16-bit) offsets: col = row + jb[k] + off,
+// jb[k] the step's least col - row (stencil operators: 8-bit in slices of
+// interior rows, 16-bit in most boundary slices); the other slices read the
+// 32-bit columns.
 struct Ell
 {
    int       nslices = 0;
@@ -116,8 +117,9 @@ struct Ell
    int      *sw = nullptr;        // exclusive prefix of slice widths (nslices + 1)
    int      *j = nullptr;         // 32-bit columns (all slices)
    unsigned short *j16 = nullptr; // 16-bit column offsets (slices with mode 1)
+   unsigned char  *j8 = nullptr;  // 8-bit column offsets (slices with mode 2)
    int      *jb = nullptr;        // per slice step: least col - row (sum of widths)
-   int      *mode = nullptr;      // per slice: 1 = 16-bit columns, 0 = 32-bit
+   int      *mode = nullptr;      // per slice: 2 = 8-bit offsets, 1 = 16-bit, 0 = 32-bit columns
    double   *a = nullptr;
    float    *a32 = nullptr;
    void      free_all();

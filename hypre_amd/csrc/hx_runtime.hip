@@ -426,8 +426,8 @@ void Csr::free_all()
 
 void Ell::free_all()
 {
-   dfree(sw); dfree(j); dfree(j16); dfree(jb); dfree(mode); dfree(a); dfree(a32);
-   sw = j = nullptr; j16 = nullptr; jb = mode = nullptr; a = nullptr; a32 = nullptr;
+   dfree(sw); dfree(j); dfree(j16); dfree(j8); dfree(jb); dfree(mode); dfree(a); dfree(a32);
+   sw = j = nullptr; j16 = nullptr; j8 = nullptr; jb = mode = nullptr; a = nullptr; a32 = nullptr;
    nslices = 0; stored = 0; packed = 0;
 }
 

@@ -336,13 +336,16 @@ struct EllView
 {
    const int *sw, *mode, *jb, *j;
    const unsigned short *j16;
+   const unsigned char *j8;
 };
 
-static EllView ell_view(const Ell &E) { return EllView{E.sw, E.mode, E.jb, E.j, E.j16}; }
+static EllView ell_view(const Ell &E) { return EllView{E.sw, E.mode, E.jb, E.j, E.j16, E.j8}; }
 
-template <int U, bool PACKED, typename V, typename X>
+// IK: 0 = 32-bit columns, 1 = 16-bit offsets, 2 = 8-bit offsets
+template <int U, int IK, typename V, typename X>
 __device__ __forceinline__ X ell_steps(int r, int w, int len, const int *__restrict__ ejs,
                                        const unsigned short *__restrict__ e16s,
+                                       const unsigned char *__restrict__ e8s,
                                        const int *__restrict__ jbs, const V *__restrict__ eas,
                                        const X *__restrict__ x)
 {
@@ -355,7 +358,9 @@ __device__ __forceinline__ X ell_steps(int r, int w, int len, const int *__restr
       for (int u = 0; u < U; u++) {
          // steps past the slice width re-read its last step (in bounds, masked)
          const int ks = min(k0 + u, w - 1);
-         if (PACKED) {
+         if (IK == 2) {
+            jj[u] = r + jbs[ks] + (int) __builtin_nontemporal_load(e8s + ks * 64);
+         } else if (IK == 1) {
             jj[u] = r + jbs[ks] + (int) __builtin_nontemporal_load(e16s + ks * 64);
          } else {
             jj[u] = __builtin_nontemporal_load(ejs + ks * 64);
@@ -375,24 +380,37 @@ __device__ __forceinline__ X ell_steps(int r, int w, int len, const int *__restr
    return t;
 }
 
-template <int U, typename V, typename X, class Epi>
+// pre(row) loads the row's own operands of the epilogue (right-hand side,
+// l1 norm, ...) before the slice's matrix stream, so they arrive with it
+// instead of costing one more round trip after it; epi(row, t, pre values)
+template <int U, typename V, typename X, class Pre, class Epi>
 __device__ __forceinline__ void ell_slice(int sl, int lane, int n, const int *__restrict__ ia,
                                           const EllView &E, const V *__restrict__ ea,
-                                          const X *__restrict__ x, Epi epi)
+                                          const X *__restrict__ x, Pre pre, Epi epi)
 {
    const int r = sl * 64 + lane;
    const int w0 = E.sw[sl], w = E.sw[sl + 1] - w0;
    const size_t base = (size_t) w0 * 64 + lane;
    int len = 0;
    if (r < n) { len = ia[r + 1] - ia[r]; }
+   const auto pv = pre(min(r, n - 1));
    X t;
-   if (E.mode[sl]) {
-      t = ell_steps<U, true, V, X>(r, w, len, nullptr, E.j16 + base, E.jb + w0, ea + base, x);
+   const int m = E.mode[sl];
+   if (m == 2) {
+      t = ell_steps<U, 2, V, X>(r, w, len, nullptr, nullptr, E.j8 + base, E.jb + w0, ea + base, x);
+   } else if (m == 1) {
+      t = ell_steps<U, 1, V, X>(r, w, len, nullptr, E.j16 + base, nullptr, E.jb + w0, ea + base, x);
    } else {
-      t = ell_steps<U, false, V, X>(r, w, len, E.j + base, nullptr, nullptr, ea + base, x);
+      t = ell_steps<U, 0, V, X>(r, w, len, E.j + base, nullptr, nullptr, nullptr, ea + base, x);
    }
-   if (r < n) { epi(r, t); }
+   if (r < n) { epi(r, t, pv); }
 }
+
+template <typename X>
+struct RowOps
+{
+   X a, b, c;
+};
 
 // slices of wave wv of block blk: a contiguous range, ranges in block order
 __device__ __forceinline__ void wave_slices(int ns, int blk, int wv, int nb, int &S0, int &S1)
@@ -420,12 +438,21 @@ __global__ __launch_bounds__(BS) void k_spmv_ell(int n, int ns, const int *__res
    wave_slices(ns, blk, wv, nb, S0, S1);
    double acc = 0.0;
    for (int sl = S0; sl < S1; sl++) {
-      ell_slice<ELL_U, V, X>(sl, lane, n, ia, E, ea, x, [&](int row, X t) {
-         const X v = (beta == (X) 0) ? alpha * t : alpha * t + beta * b[row];
-         y[row] = v;
-         if (DOT) { acc += (double) v * (double) x[row]; }
-         if (ZR) { zu[row] = (X) 0 + (zw * v) / zl1[row]; }
-      });
+      ell_slice<ELL_U, V, X>(
+         sl, lane, n, ia, E, ea, x,
+         [&](int row) {
+            RowOps<X> o{(X) 0, (X) 0, (X) 0};
+            if (beta != (X) 0) { o.a = b[row]; }
+            if (DOT) { o.b = x[row]; }
+            if (ZR) { o.c = zl1[row]; }
+            return o;
+         },
+         [&](int row, X t, const RowOps<X> &o) {
+            const X v = (beta == (X) 0) ? alpha * t : alpha * t + beta * o.a;
+            y[row] = v;
+            if (DOT) { acc += (double) v * (double) o.b; }
+            if (ZR) { zu[row] = (X) 0 + (zw * v) / o.c; }
+         });
    }
    if (DOT) {
       __shared__ double sh[BS];
@@ -454,10 +481,13 @@ __global__ __launch_bounds__(BS) void k_relax_l1_ell(int n, int ns, const int *_
    int S0, S1;
    wave_slices(ns, blk, wv, nb, S0, S1);
    for (int sl = S0; sl < S1; sl++) {
-      ell_slice<ELL_U, V, X>(sl, lane, n, ia, E, ea, u, [&](int row, X t) {
-         const X v = w * f[row] - w * t;
-         uo[row] = u[row] + v / l1[row];
-      });
+      ell_slice<ELL_U, V, X>(
+         sl, lane, n, ia, E, ea, u,
+         [&](int row) { return RowOps<X>{f[row], u[row], l1[row]}; },
+         [&](int row, X t, const RowOps<X> &o) {
+            const X v = w * o.a - w * t;
+            uo[row] = o.b + v / o.c;
+         });
    }
 }
 
@@ -550,12 +580,13 @@ __global__ __launch_bounds__(64) void k_ell_fill(int n, int ns, const int *__res
 }
 
 // per slice: the least and the largest col - row of each step over the rows
-// that have an entry there; a slice whose steps all span at most 65535 stores
-// 16-bit offsets from its steps' least values (mode 1)
+// that have an entry there; a slice whose steps all span at most 255 (65535)
+// reads 8-bit (16-bit) offsets from its steps' least values, mode 2 (1)
 __global__ __launch_bounds__(64) void k_ell_pack(int n, int ns, const int *__restrict__ ia,
                                                  const int *__restrict__ sw,
                                                  const int *__restrict__ ej,
                                                  unsigned short *__restrict__ e16,
+                                                 unsigned char *__restrict__ e8,
                                                  int *__restrict__ jb, int *__restrict__ mode)
 {
    const int lane = threadIdx.x;
@@ -566,7 +597,8 @@ __global__ __launch_bounds__(64) void k_ell_pack(int n, int ns, const int *__res
       if (r < n) { len = ia[r + 1] - ia[r]; }
       const int *ejs = ej + (size_t) w0 * 64 + lane;
       unsigned short *e16s = e16 + (size_t) w0 * 64 + lane;
-      bool ok = true;
+      unsigned char *e8s = e8 + (size_t) w0 * 64 + lane;
+      long long span = 0;
       for (int k = 0; k < w; k++) {
          const bool in = k < len;
          const int off = ejs[(size_t) k * 64] - r;   // both in [0, 2^31)
@@ -576,11 +608,13 @@ __global__ __launch_bounds__(64) void k_ell_pack(int n, int ns, const int *__res
             hi = max(hi, __shfl_xor(hi, d, 64));
          }
          // some row of the slice has step k (w is its longest row)
-         if ((long long) hi - (long long) lo > 65535) { ok = false; }
+         span = max(span, (long long) hi - (long long) lo);
          if (lane == 0) { jb[w0 + k] = lo; }
-         e16s[(size_t) k * 64] = in ? (unsigned short) (unsigned) (off - lo) : (unsigned short) 0;
+         const unsigned d = in ? (unsigned) (off - lo) : 0u;   // truncated where unused
+         e16s[(size_t) k * 64] = (unsigned short) d;
+         e8s[(size_t) k * 64] = (unsigned char) d;
       }
-      if (lane == 0) { mode[sl] = ok ? 1 : 0; }
+      if (lane == 0) { mode[sl] = span <= 255 ? 2 : (span <= 65535 ? 1 : 0); }
    }
 }
 
@@ -607,10 +641,11 @@ bool ell_build(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32)
                                                          E.a);
    HX_LAUNCH_CHECK();
    E.j16 = dalloc_n<unsigned short>((size_t) stored);
+   E.j8 = dalloc_n<unsigned char>((size_t) stored);
    E.jb = dalloc_n<int>((size_t) (stored / 64) + 1);
    E.mode = dalloc_n<int>((size_t) ns);
-   k_ell_pack<<<std::min(ns, 65536), 64, 0, stream()>>>(A.nrows, ns, A.ia, E.sw, E.j, E.j16, E.jb,
-                                                         E.mode);
+   k_ell_pack<<<std::min(ns, 65536), 64, 0, stream()>>>(A.nrows, ns, A.ia, E.sw, E.j, E.j16, E.j8,
+                                                         E.jb, E.mode);
    HX_LAUNCH_CHECK();
    if (fp32) {
       E.a32 = dalloc_n<float>((size_t) stored);

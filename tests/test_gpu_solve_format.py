@@ -75,3 +75,75 @@ def test_pcg_iterations_equal_across_formats(hx, oracle, name):
     # the fused PCG dot sums its partials in a different order: last bits only
     np.testing.assert_allclose(h1, h0, rtol=1e-9)
     np.testing.assert_allclose(x1, x0, rtol=1e-8, atol=1e-12 * np.abs(x0).max())
+
+
+def mixed_mode_matrix(n=80_000, far=70_000):
+    """Symmetric, diagonally dominant: a chain, plus couplings i <-> i + far
+    for even i < n - far, i <-> i + 2 for odd i and i <-> i + 1000 for every
+    third i in [20001, 30000); rows sorted diagonal first, then ascending.  Its 64-row slices include steps whose col - row
+    span is 0 (8-bit offsets), up to a few hundred (16-bit) and about 70000
+    (32-bit columns)."""
+    import scipy.sparse as sp
+    rows, cols = [], []
+    i = np.arange(n - 1)
+    rows += [i, i + 1]
+    cols += [i + 1, i]
+    e = np.arange(0, n - far, 2)
+    rows += [e, e + far]
+    cols += [e + far, e]
+    o = np.arange(1, n - 2, 2)
+    rows += [o, o + 2]
+    cols += [o + 2, o]
+    m = np.arange(20_001, 30_000, 3)
+    rows += [m, m + 1000]
+    cols += [m + 1000, m]
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    M = sp.csr_matrix((-np.ones(len(r)), (r, c)), shape=(n, n))
+    M.sum_duplicates()
+    d = -np.asarray(M.sum(axis=1)).ravel() + 1.0
+    M = (M + sp.diags(d)).tocsr()
+    M.sort_indices()
+    ia, ja, a = [0], [], []
+    for k in range(n):
+        cs = M.indices[M.indptr[k]:M.indptr[k + 1]]
+        vs = M.data[M.indptr[k]:M.indptr[k + 1]]
+        order = sorted(range(len(cs)), key=lambda q: (cs[q] != k, cs[q]))
+        ja += [int(cs[q]) for q in order]
+        a += [float(vs[q]) for q in order]
+        ia.append(len(ja))
+    return np.array(ia, np.int32), np.array(ja, np.int32), np.array(a)
+
+
+def slice_modes(ia, ja):
+    """Each 64-row slice's column format as the setup chooses it: 2 (8-bit)
+    when every step spans <= 255 in col - row, 1 (16-bit) when <= 65535,
+    else 0."""
+    n = len(ia) - 1
+    modes = []
+    for s0 in range(0, n, 64):
+        rr = range(s0, min(n, s0 + 64))
+        w = max(ia[r + 1] - ia[r] for r in rr)
+        span = 0
+        for k in range(w):
+            offs = [int(ja[ia[r] + k]) - r for r in rr if k < ia[r + 1] - ia[r]]
+            span = max(span, max(offs) - min(offs))
+        modes.append(2 if span <= 255 else (1 if span <= 65535 else 0))
+    return np.array(modes)
+
+
+def test_vcycle_bits_equal_across_formats_all_column_widths(hx, oracle):
+    """A matrix whose fine-level slices use all three column formats of the
+    sliced-ELL copy: the V-cycle is bit-equal to the CSR kernels'."""
+    say("mixed column widths")
+    ia, ja, a = mixed_mode_matrix()
+    modes = slice_modes(ia, ja)
+    assert {0, 1, 2} <= set(modes.tolist()), np.bincount(modes)
+    A = hx.ParCSRMatrix.from_csr(ia, ja, a)
+    f = 2 * oracle.rand_stream(9, len(ia) - 1) - 1
+    for fp32 in (False, True):
+        u0 = cycle(hx, A, 0, f, fp32)
+        u1 = cycle(hx, A, 1, f, fp32)
+        assert np.isfinite(u0).all() and np.abs(u0).max() > 0
+        bad = np.nonzero(u0.view(np.int64) != u1.view(np.int64))[0]
+        assert bad.size == 0, f"{bad.size} entries differ (fp32={fp32}), first at {bad[:5]}"
