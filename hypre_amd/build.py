@@ -23,7 +23,7 @@ LIB = os.path.join(HERE, "libhypre_mi355x.so")
 ARCH = os.environ.get("HYPRE_MI355X_ARCH", "gfx950")
 
 SOURCES = ["hx_runtime.hip", "hx_solve.hip", "hx_setup.hip", "hx_amg.hip", "hx_ij.hip",
-           "hx_mlp.hip", "hx_hmis.hip", "hx_agg.hip", "hx_noalloc.hip"]
+           "hx_mlp.hip", "hx_hmis.hip", "hx_agg.hip", "hx_noalloc.hip", "hx_stats.hip"]
 
 CFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

@@ -285,17 +285,8 @@ void AMGSolver::setup(const Csr &A0)
    sync();
    setup_done = true;
    setup_seconds = wall_seconds() - t0;
-   if (p.print_level > 0) {
-      double rows = 0, nnz = 0;
-      printf("\n BoomerAMG (MI355X) setup: %d levels\n", num_levels);
-      printf(" lev       rows     entries\n");
-      for (int j = 0; j < num_levels; j++) {
-         printf(" %3d %10d %11lld\n", j, L[j].A->nrows, L[j].A->nnz);
-         rows += L[j].A->nrows;
-         nnz += (double) L[j].A->nnz;
-      }
-      printf("  Complexity: grid = %f  operator = %f\n", rows / L[0].A->nrows, nnz / (double) L[0].A->nnz);
-   }
+   // par_amg_setup.c:3830-3833
+   if (p.print_level == 1 || p.print_level == 3) { print_setup_stats(); }
 }
 
 // ================================================================== V-cycle

@@ -106,6 +106,8 @@ struct AMGSolver : public hypre_Solver_struct
    ~AMGSolver() override;
    void free_hierarchy();
    void setup(const Csr &A);
+   // the setup statistics table (par_stats.c:20), print level 1 or 3
+   void print_setup_stats() const;
    void cycle(const double *f, double *u, bool u_zero);
    void cycle32(const double *f, double *u);
    int  solve(const Csr &A, hypre_ParVector_struct *b, hypre_ParVector_struct *x);

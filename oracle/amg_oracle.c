@@ -20,6 +20,7 @@
 #include "amg_oracle.h"
 
 #include <math.h>
+#include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -2282,4 +2283,213 @@ int orc_pcg_solve(orc_amg *H, const double *b, double *x, double tol, int max_it
    if (i >= max_iter && (i_prod / bi_prod) >= eps) { err = 256; }
    free(p); free(s); free(r);
    return err;
+}
+
+/* ---------------------------------------------------------------- setup statistics
+ * src/parcsr_ls/par_stats.c:20-1340 (hypre_BoomerAMGSetupStats), one rank,
+ * for the options of this path: PMIS or HMIS coarsening, ext+i interpolation
+ * with optional aggressive levels (modified multipass), relax_order 0 and no
+ * grid relax points.  The text is what the reference prints with print level
+ * 1 or 3 after setup; it goes to buf (NUL-terminated, at most cap bytes) and
+ * the return value is the length the full text needs. */
+typedef struct
+{
+   char  *buf;
+   size_t cap, len;
+} orc_text;
+
+static void text_add(orc_text *t, const char *fmt, ...)
+{
+   va_list ap;
+   size_t room = (t->len < t->cap) ? t->cap - t->len : 0;
+   int n;
+   va_start(ap, fmt);
+   n = vsnprintf(room ? t->buf + t->len : NULL, room, fmt, ap);
+   va_end(ap);
+   if (n > 0) { t->len += (size_t) n; }
+}
+
+/* printf.c:136 hypre_ndigits_ull */
+static int orc_ndigits(long long v)
+{
+   int d = 0;
+   while (v > 0) { v /= 10; d++; }
+   return d;
+}
+
+/* par_stats.c:587-650 (operators) and :851-960 (interpolation): entries per
+ * row, row sums in storage order, and for P the smallest weight and the
+ * largest weight other than 1.0; hypre_min / hypre_max keep their operand
+ * order so that ties resolve as there */
+static void orc_level_stats(const orc_csr *M, int weights, int *min_e, int *max_e,
+                            double *min_rs, double *max_rs, double *min_w, double *max_w)
+{
+   int i, k;
+   *min_e = 0; *max_e = 0;
+   *min_rs = 0.0; *max_rs = 0.0;
+   *min_w = 1.0; *max_w = 0.0;
+   if (M->nrows == 0) { *min_e = 1000000; *min_rs = 1.0e7; *min_w = 1.0e7; return; }
+   if (weights && M->ia[M->nrows] > 0) { *min_w = M->a[0]; }
+   for (i = 0; i < M->nrows; i++)
+   {
+      int e = M->ia[i + 1] - M->ia[i];
+      double rs = 0.0;
+      for (k = M->ia[i]; k < M->ia[i + 1]; k++)
+      {
+         double v = M->a[k];
+         if (weights)
+         {
+            *min_w = (*min_w < v) ? *min_w : v;
+            if (v != 1.0) { *max_w = (*max_w < v) ? v : *max_w; }
+         }
+         rs += v;
+      }
+      if (i == 0) { *min_e = e; *min_rs = rs; *max_rs = rs; }
+      *min_e = (e < *min_e) ? e : *min_e;
+      *max_e = (*max_e < e) ? e : *max_e;
+      *min_rs = (rs < *min_rs) ? rs : *min_rs;
+      *max_rs = (rs < *max_rs) ? *max_rs : rs;
+   }
+}
+
+int orc_amg_setup_stats(const orc_amg *H, int max_iter, double tol, char *buf, int cap)
+{
+   const orc_amg_params *p = &H->prm;
+   orc_text t = {buf, (size_t)(cap > 0 ? cap : 0), 0};
+   int L = H->num_levels, lev, i, nd[4];
+   double num_coeffs[ORC_MAXLEV], num_mem[ORC_MAXLEV], num_vars[ORC_MAXLEV];
+   double grid = 0.0, oper = 0.0, mem = 0.0, total = 0.0;
+   int up = (p->relax_up_type >= 0) ? p->relax_up_type : p->relax_type;
+
+   /* par_stats.c:209-325 */
+   text_add(&t, "\n\n Num MPI tasks = %d\n", 1);
+   text_add(&t, " Num OpenMP threads = %d\n", 1);
+   text_add(&t, " Execution policy = %s\n\n", "Device (HIP)");
+   text_add(&t, " Max levels = %d\n", p->max_levels);
+   text_add(&t, " Num levels = %d\n\n", L);
+   text_add(&t, " Strength Threshold = %f\n", p->strong_threshold);
+   text_add(&t, " Interpolation Truncation Factor = %f\n", p->trunc_factor);
+   text_add(&t, " Maximum Row Sum Threshold for Dependency Weakening = %f\n\n", p->max_row_sum);
+   text_add(&t, " Coarsening Type = %s \n", p->coarsen_type == 10 ? "HMIS" : "PMIS");
+   if (p->agg_num_levels > 0)
+   {
+      text_add(&t, "\n No. of levels of aggressive coarsening: %d\n\n", p->agg_num_levels);
+      text_add(&t, " Interpolation on agg. levels = matrix-matrix multipass\n");
+   }
+   text_add(&t, " measures are determined %s\n\n", "locally");
+   text_add(&t, " Interpolation = extended+i\n");
+   text_add(&t, "\nOperator Matrix Information:\n\n");
+
+   /* par_stats.c:443-486 */
+   nd[0] = orc_ndigits(H->A[0].nrows);
+   nd[1] = orc_ndigits(H->A[0].ia[H->A[0].nrows]);
+   nd[0] = nd[0] > 7 ? nd[0] : 7;
+   nd[1] = nd[1] > 8 ? nd[1] : 8;
+   nd[2] = 4;
+   for (lev = 0; lev < L; lev++)
+   {
+      long long nnz = H->A[lev].ia[H->A[lev].nrows];
+      int d = orc_ndigits(nnz / H->A[lev].nrows);
+      nd[2] = d > nd[2] ? d : nd[2];
+   }
+   nd[2] += 2;
+   nd[3] = nd[0] + nd[1] + nd[2];
+   text_add(&t, "%*s", nd[0] + 13, "nonzero");
+   text_add(&t, "%*s", nd[1] + 15, "entries/row");
+   text_add(&t, "%18s\n", "row sums");
+   text_add(&t, "%s %*s ", "lev", nd[0], "rows");
+   text_add(&t, "%*s", nd[1], "entries");
+   text_add(&t, "%7s %5s %4s", "sparse", "min", "max");
+   text_add(&t, "%*s %8s %11s\n", nd[2] + 2, "avg", "min", "max");
+   for (i = 0; i < 49 + nd[3]; i++) { text_add(&t, "="); }
+   text_add(&t, "\n");
+
+   /* par_stats.c:497-700 */
+   for (lev = 0; lev < L; lev++)
+   {
+      const orc_csr *A = &H->A[lev];
+      double nnz = (double) A->ia[A->nrows], rows = (double) A->nrows;
+      int mne, mxe;
+      double mnr, mxr, mnw, mxw;
+      num_coeffs[lev] = nnz;
+      num_mem[lev] = nnz;
+      num_vars[lev] = rows;
+      orc_level_stats(A, 0, &mne, &mxe, &mnr, &mxr, &mnw, &mxw);
+      text_add(&t, "%3d %*d %*.0f  %0.3f  %4d %4d", lev, nd[0], A->nrows, nd[1], nnz,
+               nnz / (rows * rows), mne, mxe);
+      text_add(&t, "  %*.1f  %10.3e  %10.3e\n", nd[2], nnz / rows, mnr, mxr);
+   }
+
+   /* par_stats.c:702-996 */
+   nd[0] = 5;
+   if (L - 1)
+   {
+      int d = orc_ndigits(H->P[0].nrows);
+      nd[0] = d > nd[0] ? d : nd[0];
+   }
+   text_add(&t, "\n\nInterpolation Matrix Information:\n");
+   text_add(&t, "%*s ", 2 * nd[0] + 21, "entries/row");
+   text_add(&t, "%10s %10s %19s\n", "min", "max", "row sums");
+   text_add(&t, "lev %*s x %-*s min  max  avgW", nd[0], "rows", nd[0], "cols");
+   text_add(&t, "%11s %11s %9s %11s\n", "weight", "weight", "min", "max");
+   for (i = 0; i < 70 + 2 * nd[0]; i++) { text_add(&t, "="); }
+   text_add(&t, "\n");
+   for (lev = 0; lev < L - 1; lev++)
+   {
+      const orc_csr *P = &H->P[lev];
+      double nnz = (double) P->ia[P->nrows];
+      int mne, mxe;
+      double mnr, mxr, mnw, mxw;
+      num_mem[lev] += nnz;
+      orc_level_stats(P, 1, &mne, &mxe, &mnr, &mxr, &mnw, &mxw);
+      text_add(&t, "%3d %*d x %-*d %3d  %3d", lev, nd[0], P->nrows, nd[0], P->ncols, mne, mxe);
+      text_add(&t, "  %4.1f  %10.3e  %10.3e  %10.3e  %10.3e\n",
+               (nnz - (double) P->ncols) / ((double)(P->nrows - P->ncols)), mnw, mxw, mnr, mxr);
+   }
+
+   /* par_stats.c:997-1020 */
+   for (lev = 0; lev < L; lev++)
+   {
+      mem += num_mem[lev] / num_coeffs[0];
+      oper += num_coeffs[lev] / num_coeffs[0];
+      total += num_vars[lev];
+   }
+   if (num_vars[0] != 0) { grid = total / num_vars[0]; }
+   text_add(&t, "\n\n");
+   text_add(&t, "     Complexity:   grid = %f\n", grid);
+   text_add(&t, "               operator = %f\n", oper);
+   text_add(&t, "                 memory = %f\n", mem);
+   text_add(&t, "\n\n");
+
+   /* par_stats.c:1022-1034, 1246-1322 (V-cycle, relax_order 0) */
+   text_add(&t, "\n\nBoomerAMG SOLVER PARAMETERS:\n\n");
+   text_add(&t, "  Maximum number of cycles:         %d \n", max_iter);
+   text_add(&t, "  Stopping Tolerance:               %e \n", tol);
+   text_add(&t, "  Cycle type (1 = V, 2 = W, etc.):  %d\n\n", 1);
+   text_add(&t, "  Relaxation Parameters:\n");
+   text_add(&t, "   Visiting Grid:                     down   up  coarse\n");
+   text_add(&t, "            Number of sweeps:         %4d   %2d  %4d \n", p->num_sweeps,
+            p->num_sweeps, H->coarse_sweeps);
+   text_add(&t, "   Type 0=Jac, 3=hGS, 6=hSGS, 9=GE:   %4d   %2d  %4d \n", p->relax_type, up,
+            H->coarse_relax);
+   text_add(&t, "   Point types, partial sweeps (1=C, -1=F):\n");
+   text_add(&t, "                  Pre-CG relaxation (down):");
+   for (i = 0; i < p->num_sweeps; i++) { text_add(&t, "  %2d", 0); }
+   text_add(&t, "\n");
+   text_add(&t, "                   Post-CG relaxation (up):");
+   for (i = 0; i < p->num_sweeps; i++) { text_add(&t, "  %2d", 0); }
+   text_add(&t, "\n");
+   text_add(&t, "                             Coarsest grid:");
+   for (i = 0; i < H->coarse_sweeps; i++) { text_add(&t, "  %2d", 0); }
+   text_add(&t, "\n");
+   text_add(&t, "\n");
+   for (lev = 0; lev < L; lev++)
+   {
+      if (p->relax_weight != 1) { text_add(&t, " Relaxation Weight %f level %d\n", p->relax_weight, lev); }
+   }
+   for (lev = 0; lev < L; lev++)
+   {
+      if (p->outer_weight != 1) { text_add(&t, " Outer relaxation weight %f level %d\n", p->outer_weight, lev); }
+   }
+   return (int) t.len;
 }

@@ -179,6 +179,9 @@ const int     *orc_amg_CF(const orc_amg *H, int lev);
 const double  *orc_amg_l1(const orc_amg *H, int lev);
 int            orc_amg_pmis_rounds(const orc_amg *H, int lev);
 int            orc_amg_coarse_relax(const orc_amg *H);
+/* the setup statistics text (src/parcsr_ls/par_stats.c:20, print level 1)
+ * into buf; returns the length the whole text needs */
+int orc_amg_setup_stats(const orc_amg *H, int max_iter, double tol, char *buf, int cap);
 /* one V-cycle: u <- V-cycle(f, u); src/parcsr_ls/par_cycle.c:23 */
 int orc_amg_cycle(orc_amg *H, const double *f, double *u);
 /* AMG as a solver, src/parcsr_ls/par_amg_solve.c:22 (converge_type 0) */

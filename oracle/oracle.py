@@ -106,6 +106,7 @@ def lib():
         L.orc_amg_l1.restype = P(C.c_double)
         L.orc_amg_pmis_rounds.argtypes = [C.c_void_p, C.c_int]
         L.orc_amg_coarse_relax.argtypes = [C.c_void_p]
+        L.orc_amg_setup_stats.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_char_p, C.c_int]
         L.orc_amg_cycle.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
         L.orc_amg_solve.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), C.c_double,
                                     C.c_int, P(C.c_int), P(C.c_double), P(C.c_double)]
@@ -470,6 +471,14 @@ class Hierarchy:
         rows = [self.A(l).nrows for l in range(self.num_levels)]
         nnz = [self.A(l).nnz for l in range(self.num_levels)]
         return sum(rows) / rows[0], sum(nnz) / nnz[0]
+
+    def setup_stats(self, max_iter=20, tol=1e-9) -> str:
+        """The setup statistics text (src/parcsr_ls/par_stats.c:20) for this
+        hierarchy, with the solver's max_iter and tol in its last part."""
+        need = lib().orc_amg_setup_stats(self._h, max_iter, tol, None, 0)
+        buf = C.create_string_buffer(need + 1)
+        lib().orc_amg_setup_stats(self._h, max_iter, tol, buf, need + 1)
+        return buf.value.decode()
 
     def cycle(self, f, u=None):
         f = np.ascontiguousarray(f, dtype=np.float64)
