@@ -3851,6 +3851,354 @@ __global__ __launch_bounds__(WAVE) void k_rap_fast(RapFastArgs p)
    }
 }
 
+// ------------------------------------------------------------------ RAP, stage 0 by fixed lane layouts
+// k_rap_fast<RAREG>'s arithmetic and outputs without flattened segments (no
+// per-chunk segment search, no prefix sums, no rounds):
+//   * RA row, MULTI: 64 / WA A rows per step, WA lanes each (WA, a power of
+//     two <= 32, at least A's longest row).  All lanes insert at once; a key
+//     created during the step is stored with bit 31 set until the first of
+//     the step's rows that holds it (in R order) numbers it and clears the
+//     bit, and the rows then apply their products one after the other, so
+//     every RA entry receives its products in R order.  !MULTI: one A row
+//     per step, 64 entries at a time (a row's columns are distinct);
+//   * Ac row: 64 / W RA entries per step, W lanes each (W, a power of two, at
+//     least P's longest row), lane order = traversal order.  Each lane ORs its
+//     bit into its column's mask; the lowest lane of a column adds the
+//     column's products of the step in lane order (reading them from LDS) and
+//     numbers the column if it is new, so columns are created in the order of
+//     their first product and receive their products in traversal order;
+//   * the loads of RS_D steps are issued while the previous RS_D steps are
+//     applied.
+constexpr int RS_D = 4;
+constexpr int RS_NEW = (int) 0x80000000;   // RA key created in the current step
+
+__device__ __forceinline__ int rs_insert(int *keys, int cap, int bits, int key, bool &isnew,
+                                         bool &fresh)
+{
+   unsigned h = ((unsigned) key * 2654435761u) >> (32 - bits);
+   for (int probe = 0; probe < cap; probe++) {
+      const int old = atomicCAS(&keys[h], -1, key | RS_NEW);
+      if (old == -1) {
+         isnew = true;
+         fresh = true;
+         return (int) h;
+      }
+      if ((old & 0x7fffffff) == key) {
+         isnew = false;
+         fresh = (old & RS_NEW) != 0;
+         return (int) h;
+      }
+      h = (h + 1) & (cap - 1);
+   }
+   isnew = false;
+   fresh = false;
+   return -1;
+}
+
+static size_t rap_seg_lds(int rc, int rcl, int ac, int acl)
+{
+   const size_t ra = (size_t) rc * 12 + (size_t) rcl * 4;
+   const size_t acb = (size_t) ac * 24 + (size_t) acl * 4 + WAVE * 8;
+   return std::max(ra, acb);
+}
+
+template <int RAREG, bool MULTI>
+__global__ __launch_bounds__(WAVE) void k_rap_seg(RapFastArgs p, int wsh, int wsa)
+{
+   extern __shared__ double dyn_lds[];
+   const int lane = threadIdx.x;
+   double *ra_val = dyn_lds;                    // rc
+   int *ra_keys = reinterpret_cast<int *>(ra_val + p.rc);
+   int *ra_order = ra_keys + p.rc;              // rcl
+   // the Ac tables reuse the RA tables' space once RA is in registers
+   double *ac_val = dyn_lds;                    // ac
+   unsigned long long *ac_mask = reinterpret_cast<unsigned long long *>(ac_val + p.ac);
+   int *ac_keys = reinterpret_cast<int *>(ac_mask + p.ac);
+   int *ac_pos = ac_keys + p.ac;
+   int *ac_order = ac_pos + p.ac;               // acl
+   double *xs = reinterpret_cast<double *>(ac_order + p.acl);   // 64 products of a step
+   RowHash HC{ac_keys, nullptr, p.ac, ilog2(p.ac)};
+   const int rbits = ilog2(p.rc);
+   const int wmask = (1 << wsh) - 1;
+   const int epc = WAVE >> wsh;                 // RA entries per Ac step
+   const int sub = lane >> wsh, pj = lane & (wmask);
+   const int amask_a = (1 << wsa) - 1;
+   const int epa = WAVE >> wsa;                 // A rows per RA step (MULTI)
+   const int sub_a = lane >> wsa, aj = lane & amask_a;
+
+   RowClaim claim(p.claim, p.nwork);
+   claim.issue(lane);
+   for (int wi = claim.take(lane); wi >= 0; wi = claim.take(lane)) {
+      claim.issue(lane);   // the next row's claim is in flight while this row runs
+      const int ic = wi;
+      for (int s = lane; s < p.rc; s += WAVE) { ra_keys[s] = -1; }
+      wave_sync();
+      bool defer = false;
+      // ---- row ic of RA (par_rap.c:1712-1800)
+      int nra = 0;
+      const int rb0 = p.R_ia[ic], rb1 = p.R_ia[ic + 1];
+      for (int rb = rb0; rb < rb1 && !defer; rb += WAVE) {
+         // lane s: R entry rb + s and its A row [ab, ae)
+         int ab = 0, ae = 0;
+         double rv = 0.0;
+         if (rb + lane < rb1) {
+            const int i1 = p.R_ja[rb + lane];
+            rv = p.R_a[rb + lane];
+            ab = p.A_ia[i1];
+            ae = p.A_ia[i1 + 1];
+         }
+         const int nseg = min(WAVE, rb1 - rb);
+         if (MULTI && __ballot(rb + lane < rb1 && ae - ab > amask_a + 1)) {
+            defer = true;   // an A row longer than WA
+            break;
+         }
+         // step cursor (uniform): segment sg (MULTI: first of epa) from entry kb
+         int sg = 0, kb = readlane_i(ab, 0), ke = readlane_i(ae, 0);
+         // two buffers of RS_D steps, named so that every register index is static
+         int k0[RS_D], k1[RS_D];
+         double a0[RS_D], a1[RS_D], r0[RS_D], r1[RS_D];
+         bool s0[RS_D], s1[RS_D];
+         auto fill = [&](int (&key)[RS_D], double (&av)[RS_D], double (&rr)[RS_D], bool (&st)[RS_D]) {
+#pragma unroll
+            for (int u = 0; u < RS_D; u++) {
+               st[u] = sg < nseg;
+               key[u] = -1;
+               av[u] = 0.0;
+               rr[u] = 0.0;
+               if (sg < nseg) {
+                  if (MULTI) {
+                     const int e = sg + sub_a;
+                     const int src = min(e, WAVE - 1);
+                     const int b_s = __shfl(ab, src, WAVE);
+                     const int e_s = __shfl(ae, src, WAVE);
+                     rr[u] = shfl_d(rv, src);
+                     if (e < nseg && b_s + aj < e_s) {
+                        key[u] = p.A_ja[b_s + aj];
+                        av[u] = p.A_a[b_s + aj];
+                     }
+                     sg += epa;
+                  } else {
+                     const int k = kb + lane;
+                     rr[u] = readlane_d(rv, sg);
+                     if (k < ke) {
+                        key[u] = p.A_ja[k];
+                        av[u] = p.A_a[k];
+                     }
+                     kb += WAVE;
+                     if (kb >= ke) {
+                        sg++;
+                        if (sg < nseg) {
+                           kb = readlane_i(ab, sg);
+                           ke = readlane_i(ae, sg);
+                        }
+                     }
+                  }
+               }
+            }
+         };
+         auto apply = [&](const int (&key)[RS_D], const double (&av)[RS_D], const double (&rr)[RS_D],
+                          const bool (&st)[RS_D]) {
+#pragma unroll
+            for (int u = 0; u < RS_D; u++) {
+               if (!st[u] || defer) { break; }
+               const bool valid = key[u] >= 0;
+               bool isn = false, fresh = false;
+               int sl = -1;
+               if (valid) { sl = rs_insert(ra_keys, p.rc, rbits, key[u], isn, fresh); }
+               const bool on = valid && sl >= 0;
+               const double x = rr[u] * av[u];
+               // the step's rows in R order (one row when !MULTI)
+               for (int h = 0; h < (MULTI ? epa : 1); h++) {
+                  const bool mine = on && (!MULTI || sub_a == h);
+                  // a key created in this step is numbered by the first row
+                  // that holds it: the one that still finds bit 31 set
+                  bool create = false;
+                  if (mine && fresh) { create = (ra_keys[sl] & RS_NEW) != 0; }
+                  const unsigned long long cm = __ballot(create);
+                  if (nra + __popcll(cm) > p.rcl) {
+                     defer = true;
+                     break;
+                  }
+                  if (mine) {
+                     if (create) {
+                        const int pos = nra + __popcll(cm & lanemask_lt(lane));
+                        ra_order[pos] = sl;
+                        ra_val[sl] = x;
+                        ra_keys[sl] = key[u];
+                     } else {
+                        ra_val[sl] += x;   // created by an earlier row
+                     }
+                  }
+                  nra += __popcll(cm);
+                  wave_sync();
+               }
+            }
+         };
+         fill(k0, a0, r0, s0);
+         for (;;) {
+            if (!s0[0] || defer) { break; }
+            fill(k1, a1, r1, s1);
+            apply(k0, a0, r0, s0);
+            if (!s1[0] || defer) { break; }
+            fill(k0, a0, r0, s0);
+            apply(k1, a1, r1, s1);
+         }
+      }
+      // ---- (RA) x P, diagonal first (par_rap.c:1612-1620, 1860-1920)
+      int nac = 0;
+      int rk[RAREG];
+      double rx[RAREG];
+      int rpb[RAREG], rpl[RAREG];
+      if (!defer) {
+         // RA entry k to lane k % 64, register k / 64, and its P row
+#pragma unroll
+         for (int t = 0; t < RAREG; t++) {
+            const int k = t * WAVE + lane;
+            rk[t] = 0;
+            rx[t] = 0.0;
+            if (k < nra) {
+               const int sl = ra_order[k];
+               rk[t] = ra_keys[sl];
+               rx[t] = ra_val[sl];
+            }
+         }
+#pragma unroll
+         for (int t = 0; t < RAREG; t++) {
+            const int k = t * WAVE + lane;
+            rpb[t] = 0;
+            rpl[t] = 0;
+            if (k < nra) {
+               rpb[t] = p.P_ia[rk[t]];
+               rpl[t] = p.P_ia[rk[t] + 1] - rpb[t];
+            }
+         }
+         wave_sync();
+         for (int s = lane; s < p.ac; s += WAVE) {
+            ac_keys[s] = -1;
+            ac_pos[s] = -1;
+            ac_mask[s] = 0ull;
+         }
+         wave_sync();
+         if (lane == 0) {
+            bool isn;
+            const int sl = HC.insert(ic, isn);
+            ac_pos[sl] = 0;
+            ac_order[0] = sl;
+            ac_val[sl] = 0.0;
+         }
+         nac = 1;
+         wave_sync();
+      }
+#pragma unroll
+      for (int t = 0; t < RAREG; t++) {
+         if (defer || t * WAVE >= nra) { break; }
+         const int nb = min(WAVE, nra - t * WAVE);   // RA entries held in register t
+         // steps of epc entries: entry e = eb + sub, P entry pj of its row
+         int eb = 0;
+         int k0[RS_D], k1[RS_D];
+         double v0[RS_D], v1[RS_D], r0[RS_D], r1[RS_D];
+         bool s0[RS_D], s1[RS_D];
+         auto fill = [&](int (&key)[RS_D], double (&pv)[RS_D], double (&rs)[RS_D], bool (&st)[RS_D]) {
+#pragma unroll
+            for (int u = 0; u < RS_D; u++) {
+               st[u] = eb < nb;
+               key[u] = -1;
+               pv[u] = 0.0;
+               rs[u] = 0.0;
+               if (eb < nb) {
+                  const int e = eb + sub;
+                  const int src = min(e, WAVE - 1);
+                  const int pb = __shfl(rpb[t], src, WAVE);
+                  const int pl = __shfl(rpl[t], src, WAVE);
+                  rs[u] = shfl_d(rx[t], src);
+                  // a P row longer than W would lose entries: the row is deferred
+                  if (__ballot(e < nb && pl > wmask + 1)) { defer = true; }
+                  if (e < nb && pj < pl) {
+                     key[u] = p.P_ja[pb + pj];
+                     pv[u] = p.P_a[pb + pj];
+                  }
+                  eb += epc;
+               }
+            }
+         };
+         auto apply = [&](const int (&key)[RS_D], const double (&pv)[RS_D], const double (&rs)[RS_D],
+                          const bool (&st)[RS_D]) {
+#pragma unroll
+            for (int u = 0; u < RS_D; u++) {
+               if (!st[u] || defer) { break; }
+               const bool valid = key[u] >= 0;
+               bool isn = false;
+               int sl = -1;
+               if (valid) { sl = HC.insert(key[u], isn); }
+               if (nac + __popcll(__ballot(isn)) > p.acl) {
+                  defer = true;
+                  break;
+               }
+               const double x = rs[u] * pv[u];
+               const bool on = valid && sl >= 0;
+               if (on) { atomicOr(&ac_mask[sl], 1ull << lane); }
+               xs[lane] = x;
+               wave_sync();
+               unsigned long long m = 0ull;
+               bool lead = false;
+               if (on) {
+                  m = ac_mask[sl];
+                  lead = (m & lanemask_lt(lane)) == 0ull;
+               }
+               const bool fresh = lead && ac_pos[sl] < 0;
+               const unsigned long long fm = __ballot(fresh);
+               if (lead) {
+                  // the column's products of this step, in lane order
+                  double acc = x;
+                  if (!fresh) {
+                     acc = ac_val[sl];
+                     acc += x;
+                  }
+                  for (unsigned long long rest = m & (m - 1); rest; rest &= rest - 1) {
+                     acc += xs[__ffsll((long long) rest) - 1];
+                  }
+                  ac_val[sl] = acc;
+                  ac_mask[sl] = 0ull;
+                  if (fresh) {
+                     const int pos = nac + __popcll(fm & lanemask_lt(lane));
+                     ac_pos[sl] = pos;
+                     ac_order[pos] = sl;
+                  }
+               }
+               nac += __popcll(fm);
+               wave_sync();
+            }
+         };
+         fill(k0, v0, r0, s0);
+         for (;;) {
+            if (!s0[0] || defer) { break; }
+            fill(k1, v1, r1, s1);
+            apply(k0, v0, r0, s0);
+            if (!s1[0] || defer) { break; }
+            fill(k0, v0, r0, s0);
+            apply(k1, v1, r1, s1);
+         }
+      }
+      if (!defer) {
+         const size_t o = (size_t) ic * p.acl;
+         for (int k = lane; k < nac; k += WAVE) {
+            const int sl = ac_order[k];
+            p.tmp_j[o + k] = ac_keys[sl];
+            p.tmp_a[o + k] = ac_val[sl];
+         }
+         if (lane == 0) {
+            p.cnt[ic] = nac;
+            p.src[ic] = (int) o;
+         }
+      } else if (lane == 0) {
+         p.src[ic] = -1;
+         const int d = atomicAdd(p.ndefer, 1);
+         p.defer_list[d] = ic;
+      }
+      wave_sync();
+   }
+}
+
 // copy the rows finished by the fast stages into Ac (16 lanes per row)
 __global__ void k_rap_compact(int nc, const int *__restrict__ cnt, const int *__restrict__ src,
                               const int *__restrict__ out_ia, const int *__restrict__ tj0,
@@ -3917,6 +4265,21 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
    const int *list = nullptr;
    int nwork = nc;
    int cur = 0;
+   // stage 0 runs k_rap_seg when P's rows fit W <= 16 lanes each (log2 W);
+   // A rows of at most WA <= 32 entries are taken 64 / WA at a time (log2 WA)
+   int pw = -1, aw = 6;
+   {
+      const int pmax = csr_max_row(P);
+      if (pmax <= 16) {
+         pw = 0;
+         while ((1 << pw) < pmax) { pw++; }
+         const int amax = csr_max_row(A);
+         if (amax <= 32) {
+            aw = 2;
+            while ((1 << aw) < amax) { aw++; }
+         }
+      }
+   }
    for (int stage = 0; stage < 3 && nwork > 0; stage++) {
       // offsets are kept as int (times 2 after stage 0)
       if ((long long) nwork * acls[stage] * 2 >= INT_MAX) { break; }
@@ -3936,7 +4299,14 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
       f.claim = claim_counters();
       if (raregs[stage] > 0) { f.rcl = raregs[stage] * WAVE; }
       const size_t lds = rap_fast_lds(f.rc, f.rcl, f.ac, f.acl, raregs[stage]);
-      if (stage == 0) {
+      if (stage == 0 && pw >= 0) {
+         const size_t lds_s = rap_seg_lds(f.rc, f.rcl, f.ac, f.acl);
+         if (aw <= 5) {
+            k_rap_seg<RAP_RAREG, true><<<launch_blocks(nwork), WAVE, lds_s, stream()>>>(f, pw, aw);
+         } else {
+            k_rap_seg<RAP_RAREG, false><<<launch_blocks(nwork), WAVE, lds_s, stream()>>>(f, pw, 6);
+         }
+      } else if (stage == 0) {
          k_rap_fast<RAP_RAREG><<<launch_blocks(nwork), WAVE, lds, stream()>>>(f);
       } else if (stage == 1) {
          k_rap_fast<2 * RAP_RAREG><<<launch_blocks(nwork), WAVE, lds, stream()>>>(f);
