@@ -2392,7 +2392,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
    __shared__ int vals_all[R * HC];
    __shared__ double wgt_all[R * MC];
    __shared__ int chat_key_all[R * MC];
-   __shared__ int mins_all[R * MC];
+   __shared__ unsigned long long wmask_all[R * MC];   // lanes of a step that hit a weight
    __shared__ int seg_ki_all[WAVE];
    __shared__ double st_a_all[R * OLC];
    __shared__ int st_k_all[R * OLC];
@@ -2403,7 +2403,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
    int *vals = vals_all + grp * HC;
    double *wgt = wgt_all + grp * MC;
    int *chat_key = chat_key_all + grp * MC;
-   int *mins = mins_all + grp * MC;
+   unsigned long long *wmask = wmask_all + grp * MC;
    int *seg_ki = seg_ki_all + grp * G;
    double *st_a = st_a_all + grp * OLC;
    int *st_k = st_k_all + grp * OLC;
@@ -2572,7 +2572,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
          ccol = p.f2c[chat_key[gl]];
          wgt[gl] = 0.0;
       }
-      if (gl < MC) { mins[gl] = 0x7fffffff; }
+      if (gl < MC) { wmask[gl] = 0ull; }
       // entry j: kind 1 C-hat (weight v), 2 strong F neighbour, 3 weak, 0 none
       int v = -1, sk = 0;
       if (live && gl < na) {
@@ -2689,14 +2689,28 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
          shas_i = (ki >= 0) && ((dg1 < 0) ? (sai > 0) : (sai < 0));
          double acc = 0.0;
          bool pend = shas_i;
-         for (int q = ooff; q < ooff + olen; q++) {
-            const int kq = st_k[q];
-            if (kq >= 0) {
-               if (pend && ki < kq) {
-                  acc += sai;
-                  pend = false;
+         // four terms' LDS reads issued together, then folded in order
+         for (int q0 = ooff; q0 < ooff + olen; q0 += 4) {
+            int kq[4];
+            double aq[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+               kq[v] = -1;
+               aq[v] = 0.0;
+               if (q0 + v < ooff + olen) {
+                  kq[v] = st_k[q0 + v];
+                  aq[v] = st_a[q0 + v];
                }
-               acc += st_a[q];
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+               if (kq[v] >= 0) {
+                  if (pend && ki < kq[v]) {
+                     acc += sai;
+                     pend = false;
+                  }
+                  acc += aq[v];
+               }
             }
          }
          if (pend) { acc += sai; }
@@ -2727,7 +2741,11 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
          if ((t / G) == grp) { diag += dt; }
          dm &= dm - 1;
       }
-      // weights, terms of one step applied lowest lane first per weight
+      // weights: the terms of one step that hit a weight are added by the
+      // weight's lowest lane, in lane order, from the step's terms in LDS
+      // (the term values' space, free once the denominators are formed)
+      wave_sync();
+      double *xs = st_a_all;
 #pragma unroll
       for (int u = 0; u < 3; u++) {
          const bool chunk = live && (u * G < OL);   // uniform within the group
@@ -2737,15 +2755,23 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
             d_s = __shfl((int) sdist, ts[u], G);
             dist_s = __shfl(dist, ts[u], G);
          }
-         bool pending = chunk && (tg[u] >= 0) && (dr[u] || d_s);
+         const bool pending = chunk && (tg[u] >= 0) && (dr[u] || d_s);
          const double x = dr[u] ? ta[u] : dist_s * ta[u];
-         while (__ballot(pending)) {
-            if (pending) { atomicMin(&mins[tg[u]], gl); }
+         if (__ballot(pending)) {
+            if (pending) { atomicOr(&wmask[tg[u]], 1ull << lane); }
+            xs[lane] = x;
             wave_sync();
-            if (pending && mins[tg[u]] == gl) {
-               wgt[tg[u]] += x;
-               mins[tg[u]] = 0x7fffffff;
-               pending = false;
+            if (pending) {
+               const unsigned long long m = wmask[tg[u]];
+               if ((m & lanemask_lt(lane)) == 0ull) {
+                  double acc = wgt[tg[u]];
+                  acc += x;
+                  for (unsigned long long rest = m & (m - 1); rest; rest &= rest - 1) {
+                     acc += xs[__ffsll((long long) rest) - 1];
+                  }
+                  wgt[tg[u]] = acc;
+                  wmask[tg[u]] = 0ull;
+               }
             }
             wave_sync();
          }
