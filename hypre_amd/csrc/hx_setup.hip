@@ -1986,7 +1986,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
    __shared__ int vals[XF_HC];   // C key: stamp, then C-hat position; F neighbour: -2 - S index
    __shared__ double wgt[MC];
    __shared__ int chat_key[MC];
-   __shared__ int mins[MC];
+   __shared__ unsigned long long wmask[MC];   // lanes of a step that hit a weight
    __shared__ int seg_ki[WAVE];     // entry j: index of column i in row A_ja[j], or -1
    __shared__ double st_a[XR_OL];   // term values
    __shared__ int st_k[XR_OL];      // term storage index if its column is in C-hat, else -1
@@ -2151,7 +2151,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                ccol = p.f2c[chat_key[lane]];
                wgt[lane] = 0.0;
             }
-            if (lane < MC) { mins[lane] = 0x7fffffff; }
+            if (lane < MC) { wmask[lane] = 0ull; }
             // entry j: kind 1 C-hat (weight v), 2 strong F neighbour, 3 weak, 0 none
             int v = -1, sk = 0;
             if (lane < na) {
@@ -2270,14 +2270,28 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                   shas_i = (ki >= 0) && ((dg1 < 0) ? (sai > 0) : (sai < 0));
                   double acc = 0.0;
                   bool pend = shas_i;
-                  for (int q = ooff; q < ooff + olen; q++) {
-                     const int kq = st_k[q];
-                     if (kq >= 0) {
-                        if (pend && ki < kq) {
-                           acc += sai;
-                           pend = false;
+                  // four terms' LDS reads issued together, then folded in order
+                  for (int q0 = ooff; q0 < ooff + olen; q0 += 4) {
+                     int kq[4];
+                     double aq[4];
+#pragma unroll
+                     for (int v = 0; v < 4; v++) {
+                        kq[v] = -1;
+                        aq[v] = 0.0;
+                        if (q0 + v < ooff + olen) {
+                           kq[v] = st_k[q0 + v];
+                           aq[v] = st_a[q0 + v];
                         }
-                        acc += st_a[q];
+                     }
+#pragma unroll
+                     for (int v = 0; v < 4; v++) {
+                        if (kq[v] >= 0) {
+                           if (pend && ki < kq[v]) {
+                              acc += sai;
+                              pend = false;
+                           }
+                           acc += aq[v];
+                        }
                      }
                   }
                   if (pend) { acc += sai; }
@@ -2299,32 +2313,65 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                   hasd = true;   // weak connection
                   dval = aij;
                }
-               unsigned long long dm = __ballot(hasd);
-               while (dm) {
-                  const int t = __ffsll((long long) dm) - 1;
-                  diag += readlane_d(dval, t);
-                  dm &= dm - 1;
+               // the diagonal terms in entry order, folded by lane 0 from LDS
+               // (the term values' space, free once the denominators are
+               // formed), four reads in flight
+               double *xs = st_a;
+               const unsigned long long dm = __ballot(hasd);
+               wave_sync();
+               xs[lane] = dval;
+               wave_sync();
+               if (lane == 0) {
+                  double acc = diag;
+                  for (unsigned long long rest = dm; rest;) {
+                     int tq[4];
+                     double vq[4];
+#pragma unroll
+                     for (int v = 0; v < 4; v++) {
+                        tq[v] = -1;
+                        vq[v] = 0.0;
+                        if (rest) {
+                           tq[v] = __ffsll((long long) rest) - 1;
+                           vq[v] = xs[tq[v]];
+                           rest &= rest - 1;
+                        }
+                     }
+#pragma unroll
+                     for (int v = 0; v < 4; v++) {
+                        if (tq[v] >= 0) { acc += vq[v]; }
+                     }
+                  }
+                  diag = acc;
                }
+               diag = readlane_d(diag, 0);
                // weights: a_ij of a C-hat entry, distribute * a of a strong F
-               // neighbour with a nonzero denominator; terms of one step that
-               // hit the same weight are applied lowest lane first
+               // neighbour with a nonzero denominator; the terms of one step
+               // that hit a weight are added by the weight's lowest lane, in
+               // lane order, from the step's terms in LDS
+               wave_sync();
 #pragma unroll
                for (int u = 0; u < 3; u++) {
                   if (u * WAVE >= OL) { continue; }   // uniform
                   const int d_s = __shfl((int) sdist, ts[u], WAVE);
                   const double dist_s = shfl_d(dist, ts[u]);
-                  bool pending = (tg[u] >= 0) && (dr[u] || d_s);
+                  const bool pending = (tg[u] >= 0) && (dr[u] || d_s);
                   const double x = dr[u] ? ta[u] : dist_s * ta[u];
-                  while (__ballot(pending)) {
-                     if (pending) { atomicMin(&mins[tg[u]], lane); }
-                     wave_sync();
-                     if (pending && mins[tg[u]] == lane) {
-                        wgt[tg[u]] += x;
-                        mins[tg[u]] = 0x7fffffff;
-                        pending = false;
+                  if (pending) { atomicOr(&wmask[tg[u]], 1ull << lane); }
+                  xs[lane] = x;
+                  wave_sync();
+                  if (pending) {
+                     const unsigned long long m = wmask[tg[u]];
+                     if ((m & lanemask_lt(lane)) == 0ull) {
+                        double acc = wgt[tg[u]];
+                        acc += x;
+                        for (unsigned long long rest = m & (m - 1); rest; rest &= rest - 1) {
+                           acc += xs[__ffsll((long long) rest) - 1];
+                        }
+                        wgt[tg[u]] = acc;
+                        wmask[tg[u]] = 0ull;
                      }
-                     wave_sync();
                   }
+                  wave_sync();
                }
                wave_sync();
                if (lane < nch) {
@@ -2733,19 +2780,40 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
             dval = aij;
          }
       }
-      // both rows' diagonal terms, each row's in entry order
-      unsigned long long dm = __ballot(hasd);
-      while (dm) {
-         const int t = __ffsll((long long) dm) - 1;
-         const double dt = readlane_d(dval, t);
-         if ((t / G) == grp) { diag += dt; }
-         dm &= dm - 1;
+      // each row's diagonal terms in entry order, folded by the group's first
+      // lane from LDS (the term values' space, free once the denominators are
+      // formed), four reads in flight
+      double *xs = st_a_all;
+      const unsigned long long dm = __ballot(hasd) & gbits;
+      wave_sync();
+      xs[lane] = dval;
+      wave_sync();
+      if (gl == 0) {
+         double acc = diag;
+         for (unsigned long long rest = dm; rest;) {
+            int tq[4];
+            double vq[4];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+               tq[v] = -1;
+               vq[v] = 0.0;
+               if (rest) {
+                  tq[v] = __ffsll((long long) rest) - 1;
+                  vq[v] = xs[tq[v]];
+                  rest &= rest - 1;
+               }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+               if (tq[v] >= 0) { acc += vq[v]; }
+            }
+         }
+         diag = acc;
       }
+      diag = __shfl(diag, 0, G);
       // weights: the terms of one step that hit a weight are added by the
       // weight's lowest lane, in lane order, from the step's terms in LDS
-      // (the term values' space, free once the denominators are formed)
       wave_sync();
-      double *xs = st_a_all;
 #pragma unroll
       for (int u = 0; u < 3; u++) {
          const bool chunk = live && (u * G < OL);   // uniform within the group
@@ -3956,7 +4024,7 @@ __global__ __launch_bounds__(WAVE) void k_rap_seg(RapFastArgs p, int wsh, int ws
    claim.issue(lane);
    for (int wi = claim.take(lane); wi >= 0; wi = claim.take(lane)) {
       claim.issue(lane);   // the next row's claim is in flight while this row runs
-      const int ic = wi;
+      const int ic = p.rowlist ? p.rowlist[wi] : wi;
       for (int s = lane; s < p.rc; s += WAVE) { ra_keys[s] = -1; }
       wave_sync();
       bool defer = false;
@@ -4206,7 +4274,7 @@ __global__ __launch_bounds__(WAVE) void k_rap_seg(RapFastArgs p, int wsh, int ws
          }
       }
       if (!defer) {
-         const size_t o = (size_t) ic * p.acl;
+         const size_t o = (size_t) (p.rowlist ? wi : ic) * p.acl;
          for (int k = lane; k < nac; k += WAVE) {
             const int sl = ac_order[k];
             p.tmp_j[o + k] = ac_keys[sl];
@@ -4214,7 +4282,8 @@ __global__ __launch_bounds__(WAVE) void k_rap_seg(RapFastArgs p, int wsh, int ws
          }
          if (lane == 0) {
             p.cnt[ic] = nac;
-            p.src[ic] = (int) o;
+            // stage 0: the offset; stage 1 or 2: -2 - (2 * offset + stage - 1)
+            p.src[ic] = (p.stage == 0) ? (int) o : -2 - (2 * (int) o + p.stage - 1);
          }
       } else if (lane == 0) {
          p.src[ic] = -1;
@@ -4291,7 +4360,7 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
    const int *list = nullptr;
    int nwork = nc;
    int cur = 0;
-   // stage 0 runs k_rap_seg when P's rows fit W <= 16 lanes each (log2 W);
+   // stages 0 and 1 run k_rap_seg when P's rows fit W <= 16 lanes each (log2 W);
    // A rows of at most WA <= 32 entries are taken 64 / WA at a time (log2 WA)
    int pw = -1, aw = 6;
    {
@@ -4325,12 +4394,17 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
       f.claim = claim_counters();
       if (raregs[stage] > 0) { f.rcl = raregs[stage] * WAVE; }
       const size_t lds = rap_fast_lds(f.rc, f.rcl, f.ac, f.acl, raregs[stage]);
-      if (stage == 0 && pw >= 0) {
+      if (stage <= 1 && pw >= 0) {
          const size_t lds_s = rap_seg_lds(f.rc, f.rcl, f.ac, f.acl);
-         if (aw <= 5) {
-            k_rap_seg<RAP_RAREG, true><<<launch_blocks(nwork), WAVE, lds_s, stream()>>>(f, pw, aw);
+         const int nb = launch_blocks(nwork);
+         if (stage == 0 && aw <= 5) {
+            k_rap_seg<RAP_RAREG, true><<<nb, WAVE, lds_s, stream()>>>(f, pw, aw);
+         } else if (stage == 0) {
+            k_rap_seg<RAP_RAREG, false><<<nb, WAVE, lds_s, stream()>>>(f, pw, 6);
+         } else if (aw <= 5) {
+            k_rap_seg<2 * RAP_RAREG, true><<<nb, WAVE, lds_s, stream()>>>(f, pw, aw);
          } else {
-            k_rap_seg<RAP_RAREG, false><<<launch_blocks(nwork), WAVE, lds_s, stream()>>>(f, pw, 6);
+            k_rap_seg<2 * RAP_RAREG, false><<<nb, WAVE, lds_s, stream()>>>(f, pw, 6);
          }
       } else if (stage == 0) {
          k_rap_fast<RAP_RAREG><<<launch_blocks(nwork), WAVE, lds, stream()>>>(f);
