@@ -40,6 +40,7 @@ AMGSolver::~AMGSolver() { free_hierarchy(); }
 
 void AMGSolver::free_hierarchy()
 {
+   side_sync();   // solve-phase copies may still be filling
    for (auto &l : L) {
       l.Aown.free_all();
       l.P.free_all();
@@ -106,6 +107,20 @@ void AMGSolver::setup(const Csr &A0)
    L.reserve(std::max(p.max_levels, 1) + 1);
    L.emplace_back();
    L[0].A = &A0;
+   // solve-phase copies of the level operators: sliced ELL where the slices'
+   // padding stays within 25% of the entries (levels of at least 4096 rows).
+   // A level's copy is filled on the side stream while the level's RAP runs
+   // (a latency-bound kernel that leaves HBM bandwidth free), the coarsest
+   // level's after the loop.
+   int ell_started = 0;   // levels whose copy is under way
+   auto start_ell = [&](int j) {
+      Level &lv = L[j];
+      if (p.solve_format == 1 && ell_plan(*lv.A, lv.ell, 0.25, 4096, p.cycle_fp32 != 0)) {
+         side_after_main();
+         ell_fill(*lv.A, lv.ell, p.cycle_fp32 != 0, side_stream());
+      }
+      ell_started = j + 1;
+   };
    coarse_relax = p.relax_type[3];
    coarse_sweeps = p.num_sweeps[3];
    int level = 0;
@@ -183,7 +198,10 @@ void AMGSolver::setup(const Csr &A0)
       L.emplace_back();
       Level &nx = L[level + 1];
       Level &cur = L[level];
-      rap(cur.R, *cur.A, cur.P, nx.Aown);
+      start_ell(level);
+      // P's rows: one entry for a C point, at most P_max_elmts otherwise
+      const int pmx = aggressive ? p.agg_P_max_elmts : p.P_max_elmts;
+      rap(cur.R, *cur.A, cur.P, nx.Aown, pmx > 0 ? std::max(pmx, 1) : 0);
       nx.A = &nx.Aown;
       level++;
       // par_amg_setup.c:3136-3141 (the host-only switch to CLJP coarsening
@@ -271,13 +289,14 @@ void AMGSolver::setup(const Csr &A0)
          }
       }
    }
-   // solve-phase copies of the level operators: sliced ELL where the slices'
-   // padding stays within 25% of the entries (levels of at least 4096 rows)
+   for (int j = ell_started; j < num_levels; j++) { start_ell(j); }
+   // the solve phase runs after the copies are filled
+   main_after_side();
    for (int j = 0; j < num_levels; j++) {
       Level &lv = L[j];
       lv.Av = *lv.A;
       lv.Av.ell = nullptr;
-      if (p.solve_format == 1 && ell_build(*lv.A, lv.ell, 0.25, 4096, p.cycle_fp32 != 0)) {
+      if (lv.ell.stored > 0) {
          lv.Av.ell = &lv.ell;
          if (p.cycle_fp32) { lv.A32.ell = &lv.ell; }
       }

@@ -69,7 +69,11 @@ void        ensure_init();
 // points on the current device (called once per device by ensure_init)
 void        noalloc_attach_device();
 hipStream_t stream();
-void        sync();                       // hipStreamSynchronize(stream())
+void        sync();
+hipStream_t side_stream();
+void side_after_main();
+void main_after_side();
+void side_sync();                       // hipStreamSynchronize(stream())
 void       *dalloc(size_t bytes);         // caching device allocator
 void        dfree(void *p);
 void        release_cache();
@@ -113,7 +117,6 @@ struct Ell
 {
    int       nslices = 0;
    long long stored = 0;          // 64 * sum of slice widths
-   long long packed = 0;          // entries of slices with 16-bit columns
    int      *sw = nullptr;        // exclusive prefix of slice widths (nslices + 1)
    int      *j = nullptr;         // 32-bit columns (all slices)
    unsigned short *j16 = nullptr; // 16-bit column offsets (slices with mode 1)
@@ -142,6 +145,10 @@ struct Csr
 // (1 + max_pad) nnz) and A has at least min_rows rows; returns whether it did.
 // fp32: also the FP32 values.
 bool ell_build(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32);
+// ell_build in two parts: ell_plan sizes and allocates the copy on the library
+// stream (false: no copy for this operator), ell_fill launches the copy on s
+bool ell_plan(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32);
+void ell_fill(const Csr &A, Ell &E, bool fp32, hipStream_t s);
 
 void csr_copy(const Csr &src, Csr &dst);
 void csr_to_host(const Csr &A, std::vector<int> &ia, std::vector<int> &ja, std::vector<double> &a);
@@ -254,7 +261,9 @@ void correct_cf_marker(int n, int *CF, const int *f2c1, const int *CFN);
 void mod_multipass_interp(const Csr &A, const Csr &S, int *CF, int nc, double trunc_factor,
                           int max_elmts, Csr &P);
 void transpose(const Csr &P, Csr &R);
-void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac);
+// p_row_bound: an upper bound of P's row lengths when the caller knows one
+// (truncation to P_max_elmts), else <= 0
+void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac, int p_row_bound = 0);
 // dense column-major copy of a small CSR matrix (n <= 64)
 void csr_to_dense_colmajor(const Csr &A, double *M_dev);
 

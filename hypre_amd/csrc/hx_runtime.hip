@@ -49,6 +49,45 @@ hipStream_t stream()
 
 void sync() { HX_HIP(hipStreamSynchronize(stream())); }
 
+// A second stream for setup work that runs beside the library stream (the
+// solve-phase copies of the level operators).  Work is handed over in both
+// directions with events: side_after_main() makes later side-stream work wait
+// for everything enqueued on the library stream so far (so no block the
+// allocator recycled is still in use there), main_after_side() the reverse.
+// Both streams have the default priority.
+static hipStream_t g_side = nullptr;
+static hipEvent_t  g_ev_main = nullptr, g_ev_side = nullptr;
+
+hipStream_t side_stream()
+{
+   ensure_init();
+   if (!g_side) {
+      HX_HIP(hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking));
+      HX_HIP(hipEventCreateWithFlags(&g_ev_main, hipEventDisableTiming));
+      HX_HIP(hipEventCreateWithFlags(&g_ev_side, hipEventDisableTiming));
+   }
+   return g_side;
+}
+
+void side_after_main()
+{
+   hipStream_t s = side_stream();
+   HX_HIP(hipEventRecord(g_ev_main, stream()));
+   HX_HIP(hipStreamWaitEvent(s, g_ev_main, 0));
+}
+
+void main_after_side()
+{
+   if (!g_side) { return; }
+   HX_HIP(hipEventRecord(g_ev_side, g_side));
+   HX_HIP(hipStreamWaitEvent(stream(), g_ev_side, 0));
+}
+
+void side_sync()
+{
+   if (g_side) { HX_HIP(hipStreamSynchronize(g_side)); }
+}
+
 int *device_flags()
 {
    ensure_init();
@@ -129,6 +168,7 @@ void release_cache()
 {
    if (!g_inited) { return; }
    (void) hipStreamSynchronize(g_stream);
+   if (g_side) { (void) hipStreamSynchronize(g_side); }
    for (auto &kv : g_free) { (void) hipFree(kv.second); }
    g_free.clear();
    g_cached_bytes = 0;
@@ -428,7 +468,7 @@ void Ell::free_all()
 {
    dfree(sw); dfree(j); dfree(j16); dfree(j8); dfree(jb); dfree(mode); dfree(a); dfree(a32);
    sw = j = nullptr; j16 = nullptr; j8 = nullptr; jb = mode = nullptr; a = nullptr; a32 = nullptr;
-   nslices = 0; stored = 0; packed = 0;
+   nslices = 0; stored = 0;
 }
 
 void Csr::alloc(int n, int nc, long long nz, bool values)

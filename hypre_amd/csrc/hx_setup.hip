@@ -4320,7 +4320,7 @@ __global__ void k_rap_compact(int nc, const int *__restrict__ cnt, const int *__
    }
 }
 
-void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
+void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac, int p_row_bound)
 {
    int nc = R.nrows;
    TimedLaunch tl("setup_rap", 0.0);
@@ -4364,7 +4364,7 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac)
    // A rows of at most WA <= 32 entries are taken 64 / WA at a time (log2 WA)
    int pw = -1, aw = 6;
    {
-      const int pmax = csr_max_row(P);
+      const int pmax = p_row_bound > 0 ? p_row_bound : csr_max_row(P);
       if (pmax <= 16) {
          pw = 0;
          while ((1 << pw) < pmax) { pw++; }

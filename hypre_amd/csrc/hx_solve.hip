@@ -618,7 +618,10 @@ __global__ __launch_bounds__(64) void k_ell_pack(int n, int ns, const int *__res
    }
 }
 
-bool ell_build(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32)
+__global__ void k_d2f(int n, const double *__restrict__ x, float *__restrict__ y);
+static int blas_grid(int n);
+
+bool ell_plan(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32)
 {
    E.free_all();
    if (A.nrows < std::max(min_rows, 1) || A.nnz <= 0 || A.ncols <= 0) { return false; }
@@ -637,20 +640,32 @@ bool ell_build(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32)
    E.sw = w;
    E.j = dalloc_n<int>((size_t) stored);
    E.a = dalloc_n<double>((size_t) stored);
-   k_ell_fill<<<std::min(ns, 65536), 64, 0, stream()>>>(A.nrows, ns, A.ia, A.ja, A.a, E.sw, E.j,
-                                                         E.a);
-   HX_LAUNCH_CHECK();
    E.j16 = dalloc_n<unsigned short>((size_t) stored);
    E.j8 = dalloc_n<unsigned char>((size_t) stored);
    E.jb = dalloc_n<int>((size_t) (stored / 64) + 1);
    E.mode = dalloc_n<int>((size_t) ns);
-   k_ell_pack<<<std::min(ns, 65536), 64, 0, stream()>>>(A.nrows, ns, A.ia, E.sw, E.j, E.j16, E.j8,
-                                                         E.jb, E.mode);
+   if (fp32) { E.a32 = dalloc_n<float>((size_t) stored); }
+   return true;
+}
+
+void ell_fill(const Csr &A, Ell &E, bool fp32, hipStream_t s)
+{
+   const int ns = E.nslices;
+   k_ell_fill<<<std::min(ns, 65536), 64, 0, s>>>(A.nrows, ns, A.ia, A.ja, A.a, E.sw, E.j, E.a);
+   HX_LAUNCH_CHECK();
+   k_ell_pack<<<std::min(ns, 65536), 64, 0, s>>>(A.nrows, ns, A.ia, E.sw, E.j, E.j16, E.j8, E.jb,
+                                                  E.mode);
    HX_LAUNCH_CHECK();
    if (fp32) {
-      E.a32 = dalloc_n<float>((size_t) stored);
-      d2f((int) stored, E.a, E.a32);
+      k_d2f<<<blas_grid((int) E.stored), BS, 0, s>>>((int) E.stored, E.a, E.a32);
+      HX_LAUNCH_CHECK();
    }
+}
+
+bool ell_build(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32)
+{
+   if (!ell_plan(A, E, max_pad, min_rows, fp32)) { return false; }
+   ell_fill(A, E, fp32, stream());
    return true;
 }
 
