@@ -153,9 +153,9 @@ def north_star(hx, steps, N=320):
                                "rel_error_vs_fp64_solution": err},
         "speedup_over_plain": round(plain["step_s"] / mixed["step_s"], 4),
         "learned_decision_s": round(decide_s, 4),
-        "learned_decision": "once per matrix, before the timed steps: one FP32-cycle solve with "
-                            "the predicted settings and one with the defaults; kept if faster "
-                            "per digit of residual reduction",
+        "learned_decision": "once per matrix, before the timed steps: FP32-cycle solves with "
+                            "the predicted settings and with the defaults, best of two each; "
+                            "kept if faster per digit of residual reduction",
         "one_off_solve_with_decision_s": round(mixed["step_s"] + decide_s, 4),
         "steps": steps,
     }
@@ -239,7 +239,7 @@ def main():
                         "iterations_learned": lst.iterations,
                         "iterations_default": dst.iterations,
                         "decision_s": round(decide_s, 4),
-                        "decided": "once before the timed steps, from two untimed solves"}
+                        "decided": "once before the timed steps, from the best of two untimed solves with each candidate"}
 
     if args.probe:
         x.fill(0.0)

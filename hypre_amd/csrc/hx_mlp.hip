@@ -37,6 +37,10 @@ struct hx_param_model_struct
    float *Y = nullptr, *T = nullptr, *D3 = nullptr, *D2 = nullptr, *D1 = nullptr, *DH = nullptr;
    float *loss = nullptr;        // device scalar
    int    oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0, oW3 = 0, ob3 = 0;
+   // prior of the cost-model reading: grid settings kept on the fine and
+   // coarse levels unless another is predicted faster by more than the margin
+   int    prior_f = -1, prior_c = -1;
+   double prior_margin = 0.0;
    // settings of the cost-model reading (empty: per-level reading)
    std::vector<double> g_theta, g_omega;
    std::vector<int>    g_sweeps;
@@ -475,6 +479,24 @@ HYPRE_Int HYPRE_MI355X_ParamModelSetGrid(HYPRE_MI355X_ParamModel model, HYPRE_In
       m->g_theta.assign(theta, theta + ng);
       m->g_omega.assign(omega, omega + ng);
       m->g_sweeps.assign(sweeps, sweeps + ng);
+      m->prior_f = m->prior_c = -1;
+      m->prior_margin = 0.0;
+   });
+}
+
+HYPRE_Int HYPRE_MI355X_ParamModelSetPrior(HYPRE_MI355X_ParamModel model, HYPRE_Int fine,
+                                          HYPRE_Int coarse, HYPRE_Real margin)
+{
+   return guard([&] {
+      auto *m = as_model(model);
+      const int G = (int) m->g_theta.size();
+      if (G == 0) { fail(HYPRE_ERROR_ARG, "the prior needs a grid (ParamModelSetGrid)"); }
+      if (fine < -1 || fine >= G || coarse < -1 || coarse >= G || !(margin >= 0.0)) {
+         fail(HYPRE_ERROR_ARG, "prior index outside the grid or negative margin");
+      }
+      m->prior_f = fine;
+      m->prior_c = coarse;
+      m->prior_margin = margin;
    });
 }
 
@@ -499,10 +521,18 @@ HYPRE_Int HYPRE_MI355X_BoomerAMGSetParamModel(HYPRE_Solver solver, HYPRE_MI355X_
       HX_LAUNCH_CHECK();
       d2h(y.data(), m->Y, sizeof(float) * m->nout);
       if (!m->g_theta.empty()) {
-         // cost model: the first predicted minimum of each group
+         // cost model: the first predicted minimum of each group, unless the
+         // group's prior setting is predicted within the margin of it
          const int G = (int) m->g_theta.size();
-         const int gf = (int) (std::min_element(y.begin(), y.begin() + G) - y.begin());
-         const int gc = (int) (std::min_element(y.begin() + G, y.begin() + 2 * G) - y.begin()) - G;
+         int gf = (int) (std::min_element(y.begin(), y.begin() + G) - y.begin());
+         int gc = (int) (std::min_element(y.begin() + G, y.begin() + 2 * G) - y.begin()) - G;
+         if (m->prior_f >= 0 && (double) y[gf] > (double) y[m->prior_f] - m->prior_margin) {
+            gf = m->prior_f;
+         }
+         if (m->prior_c >= 0 &&
+             (double) y[G + gc] > (double) y[G + m->prior_c] - m->prior_margin) {
+            gc = m->prior_c;
+         }
          for (int l = 0; l < kModelLevels; l++) {
             const int g = (l == 0) ? gf : gc;
             HYPRE_MI355X_BoomerAMGSetLevelStrongThreshold(solver, l, m->g_theta[g]);

@@ -195,6 +195,12 @@ HYPRE_Int HYPRE_MI355X_MatrixFeatures(HYPRE_ParCSRMatrix A, HYPRE_Real *features
 HYPRE_Int HYPRE_MI355X_ParamModelSetGrid(HYPRE_MI355X_ParamModel model, HYPRE_Int ng,
                                          const HYPRE_Real *theta, const HYPRE_Real *omega,
                                          const HYPRE_Int *sweeps);
+/* prior of the cost-model reading: grid indices of the settings kept on the
+ * fine level and on the coarse levels (-1: none) unless another setting is
+ * predicted faster by more than `margin` (in relative time); set after the
+ * grid, which clears it */
+HYPRE_Int HYPRE_MI355X_ParamModelSetPrior(HYPRE_MI355X_ParamModel model, HYPRE_Int fine,
+                                          HYPRE_Int coarse, HYPRE_Real margin);
 /* features of A -> model -> per-level theta, omega, sweeps of the solver
  * (with a grid: the predicted fastest setting on level 0 and on levels
  * 1..24) */

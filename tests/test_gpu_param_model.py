@@ -146,6 +146,49 @@ def test_cost_model_applies_its_predicted_fastest_settings(hx, L, PM):
         m.destroy()
 
 
+def test_cost_model_prior_keeps_the_constant_rule_within_its_margin(hx, L, PM):
+    """With a prior (HYPRE_MI355X_ParamModelSetPrior), a group keeps the
+    prior setting unless another is predicted faster by more than the margin:
+    a margin above every predicted difference applies the prior, margin 0 the
+    plain argmin, and the library's choice equals decode_costs' on the same
+    outputs for margins in between."""
+    say("cost model prior")
+    dims = (8, 64, 64, L.NOUT)
+    A = hx.ParCSRMatrix.aniso7(16, 0.05, 0.4)
+    prior = (L.GRID[7], L.GRID[18])
+    for seed in (21, 22):
+        p = PM.init_params(dims, seed=seed).astype(np.float32)
+        m = L.ParamModel(*dims, params=p, grid=L.GRID)
+        y = m.forward(L.matrix_features(A)[None, :].astype(np.float32))[0]
+        plain = L.decode_costs(y)
+        m.set_prior(*prior, 2.0)
+        assert m.predict(A) == prior
+        m.set_prior(*prior, 0.0)
+        assert m.predict(A) == plain
+        G = len(L.GRID)
+        gaps = sorted({float(y[L.GRID.index(prior[0])] - y[:G].min()),
+                       float(y[G + L.GRID.index(prior[1])] - y[G:].min())})
+        for margin in [0.5 * gaps[0], 0.5 * (gaps[0] + gaps[1]), 2 * gaps[1] + 1e-3]:
+            m.set_prior(*prior, margin)
+            want = L.decode_costs(y, L.GRID, (prior, margin))
+            assert m.predict(A) == want
+            # the settings the library applies are the decoded ones
+            a1 = hx.BoomerAMG(keep_setup_data=1, max_iter=1, tol=0.0)
+            m.apply(a1, A)
+            a1.setup(A)
+            a2 = hx.BoomerAMG(keep_setup_data=1, max_iter=1, tol=0.0)
+            th, om, sw = zip(*L.per_level(*want))
+            a2.set_level_params(theta=th, omega=om, sweeps=sw)
+            a2.setup(A)
+            assert a1.num_levels() == a2.num_levels()
+            for lev in range(a1.num_levels()):
+                assert np.array_equal(a1.level_matrix(lev, "A").a, a2.level_matrix(lev, "A").a)
+            a1.destroy()
+            a2.destroy()
+        m.destroy()
+    A.destroy()
+
+
 HELD_OUT = [("aniso", 128, s) for s in (1001, 1002, 1003, 1004, 1005, 1006)] + \
            [("jump", 96, s) for s in (2001, 2002, 2003)]
 
