@@ -349,7 +349,7 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": "k_spmv_ell<double,double,DOT> (PCG s = A p on the fine level; sliced ELL "
-                      "with 16-bit column offsets where a slice allows them)",
+                      "with 8- or 16-bit column offsets where a slice allows them)",
             "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
