@@ -103,6 +103,7 @@ void AMGSolver::setup(const Csr &A0)
                             "modified multipass interpolation (agg_interp_type 8) only");
    }
    free_hierarchy();
+   Range range("AMGsetup");   // par_amg_setup.c:251
    TimedLaunch tl("setup", 0.0);
    L.reserve(std::max(p.max_levels, 1) + 1);
    L.emplace_back();
@@ -125,6 +126,7 @@ void AMGSolver::setup(const Csr &A0)
    coarse_sweeps = p.num_sweeps[3];
    int level = 0;
    while (true) {
+      Range lrange("AMG Level-%d", level);   // par_amg_setup.c:976, 3103
       Level &lv = L[level];
       const Csr &A = *lv.A;
       const int fine_size = A.nrows;
@@ -138,11 +140,18 @@ void AMGSolver::setup(const Csr &A0)
       }
       double *s_thr = dalloc_n<double>((size_t) std::max(fine_size, 1));
       int *s_flg = dalloc_n<int>((size_t) std::max(fine_size, 1));
-      create_s(A, theta(level), p.max_row_sum, lv.S, stc, lv.l1, l1_zero_flag(level), s_thr, s_flg);
+      {
+         Range r("CreateS");   // par_strength.c:538
+         create_s(A, theta(level), p.max_row_sum, lv.S, stc, lv.l1, l1_zero_flag(level), s_thr,
+                  s_flg);
+      }
       lv.CF = dalloc_n<int>((size_t) fine_size);
       if (p.keep_setup_data) { lv.measure = dalloc_n<double>((size_t) fine_size); }
-      lv.pmis_rounds = (p.coarsen_type == 10) ? hmis(lv.S, nullptr, lv.CF, lv.measure, stc)
-                                              : pmis(lv.S, nullptr, lv.CF, lv.measure, stc);
+      {
+         Range r("PMIS");   // par_coarsen.c:2815
+         lv.pmis_rounds = (p.coarsen_type == 10) ? hmis(lv.S, nullptr, lv.CF, lv.measure, stc)
+                                                 : pmis(lv.S, nullptr, lv.CF, lv.measure, stc);
+      }
       dfree(stc);
       int *f2c = dalloc_n<int>((size_t) fine_size + 1);
       const bool aggressive = level < p.agg_num_levels;
@@ -152,10 +161,17 @@ void AMGSolver::setup(const Csr &A0)
          // par_amg_setup.c:1799-1806 corrects the split with the result
          const int nc1 = coarse_map(fine_size, lv.CF, f2c);
          Csr S2;
-         create_2nd_s(lv.S, lv.CF, f2c, nc1, S2);
+         {
+            Range r("Create2ndS");   // par_strength.c:3041
+            create_2nd_s(lv.S, lv.CF, f2c, nc1, S2);
+         }
          if (nc1 > 0) {
             int *CFN = dalloc_n<int>((size_t) nc1);
-            pmis(S2, nullptr, CFN, nullptr, nullptr, 3);
+            {
+               Range r("PMIS");
+               pmis(S2, nullptr, CFN, nullptr, nullptr, 3);
+            }
+            Range r("CorrectCFMarker");   // par_strength.c:3127
             correct_cf_marker(fine_size, lv.CF, f2c, CFN);
             dfree(CFN);
          }
@@ -187,6 +203,7 @@ void AMGSolver::setup(const Csr &A0)
          mod_multipass_interp(A, lv.S, lv.CF, coarse_size, p.agg_trunc_factor, p.agg_P_max_elmts,
                               lv.P);
       } else {
+         Range r("ExtPIInterp");   // par_lr_interp.c:5564
          extpi_interp(A, lv.S, lv.CF, f2c, coarse_size, p.trunc_factor, p.P_max_elmts, lv.P, s_thr,
                       s_flg);
       }
@@ -201,7 +218,10 @@ void AMGSolver::setup(const Csr &A0)
       start_ell(level);
       // P's rows: one entry for a C point, at most P_max_elmts otherwise
       const int pmx = aggressive ? p.agg_P_max_elmts : p.P_max_elmts;
-      rap(cur.R, *cur.A, cur.P, nx.Aown, pmx > 0 ? std::max(pmx, 1) : 0);
+      {
+         Range r("TripleMat-RAP");   // par_csr_triplemat.c:952
+         rap(cur.R, *cur.A, cur.P, nx.Aown, pmx > 0 ? std::max(pmx, 1) : 0);
+      }
       nx.A = &nx.Aown;
       level++;
       // par_amg_setup.c:3136-3141 (the host-only switch to CLJP coarsening
@@ -225,6 +245,8 @@ void AMGSolver::setup(const Csr &A0)
    // option 1 for l1-Jacobi, evaluated after it there and so taking precedence
    auto l1_gs = [](int t) { return t == 8 || t == 13 || t == 14; };
    for (int j = 0; j < num_levels; j++) {
+      Range lrange("AMG Level-%d", j);    // par_amg_setup.c:3293-3297
+      Range rrange("Relaxation-%d", j);
       Level &lv = L[j];
       const int n = lv.A->nrows;
       int opt = 0;
@@ -348,6 +370,7 @@ void AMGSolver::relax_any32(int type, int lev, const Csr &A, const float *f, con
 
 void AMGSolver::cycle(const double *f, double *u, bool u_zero)
 {
+   Range range("AMGCycle");   // par_cycle.c:119
    const int nl = num_levels;
    // level 0 works on the caller's buffers; the result must end in `u`
    double *cur0 = u;
@@ -359,6 +382,7 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
    bool zero = u_zero;
    bool formed = false;   // this level's first sweep was formed by the restriction
    for (int l = 0; l < nl - 1; l++) {
+      Range lrange("AMG Level-%d", l);   // par_cycle.c:302
       const Level &lv = L[l];
       const Csr &A = lv.Av;
       if (formed) {
@@ -366,6 +390,7 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
          zero = false;
       }
       for (int s = formed ? 1 : 0; s < sweeps(l, 1); s++) {
+         Range r("Relaxation");   // par_cycle.c:430
          // level-0 sweeps from a zero guess are timed as a class of their own
          const char *cls = (l == 0) ? (zero ? "relaxz_A0" : "relax_A0") : "relax";
          relax_any(prm.relax_type[1], l, A, F[l], cur[l], lv.l1, spare[l], zero, cls);
@@ -373,7 +398,11 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
          zero = false;
       }
       // residual and restriction (par_cycle.c:755-790)
-      spmv(A, -1.0, cur[l], 1.0, F[l], lv.r, l == 0 ? "resid_A0" : "spmv");
+      {
+         Range r("Residual");   // par_cycle.c:748
+         spmv(A, -1.0, cur[l], 1.0, F[l], lv.r, l == 0 ? "resid_A0" : "spmv");
+      }
+      Range rr("Restriction");   // par_cycle.c:768
       // a next level that opens with an l1-Jacobi sweep from a zero guess gets
       // that sweep, u = (0 + w f)/l1, from the restriction's epilogue
       formed = prm.solve_format == 1 && (l + 1 < nl - 1) && prm.relax_type[1] == 18 &&
@@ -388,6 +417,8 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
    }
    {
       const int l = nl - 1;
+      Range lrange("AMG Level-%d", l);
+      Range r("Coarse solve");   // par_cycle.c:381
       const Level &lv = L[l];
       if (coarse_relax == 9 && Amat) {
          gselim_solve(coarse_n, Amat, F[l], cur[l]);
@@ -400,11 +431,16 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
       }
    }
    for (int l = nl - 2; l >= 0; l--) {
+      Range lrange("AMG Level-%d", l);   // par_cycle.c:806, 860
       const Level &lv = L[l];
       // interpolation u += P e (par_cycle.c:820-840)
-      spmv(lv.P, 1.0, cur[l + 1], 1.0, cur[l], cur[l], "interp");
+      {
+         Range r("Interpolation");   // par_cycle.c:827
+         spmv(lv.P, 1.0, cur[l + 1], 1.0, cur[l], cur[l], "interp");
+      }
       const char *cls = (l == 0) ? "relax_A0" : "relax";
       for (int s = 0; s < sweeps(l, 2); s++) {
+         Range r("Relaxation");
          relax_any(prm.relax_type[2], l, lv.Av, F[l], cur[l], lv.l1, spare[l], false, cls);
          std::swap(cur[l], spare[l]);
       }
@@ -420,6 +456,7 @@ void AMGSolver::cycle(const double *f, double *u, bool u_zero)
 // computes in FP64.
 void AMGSolver::cycle32(const double *f, double *u)
 {
+   Range range("AMGCycle");
    const int nl = num_levels;
    std::vector<float *> cur(nl), spare(nl);
    for (int l = 0; l < nl; l++) { cur[l] = L[l].u32; spare[l] = L[l].ut32; }
@@ -907,6 +944,7 @@ HYPRE_Int HYPRE_ParCSRPCGSetup(HYPRE_Solver solver, HYPRE_ParCSRMatrix A, HYPRE_
                                HYPRE_ParVector x)
 {
    return guard([&] {
+      Range range("PCG-Setup");   // pcg.c:215
       PCGSolver *s = as_pcg(solver);
       if (!A) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "NULL matrix"); }
       double t0 = wall_seconds();
@@ -951,6 +989,7 @@ HYPRE_Int HYPRE_ParCSRPCGSolve(HYPRE_Solver solver, HYPRE_ParCSRMatrix A, HYPRE_
                                HYPRE_ParVector x)
 {
    return guard([&] {
+      Range range("PCG-Solve");   // pcg.c:379
       PCGSolver *s = as_pcg(solver);
       if (!A) { g_error_arg = 2; fail(HYPRE_ERROR_ARG, "NULL matrix"); }
       if (!b) { g_error_arg = 3; fail(HYPRE_ERROR_ARG, "NULL rhs"); }

@@ -103,6 +103,31 @@ struct TimedLaunch
    ~TimedLaunch();
 };
 
+// Profiling ranges named as the reference's GPU profiling ranges
+// (src/utilities/device_markers.c), emitted through roctx while enabled
+// (HYPRE_MI355X_SetProfilingRanges or HYPRE_MI355X_ROCTX=1 in the
+// environment); closing a range waits for the library stream, as the
+// reference's StreamSync0, so that a range covers its kernels.  When disabled
+// a range costs one branch.
+bool ranges_on();
+void range_push(const char *name);
+void range_pop();
+struct Range
+{
+   bool on;
+   explicit Range(const char *name) : on(ranges_on())
+   {
+      if (on) { range_push(name); }
+   }
+   Range(const char *fmt, int v);   // "AMG Level-%d" and the like
+   ~Range()
+   {
+      if (on) { range_pop(); }
+   }
+   Range(const Range &) = delete;
+   Range &operator=(const Range &) = delete;
+};
+
 // ------------------------------------------------------------------ CSR
 // Sliced-ELL copy of an operator for the solve phase: slices of 64
 // consecutive rows; a slice of width w (its longest row) stores 64*w entries

@@ -2,7 +2,11 @@
 // allocator, prefix scan / reductions, HIP-event kernel timing, CSR helpers.
 #include "hx_internal.h"
 
+#include <dlfcn.h>
+
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -86,6 +90,68 @@ void main_after_side()
 void side_sync()
 {
    if (g_side) { HX_HIP(hipStreamSynchronize(g_side)); }
+}
+
+// ------------------------------------------------------------------ profiling ranges
+// roctx is loaded when ranges are first enabled (the rocprofiler-sdk roctx
+// library that rocprofv3 --marker-trace records, else the older roctx64).
+static int (*g_roctx_push)(const char *) = nullptr;
+static int (*g_roctx_pop)() = nullptr;
+static int g_ranges = -1;          // -1: environment not read yet
+static long long g_range_pushed = 0, g_range_open = 0;
+
+static bool roctx_load()
+{
+   if (g_roctx_push) { return true; }
+   static const char *libs[] = {"librocprofiler-sdk-roctx.so.1", "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1",
+                                "libroctx64.so.4", "/opt/rocm/lib/libroctx64.so.4"};
+   for (const char *name : libs) {
+      void *h = dlopen(name, RTLD_NOW | RTLD_GLOBAL);
+      if (!h) { continue; }
+      auto push = reinterpret_cast<int (*)(const char *)>(dlsym(h, "roctxRangePushA"));
+      auto pop = reinterpret_cast<int (*)()>(dlsym(h, "roctxRangePop"));
+      if (push && pop) {
+         g_roctx_push = push;
+         g_roctx_pop = pop;
+         return true;
+      }
+   }
+   return false;
+}
+
+bool ranges_on()
+{
+   if (g_ranges < 0) {
+      const char *e = std::getenv("HYPRE_MI355X_ROCTX");
+      g_ranges = (e && std::atoi(e) != 0 && roctx_load()) ? 1 : 0;
+   }
+   return g_ranges == 1;
+}
+
+void range_push(const char *name)
+{
+   g_roctx_push(name);
+   g_range_pushed++;
+   g_range_open++;
+}
+
+void range_pop()
+{
+   // src/utilities/device_markers.c:117-131: the stream sync inside a range
+   g_roctx_push("StreamSync0");
+   (void) hipStreamSynchronize(stream());
+   g_roctx_pop();
+   g_roctx_pop();
+   g_range_open--;
+}
+
+Range::Range(const char *fmt, int v) : on(ranges_on())
+{
+   if (on) {
+      char name[64];
+      std::snprintf(name, sizeof(name), fmt, v);
+      range_push(name);
+   }
 }
 
 int *device_flags()
@@ -515,6 +581,24 @@ void csr_from_host(Csr &A, int n, int ncols, const int *ia, const int *ja, const
 
 // ------------------------------------------------------------------ C API
 extern "C" {
+
+HYPRE_Int HYPRE_MI355X_SetProfilingRanges(HYPRE_Int on)
+{
+   if (on && !hx::roctx_load()) {
+      hx::set_error(HYPRE_ERROR_GENERIC, "roctx library not found");
+      hx::g_ranges = 0;
+      return hx::g_error_flag;
+   }
+   hx::g_ranges = on ? 1 : 0;
+   return hx::g_error_flag;
+}
+
+HYPRE_Int HYPRE_MI355X_GetProfilingRanges(HYPRE_Int *pushed, HYPRE_Int *open)
+{
+   if (pushed) { *pushed = (HYPRE_Int) hx::g_range_pushed; }
+   if (open) { *open = (HYPRE_Int) hx::g_range_open; }
+   return hx::g_error_flag;
+}
 
 HYPRE_Int HYPRE_MI355X_SetKernelTiming(HYPRE_Int on)
 {

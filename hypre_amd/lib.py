@@ -161,6 +161,8 @@ SIGNATURES = {
     "HYPRE_MI355X_BoomerAMGSetParamModel": (I, [P, P, P]),
     "HYPRE_MI355X_ParamModelSetGrid": (I, [P, I, DP, DP, IP]),
     "HYPRE_MI355X_ParamModelSetPrior": (I, [P, I, I, D]),
+    "HYPRE_MI355X_SetProfilingRanges": (I, [I]),
+    "HYPRE_MI355X_GetProfilingRanges": (I, [IP, IP]),
     "HYPRE_MI355X_SetKernelTiming": (I, [I]),
     "HYPRE_MI355X_SetKernelTimingClass": (I, [C.c_char_p]),
     "HYPRE_MI355X_ResetKernelTiming": (I, []),

@@ -218,6 +218,20 @@ HYPRE_Int HYPRE_MI355X_ResetKernelTiming(void);
 HYPRE_Int HYPRE_MI355X_GetKernelTiming(const char *name, HYPRE_Real *total_ms,
                                        HYPRE_Int *launches, HYPRE_Real *bytes);
 
+/* ---- profiling ranges ---------------------------------------------------
+ * roctx ranges with the names of the reference's GPU profiling ranges
+ * (src/utilities/device_markers.c; "PCG-Setup", "PCG-Solve", "AMGsetup",
+ * "AMG Level-<l>", "CreateS", "PMIS", "ExtPIInterp", "TripleMat-RAP",
+ * "AMGCycle", "Relaxation", "Residual", "Restriction", "Interpolation",
+ * "Coarse solve"), for rocprofv3 --marker-trace.  Off by default (also
+ * enabled by HYPRE_MI355X_ROCTX=1 in the environment); closing a range waits
+ * for the library stream, as the reference does, so timings with ranges on
+ * include those waits.  HYPRE_ERROR_GENERIC when the roctx library cannot be
+ * loaded. */
+HYPRE_Int HYPRE_MI355X_SetProfilingRanges(HYPRE_Int on);
+/* ranges opened so far and ranges still open */
+HYPRE_Int HYPRE_MI355X_GetProfilingRanges(HYPRE_Int *pushed, HYPRE_Int *open);
+
 /* ---- allocation-free variant (no counterpart in hypre) ------------------
  * For callers that cannot allocate device or pinned memory: code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc.
