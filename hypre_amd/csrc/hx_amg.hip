@@ -143,7 +143,7 @@ void AMGSolver::setup(const Csr &A0)
       {
          Range r("CreateS");   // par_strength.c:538
          create_s(A, theta(level), p.max_row_sum, lv.S, stc, lv.l1, l1_zero_flag(level), s_thr,
-                  s_flg);
+                  s_flg, level == 0);
       }
       lv.CF = dalloc_n<int>((size_t) fine_size);
       if (p.keep_setup_data) { lv.measure = dalloc_n<double>((size_t) fine_size); }

@@ -242,9 +242,11 @@ void   gselim_solve32(int n, const double *Amat_dev, const float *f, float *u);
 // l1 norms of A's rows (l1, n doubles; *l1_zero set to 1 on a zero norm) and
 // each row's strength threshold and flags (thr_out, flg_out: n each; bit 0
 // negative diagonal, bit 1 all entries weak) for extpi_interp
+// column_runs: A's rows hold runs of consecutive columns (the fine-grid
+// operator of a stencil in its natural order; a hint for the column counts)
 void create_s(const Csr &A, double theta, double max_row_sum, Csr &S, int *st_count = nullptr,
               double *l1 = nullptr, int *l1_zero = nullptr, double *thr_out = nullptr,
-              int *flg_out = nullptr);
+              int *flg_out = nullptr, bool column_runs = false);
 // l1 norms option 1 of hypre_ParCSRComputeL1Norms
 void l1_norms(const Csr &A, double *l1);
 // option 4 on one rank (signed diagonal), for the l1 hybrid Gauss-Seidel types

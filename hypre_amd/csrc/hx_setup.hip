@@ -364,7 +364,7 @@ constexpr int SF_ROWS = 256;
 constexpr int SF_SLOTS = 4096;
 constexpr int SF_U = 4;
 
-template <int K>
+template <int K, bool RUNS>
 __global__ __launch_bounds__(256) void k_strength_fill(int n, const int *__restrict__ ia,
                                                        const int *__restrict__ ja,
                                                        const double *__restrict__ a,
@@ -422,7 +422,14 @@ __global__ __launch_bounds__(256) void k_strength_fill(int n, const int *__restr
             if (strong) {
                S_ja[o + c + __popcll(m & ((1ULL << gl) - 1))] = j;
                if (stc) {
-                  unsigned h = ((unsigned) j * 2654435761u) >> (32 - ilog2(SF_SLOTS));
+                  // RUNS (a fine-grid stencil operator in its natural order): a
+                  // block's columns are a few runs of consecutive indices; keyed
+                  // by the column's low bits they fill consecutive slots, and the
+                  // flush below adds to consecutive counters (one memory
+                  // transaction for many lanes).  Otherwise (coarse operators,
+                  // scattered columns) a multiplicative hash spreads them.
+                  unsigned h = RUNS ? ((unsigned) j & (SF_SLOTS - 1))
+                                    : (((unsigned) j * 2654435761u) >> (32 - ilog2(SF_SLOTS)));
                   bool done = false;
                   for (int probe = 0; probe < 32; probe++) {
                      const int old = atomicCAS(&hkey[h], -1, j);
@@ -449,7 +456,7 @@ __global__ __launch_bounds__(256) void k_strength_fill(int n, const int *__restr
 }
 
 void create_s(const Csr &A, double theta, double max_row_sum, Csr &S, int *st_count, double *l1,
-              int *l1_zero, double *thr_out, int *flg_out)
+              int *l1_zero, double *thr_out, int *flg_out, bool column_runs)
 {
    int n = A.nrows;
    TimedLaunch tl("setup_strength", 0.0);
@@ -481,8 +488,13 @@ void create_s(const Csr &A, double theta, double max_row_sum, Csr &S, int *st_co
    if (n > 0) {
       const int nbf = (int) (((long long) n + SF_ROWS - 1) / SF_ROWS);
       dispatch_k(K, [&](auto kc) {
-         k_strength_fill<decltype(kc)::value><<<nbf, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, thr, flg,
-                                                                         S.ia, S.ja, st_count);
+         if (column_runs) {
+            k_strength_fill<decltype(kc)::value, true><<<nbf, 256, 0, stream()>>>(
+               n, A.ia, A.ja, A.a, thr, flg, S.ia, S.ja, st_count);
+         } else {
+            k_strength_fill<decltype(kc)::value, false><<<nbf, 256, 0, stream()>>>(
+               n, A.ia, A.ja, A.a, thr, flg, S.ia, S.ja, st_count);
+         }
       });
       HX_LAUNCH_CHECK();
    }
@@ -3269,9 +3281,12 @@ constexpr int TF_SLOTS = 4096;
 constexpr int TF_E = 8;
 constexpr int TC_U = 8;
 
+// the column's low bits: the columns of a chunk are runs of consecutive
+// coarse indices, so they take consecutive slots and the flushes add to
+// consecutive counters (one memory transaction for many lanes)
 __device__ __forceinline__ unsigned tf_hash(int c)
 {
-   return ((unsigned) c * 2654435761u) >> (32 - ilog2(TF_SLOTS));
+   return (unsigned) c & (TF_SLOTS - 1);
 }
 
 __global__ __launch_bounds__(256) void k_col_count_h(long long nnz, const int *__restrict__ ja,
