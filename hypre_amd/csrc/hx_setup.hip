@@ -1195,28 +1195,37 @@ struct RowHash
    {
       return ((unsigned) key * 2654435761u) >> (32 - bits);
    }
+   // the probe loops have one exit (a lane leaves when its slot is settled),
+   // which compiles to far fewer scalar mask operations per probe than
+   // returning from inside the loop
    __device__ __forceinline__ int insert(int key, bool &isnew)
    {
       unsigned h = h0(key);
+      int slot = -1, old = 0;
       for (int probe = 0; probe < cap; probe++) {
-         int old = atomicCAS(&keys[h], -1, key);
-         if (old == -1) { isnew = true; return (int) h; }
-         if (old == key) { isnew = false; return (int) h; }
+         old = atomicCAS(&keys[h], -1, key);
+         if (old == -1 || old == key) {
+            slot = (int) h;
+            break;
+         }
          h = (h + 1) & (cap - 1);
       }
-      isnew = false;
-      return -1;
+      isnew = slot >= 0 && old == -1;
+      return slot;
    }
    __device__ __forceinline__ int find(int key) const
    {
       unsigned h = h0(key);
+      int slot = -1;
       for (int probe = 0; probe < cap; probe++) {
-         int k = keys[h];
-         if (k == key) { return (int) h; }
-         if (k == -1) { return -1; }
+         const int k = keys[h];
+         if (k == key || k == -1) {
+            slot = (k == key) ? (int) h : -1;
+            break;
+         }
          h = (h + 1) & (cap - 1);
       }
-      return -1;
+      return slot;
    }
 };
 
@@ -3985,23 +3994,18 @@ __device__ __forceinline__ int rs_insert(int *keys, int cap, int bits, int key, 
                                          bool &fresh)
 {
    unsigned h = ((unsigned) key * 2654435761u) >> (32 - bits);
+   int slot = -1, old = 0;
    for (int probe = 0; probe < cap; probe++) {
-      const int old = atomicCAS(&keys[h], -1, key | RS_NEW);
-      if (old == -1) {
-         isnew = true;
-         fresh = true;
-         return (int) h;
-      }
-      if ((old & 0x7fffffff) == key) {
-         isnew = false;
-         fresh = (old & RS_NEW) != 0;
-         return (int) h;
+      old = atomicCAS(&keys[h], -1, key | RS_NEW);
+      if (old == -1 || (old & 0x7fffffff) == key) {
+         slot = (int) h;
+         break;
       }
       h = (h + 1) & (cap - 1);
    }
-   isnew = false;
-   fresh = false;
-   return -1;
+   isnew = slot >= 0 && old == -1;
+   fresh = slot >= 0 && (old == -1 || (old & RS_NEW) != 0);
+   return slot;
 }
 
 static size_t rap_seg_lds(int rc, int rcl, int ac, int acl)
