@@ -574,91 +574,76 @@ __global__ void k_c_lists(int n, const int *__restrict__ ia, const int *__restri
    }
 }
 
-// The lists of k_c_lists from one windowed pass over A (one row per lane, 64
-// rows per wave, values, columns and the columns' C/F flags staged in LDS per
-// window of CW_SW entries, as in k_row_stats); strong entries are those the
-// strength threshold of the row (kept from k_row_stats) marks, in A order,
-// i.e. S order.
-constexpr int CW_SW = 512;
-constexpr int CW_WPB = 4;
-
-__global__ __launch_bounds__(CW_WPB * WAVE) void k_c_lists_w(int n, const int *__restrict__ ia,
-                                                               const int *__restrict__ ja,
-                                                               const double *__restrict__ a,
-                                                               const double *__restrict__ thr,
-                                                               const int *__restrict__ flg,
-                                                               const int *__restrict__ S_ia,
-                                                               const int *__restrict__ CF,
-                                                               int *__restrict__ SC_cnt,
-                                                               int *__restrict__ SC_ja,
-                                                               int *__restrict__ OC_cnt,
-                                                               OCEntry *__restrict__ OC)
+// The lists of k_c_lists from A and the strength threshold of each row (kept
+// from k_row_stats), K lanes per row and SF_ROWS consecutive rows per block:
+// SF_U strided steps of the row per batch, every load of a batch (values,
+// columns, then the columns' C/F flags) issued before any is used; strong
+// entries are those the threshold marks, in A order, i.e. S order, and both
+// lists are compacted in storage order by ballots over the row's lanes.
+template <int K>
+__global__ __launch_bounds__(256) void k_c_lists_t(int n, const int *__restrict__ ia,
+                                                   const int *__restrict__ ja,
+                                                   const double *__restrict__ a,
+                                                   const double *__restrict__ thr,
+                                                   const int *__restrict__ flg,
+                                                   const int *__restrict__ S_ia,
+                                                   const int *__restrict__ CF,
+                                                   int *__restrict__ SC_cnt,
+                                                   int *__restrict__ SC_ja,
+                                                   int *__restrict__ OC_cnt,
+                                                   OCEntry *__restrict__ OC)
 {
-   constexpr int U = CW_SW / WAVE;
-   __shared__ double sa_all[CW_WPB * CW_SW];
-   __shared__ int sj_all[CW_WPB * CW_SW];
-   __shared__ int sc_all[CW_WPB * CW_SW];
-   const int nb = gridDim.x;
-   const int blk = xcd_blk(blockIdx.x, nb);
-   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-   double *sa = sa_all + wv * CW_SW;
-   int *sj = sj_all + wv * CW_SW;
-   int *sc = sc_all + wv * CW_SW;
-   const int ng = (n + WAVE - 1) / WAVE;
-   const int gpb = (ng + nb - 1) / nb;
-   const int g0 = blk * gpb, g1 = min(ng, g0 + gpb);
-   for (int g = g0 + wv; g < g1; g += CW_WPB) {
-      const int r0 = g * WAVE;
-      const int nr = min(WAVE, n - r0);
-      int rs = 0, re = 0, so = 0, fl = 2;
-      double t = 0.0;
-      if (lane < nr) {
-         rs = ia[r0 + lane];
-         re = ia[r0 + lane + 1];
-         so = S_ia[r0 + lane];
-         fl = flg[r0 + lane];
-         t = thr[r0 + lane];
-      }
+   const int gl = threadIdx.x & (K - 1);
+   const int rows_per_iter = 256 / K;
+   const int gshift = (threadIdx.x & 63) & ~(K - 1);
+   const unsigned long long gmask = (K == 64) ? ~0ULL : ((1ULL << K) - 1);
+   const unsigned long long below = (1ULL << gl) - 1;
+   const long long r_begin = (long long) blockIdx.x * SF_ROWS;
+   const long long r_end = min((long long) n, r_begin + SF_ROWS);
+   for (long long rb = r_begin; rb < r_end; rb += rows_per_iter) {
+      const long long ir = rb + threadIdx.x / K;
+      const bool act = ir < r_end;
+      const int i = act ? (int) ir : 0;
+      const int s = act ? ia[i] : 0, e = act ? ia[i + 1] : 0;
+      const double t = act ? thr[i] : 0.0;
+      const int fl = act ? flg[i] : 2;
       const bool neg = fl & 1, allweak = fl & 2;
-      const int base = __shfl(rs, 0, WAVE);
-      const int end = __shfl(re, nr - 1, WAVE);
+      const int so = act ? S_ia[i] : 0;
+      // positions past the row end read its last entry and are masked (a
+      // batch is only entered when some row of the wave has an entry left)
+      const int klast = max(e - 1, 0);
       int coc = 0, csc = 0;
-      for (int wb = base; wb < end; wb += CW_SW) {
-         int jn[U];
-         double an[U];
+      for (int kb = s + 1; __any(kb < e); kb += SF_U * K) {
+         double vv[SF_U];
+         int jj[SF_U], cc[SF_U];
 #pragma unroll
-         for (int u = 0; u < U; u++) {
-            const int k = min(wb + u * WAVE + lane, end - 1);
-            jn[u] = __builtin_nontemporal_load(ja + k);
-            an[u] = __builtin_nontemporal_load(a + k);
+         for (int u = 0; u < SF_U; u++) {
+            const int kq = min(kb + gl + u * K, klast);
+            vv[u] = __builtin_nontemporal_load(a + kq);
+            jj[u] = __builtin_nontemporal_load(ja + kq);
          }
-         int cv[U];
 #pragma unroll
-         for (int u = 0; u < U; u++) { cv[u] = CF[jn[u]]; }
+         for (int u = 0; u < SF_U; u++) { cc[u] = CF[jj[u]]; }
 #pragma unroll
-         for (int u = 0; u < U; u++) {
-            sa[u * WAVE + lane] = an[u];
-            sj[u * WAVE + lane] = jn[u];
-            sc[u * WAVE + lane] = cv[u];
-         }
-         __builtin_amdgcn_wave_barrier();
-         if (lane < nr) {
-            const int we = min(end, wb + CW_SW);
-            const int s = max(rs + 1, wb), e = min(re, we);
-            for (int k = s; k < e; k++) {
-               const double v = sa[k - wb];
-               if (sc[k - wb] >= 0) {
-                  const int j = sj[k - wb];
-                  if (neg ? (v > 0) : (v < 0)) { OC[rs + coc++] = OCEntry{k, j, v}; }
-                  if (!allweak && (neg ? !(v <= t) : !(v >= t))) { SC_ja[so + csc++] = j; }
-               }
+         for (int u = 0; u < SF_U; u++) {
+            const int k = kb + gl + u * K;
+            bool toc = false, tsc = false;
+            if (k < e && cc[u] >= 0) {
+               const double v = vv[u];
+               toc = neg ? (v > 0) : (v < 0);
+               tsc = !allweak && (neg ? !(v <= t) : !(v >= t));
             }
+            const unsigned long long mo = (__ballot(toc) >> gshift) & gmask;
+            const unsigned long long ms = (__ballot(tsc) >> gshift) & gmask;
+            if (toc) { OC[s + coc + __popcll(mo & below)] = OCEntry{k, jj[u], vv[u]}; }
+            if (tsc) { SC_ja[so + csc + __popcll(ms & below)] = jj[u]; }
+            coc += __popcll(mo);
+            csc += __popcll(ms);
          }
-         __builtin_amdgcn_wave_barrier();
       }
-      if (lane < nr) {
-         SC_cnt[r0 + lane] = csc;
-         OC_cnt[r0 + lane] = coc;
+      if (act && gl == 0) {
+         SC_cnt[i] = csc;
+         OC_cnt[i] = coc;
       }
    }
 }
@@ -685,16 +670,16 @@ static void c_lists(const Csr &A, const Csr &S, const int *CF, CLists &L, const 
    L.sc_ja = dalloc_n<int>((size_t) std::max<long long>(S.nnz, 1));
    L.oc = dalloc_n<OCEntry>((size_t) std::max<long long>(A.nnz, 1));
    if (n == 0) { return; }
+   const int K = lanes_per_row(A.nnz, n);
    if (thr && flg) {
-      const long long ng = ((long long) n + WAVE - 1) / WAVE;
-      long long nbs = std::min<long long>((ng + CW_WPB - 1) / CW_WPB, 8192);
-      if (nbs >= 64) { nbs = (nbs + 7) / 8 * 8; }
-      k_c_lists_w<<<(int) std::max<long long>(nbs, 1), CW_WPB * WAVE, 0, stream()>>>(
-          n, A.ia, A.ja, A.a, thr, flg, S.ia, CF, L.sc_cnt, L.sc_ja, L.oc_cnt, L.oc);
+      const int nbf = (int) (((long long) n + SF_ROWS - 1) / SF_ROWS);
+      dispatch_k(K, [&](auto kc) {
+         k_c_lists_t<decltype(kc)::value><<<nbf, 256, 0, stream()>>>(
+             n, A.ia, A.ja, A.a, thr, flg, S.ia, CF, L.sc_cnt, L.sc_ja, L.oc_cnt, L.oc);
+      });
       HX_LAUNCH_CHECK();
       return;
    }
-   const int K = lanes_per_row(A.nnz, n);
    const int nb = (int) std::max<long long>(1, std::min<long long>(((long long) n * K + 255) / 256, 16384));
    dispatch_k(K, [&](auto kc) {
       k_c_lists<decltype(kc)::value><<<nb, 256, 0, stream()>>>(n, A.ia, A.ja, A.a, S.ia, S.ja, CF,
@@ -2884,49 +2869,64 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
    }
 }
 
-// hypre_qsort2_abs (src/utilities/qsort.c:452) on a row stored with stride SS,
-// refined only as far as the first `keep` positions need; explicit stack.
+// hypre_qsort2_abs (src/utilities/qsort.c:452) on a row stored with stride SS:
+// w holds the values, v the positions of the entries in the staged row (one
+// byte each), refined only as far as the first `keep` positions need.  The
+// stack of (left, right) pairs lives in two registers, 8 bits a level.
 template <int SS>
-__device__ void qsort2_abs_strided(int *v, double *w, int cnt, int keep, int *stl, int *str,
-                                   int stack_cap)
+__device__ void qsort2_abs_strided(unsigned char *v, double *w, int cnt, int keep)
 {
-   int sp = 0;
-   stl[0] = 0; str[0] = cnt - 1; sp = 1;
+   unsigned long long stl = 0ull, str = (unsigned long long) (cnt - 1);
+   int sp = 1;
    while (sp > 0) {
       sp--;
-      const int left = stl[sp * SS], right = str[sp * SS];
+      const int left = (int) ((stl >> (8 * sp)) & 0xffull);
+      const int right = (int) ((str >> (8 * sp)) & 0xffull);
       if (left >= right || left >= keep) { continue; }
       const int mid = (left + right) / 2;
-      { int t = v[left * SS]; v[left * SS] = v[mid * SS]; v[mid * SS] = t;
+      { unsigned char t = v[left * SS]; v[left * SS] = v[mid * SS]; v[mid * SS] = t;
         double tw = w[left * SS]; w[left * SS] = w[mid * SS]; w[mid * SS] = tw; }
       int last = left;
       const double piv = fabs(w[left * SS]);
       for (int i = left + 1; i <= right; i++) {
          if (fabs(w[i * SS]) > piv) {
             ++last;
-            int t = v[last * SS]; v[last * SS] = v[i * SS]; v[i * SS] = t;
+            unsigned char t = v[last * SS]; v[last * SS] = v[i * SS]; v[i * SS] = t;
             double tw = w[last * SS]; w[last * SS] = w[i * SS]; w[i * SS] = tw;
          }
       }
-      { int t = v[left * SS]; v[left * SS] = v[last * SS]; v[last * SS] = t;
+      { unsigned char t = v[left * SS]; v[left * SS] = v[last * SS]; v[last * SS] = t;
         double tw = w[left * SS]; w[left * SS] = w[last * SS]; w[last * SS] = tw; }
       const int l1 = left, r1 = last - 1, l2 = last + 1, r2 = right;
-      // push the larger side first so the smaller one is refined next; a side
-      // that cannot reach the first `keep` positions is dropped here
+      // push the larger side first so the smaller one is refined next (the
+      // stack then never holds more than log2(cnt) + 1 pairs); a side that
+      // cannot reach the first `keep` positions is dropped here
       const bool big1 = (r1 - l1) > (r2 - l2);
       const int la = big1 ? l1 : l2, ra = big1 ? r1 : r2;
       const int lb = big1 ? l2 : l1, rb = big1 ? r2 : r1;
-      if (la < ra && la < keep && sp < stack_cap) { stl[sp * SS] = la; str[sp * SS] = ra; sp++; }
-      if (lb < rb && lb < keep && sp < stack_cap) { stl[sp * SS] = lb; str[sp * SS] = rb; sp++; }
+      if (la < ra && la < keep) {
+         const unsigned long long m = ~(0xffull << (8 * sp));
+         stl = (stl & m) | ((unsigned long long) la << (8 * sp));
+         str = (str & m) | ((unsigned long long) ra << (8 * sp));
+         sp++;
+      }
+      if (lb < rb && lb < keep) {
+         const unsigned long long m = ~(0xffull << (8 * sp));
+         stl = (stl & m) | ((unsigned long long) lb << (8 * sp));
+         str = (str & m) | ((unsigned long long) rb << (8 * sp));
+         sp++;
+      }
    }
 }
 
 // par_csr_matrix.c:3040-3320 (trunc_factor drop + rescale, then P_max_elmts
-// with qsort2_abs and rescale), one thread per row, rows staged in LDS with a
-// stride of 65 so that a wave's accesses fall into distinct banks.
+// with qsort2_abs and rescale), one thread per row.  The values of 64 rows are
+// staged in LDS with a stride of 65 so that a wave's accesses fall into
+// distinct banks; the columns stay in the raw rows and the sort moves one-byte
+// positions instead, which keeps a block at MC x 585 + 256 bytes of LDS (8
+// blocks of MC = 32 rows per CU).
 constexpr int TR_ROWS = 64;
 constexpr int TR_SS = 65;
-constexpr int TR_STACK = 8;    // >= log2(64) + 1: the smaller side is always refined first
 constexpr int TR_U = 8;        // staging loads in flight per thread
 
 template <int MC>
@@ -2939,9 +2939,9 @@ __global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__rest
                                                         int *__restrict__ out_j,
                                                         double *__restrict__ out_a)
 {
+   static_assert(MC <= 128, "one-byte positions and an 8-level stack");
    __shared__ double sw[MC * TR_SS];
-   __shared__ int sj[MC * TR_SS];
-   __shared__ int stl[TR_STACK * TR_SS], str[TR_STACK * TR_SS];
+   __shared__ unsigned char sv[MC * TR_SS];
    __shared__ int sc[TR_ROWS];
    const int t = threadIdx.x;
    for (long long base = (long long) blockIdx.x * TR_ROWS; base < n;
@@ -2950,8 +2950,8 @@ __global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__rest
       sc[t] = (i < n) ? cnt_raw[i] : -1;
       __syncthreads();
       // TR_U loads per thread issued before their LDS stores
+#pragma unroll 1
       for (int k0 = t; k0 < TR_ROWS * MC; k0 += TR_ROWS * TR_U) {
-         int jv[TR_U];
          double wv[TR_U];
          bool ok[TR_U];
 #pragma unroll
@@ -2960,25 +2960,22 @@ __global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__rest
             const bool in = k < TR_ROWS * MC;
             const int r = in ? k / MC : 0, e = in ? k % MC : 0;
             ok[u] = in && e < sc[r];
-            if (ok[u]) {
-               jv[u] = raw_j[(size_t) (base + r) * MC + e];
-               wv[u] = raw_a[(size_t) (base + r) * MC + e];
-            }
+            if (ok[u]) { wv[u] = raw_a[(size_t) (base + r) * MC + e]; }
          }
 #pragma unroll
          for (int u = 0; u < TR_U; u++) {
             if (ok[u]) {
                const int k = k0 + u * TR_ROWS;
                const int r = k / MC, e = k % MC;
-               sj[e * TR_SS + r] = jv[u];
                sw[e * TR_SS + r] = wv[u];
+               sv[e * TR_SS + r] = (unsigned char) e;
             }
          }
       }
       __syncthreads();
       const int c = sc[t];
       if (c >= 0) {
-         int *vj = sj + t;
+         unsigned char *vj = sv + t;
          double *vw = sw + t;
          int kept = c;
          if (trunc_factor > 0) {
@@ -3010,7 +3007,7 @@ __global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__rest
          if (max_elmts > 0 && kept > max_elmts) {
             double row_sum = 0.0;
             for (int k = 0; k < kept; k++) { row_sum += vw[k * TR_SS]; }
-            qsort2_abs_strided<TR_SS>(vj, vw, kept, max_elmts, stl + t, str + t, TR_STACK);
+            qsort2_abs_strided<TR_SS>(vj, vw, kept, max_elmts);
             double scale = 0.0;
             for (int k = 0; k < max_elmts; k++) { scale += vw[k * TR_SS]; }
             if (scale != 0.) {
@@ -3023,8 +3020,9 @@ __global__ __launch_bounds__(TR_ROWS) void k_trunc_rows(int n, const int *__rest
          }
          const long long row = rowlist ? rowlist[i] : i;
          out_cnt[row] = kept;
+         const int *rj = raw_j + (size_t) i * MC;
          for (int k = 0; k < kept; k++) {
-            out_j[(size_t) row * out_stride + k] = vj[k * TR_SS];
+            out_j[(size_t) row * out_stride + k] = rj[vj[k * TR_SS]];
             out_a[(size_t) row * out_stride + k] = vw[k * TR_SS];
          }
       }
