@@ -125,6 +125,7 @@ void AMGSolver::setup(const Csr &A0)
    coarse_relax = p.relax_type[3];
    coarse_sweeps = p.num_sweeps[3];
    int level = 0;
+   int a_max_row = 0;   // longest row of the current level's operator, once a product gave it
    while (true) {
       Range lrange("AMG Level-%d", level);   // par_amg_setup.c:976, 3103
       Level &lv = L[level];
@@ -220,7 +221,8 @@ void AMGSolver::setup(const Csr &A0)
       const int pmx = aggressive ? p.agg_P_max_elmts : p.P_max_elmts;
       {
          Range r("TripleMat-RAP");   // par_csr_triplemat.c:952
-         rap(cur.R, *cur.A, cur.P, nx.Aown, pmx > 0 ? std::max(pmx, 1) : 0);
+         rap(cur.R, *cur.A, cur.P, nx.Aown, pmx > 0 ? std::max(pmx, 1) : 0, a_max_row,
+             &a_max_row);
       }
       nx.A = &nx.Aown;
       level++;

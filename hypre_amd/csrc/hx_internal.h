@@ -88,8 +88,11 @@ void d2h(void *h, const void *d, size_t bytes);
 void d2d(void *dst, const void *src, size_t bytes);
 void dzero(void *d, size_t bytes);
 
-// exclusive scan of n ints -> out[0..n], out[n] = total (in may alias out)
-long long exclusive_scan(const int *in, int *out, int n);
+// exclusive scan of n ints -> out[0..n], out[n] = total (in may alias out);
+// waits for the total, and with max_in also returns the largest input
+long long exclusive_scan(const int *in, int *out, int n, int *max_in = nullptr);
+// the same scan without waiting (the caller knows or does not need the total)
+void exclusive_scan_nowait(const int *in, int *out, int n);
 // reductions returning to host
 long long reduce_sum_int(const int *in, int n);
 int       reduce_max_int(const int *in, int n);
@@ -288,9 +291,11 @@ void correct_cf_marker(int n, int *CF, const int *f2c1, const int *CFN);
 void mod_multipass_interp(const Csr &A, const Csr &S, int *CF, int nc, double trunc_factor,
                           int max_elmts, Csr &P);
 void transpose(const Csr &P, Csr &R);
-// p_row_bound: an upper bound of P's row lengths when the caller knows one
-// (truncation to P_max_elmts), else <= 0
-void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac, int p_row_bound = 0);
+// p_row_bound / a_row_bound: an upper bound of P's / A's row lengths when the
+// caller knows one (truncation to P_max_elmts; the previous product's
+// ac_max_row), else <= 0; ac_max_row, when given, receives Ac's longest row
+void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac, int p_row_bound = 0,
+         int a_row_bound = 0, int *ac_max_row = nullptr);
 // dense column-major copy of a small CSR matrix (n <= 64)
 void csr_to_dense_colmajor(const Csr &A, double *M_dev);
 

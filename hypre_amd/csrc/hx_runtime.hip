@@ -278,8 +278,9 @@ void dzero(void *d, size_t bytes)
 }
 
 // ------------------------------------------------------------------ scan
-// Three passes: per-block totals (2048 items / block), a single-block scan of
-// the totals (64-bit, overflow-checked), then the per-block exclusive scan.
+// Three passes: per-block totals and maxima (2048 items / block), a
+// single-block scan of the totals (64-bit, overflow-checked) and reduction of
+// the maxima, then the per-block exclusive scan.
 constexpr int SCAN_BS = 256;
 constexpr int SCAN_IT = 8;
 constexpr int SCAN_TILE = SCAN_BS * SCAN_IT;
@@ -303,33 +304,55 @@ __device__ __forceinline__ long long block_excl_scan_ll(long long v, long long *
 }
 
 __global__ __launch_bounds__(SCAN_BS) void k_scan_tiles(const int *__restrict__ in, int n,
-                                                        long long *__restrict__ tile_sum)
+                                                        long long *__restrict__ tile_sum,
+                                                        int *__restrict__ tile_max)
 {
    __shared__ long long sh[SCAN_BS];
+   __shared__ int wmax[SCAN_BS / 64];
    long long base = (long long) blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_IT;
    long long s = 0;
+   int m = INT_MIN;
 #pragma unroll
    for (int k = 0; k < SCAN_IT; k++) {
       long long i = base + k;
-      if (i < n) { s += in[i]; }
+      if (i < n) {
+         const int v = in[i];
+         s += v;
+         m = max(m, v);
+      }
    }
+   for (int o = 32; o > 0; o >>= 1) { m = max(m, __shfl_xor(m, o, 64)); }
+   if ((threadIdx.x & 63) == 0) { wmax[threadIdx.x >> 6] = m; }
    long long tot;
-   block_excl_scan_ll(s, sh, &tot);
-   if (threadIdx.x == 0) { tile_sum[blockIdx.x] = tot; }
+   block_excl_scan_ll(s, sh, &tot);   // its barriers order the wmax stores
+   if (threadIdx.x == 0) {
+      tile_sum[blockIdx.x] = tot;
+      int bm = wmax[0];
+      for (int w = 1; w < SCAN_BS / 64; w++) { bm = max(bm, wmax[w]); }
+      tile_max[blockIdx.x] = bm;
+   }
 }
 
-__global__ __launch_bounds__(SCAN_BS) void k_scan_sums(long long *__restrict__ tile_sum, int ntiles,
+__global__ __launch_bounds__(SCAN_BS) void k_scan_sums(long long *__restrict__ tile_sum,
+                                                       const int *__restrict__ tile_max, int ntiles,
                                                        int *__restrict__ out_total_slot,
                                                        int *__restrict__ overflow)
 {
    __shared__ long long sh[SCAN_BS];
+   __shared__ int wmax[SCAN_BS / 64];
    int per = (ntiles + SCAN_BS - 1) / SCAN_BS;
    int b0 = threadIdx.x * per;
    long long s = 0;
+   int m = INT_MIN;
    for (int k = 0; k < per; k++) {
       int i = b0 + k;
-      if (i < ntiles) { s += tile_sum[i]; }
+      if (i < ntiles) {
+         s += tile_sum[i];
+         m = max(m, tile_max[i]);
+      }
    }
+   for (int o = 32; o > 0; o >>= 1) { m = max(m, __shfl_xor(m, o, 64)); }
+   if ((threadIdx.x & 63) == 0) { wmax[threadIdx.x >> 6] = m; }
    long long tot;
    long long ex = block_excl_scan_ll(s, sh, &tot);
    long long run = ex;
@@ -344,6 +367,9 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_sums(long long *__restrict__ t
    if (threadIdx.x == 0) {
       *out_total_slot = (tot > 0x7fffffffLL) ? 0x7fffffff : (int) tot;
       overflow[0] = (tot > 0x7fffffffLL) ? 1 : 0;
+      int bm = wmax[0];
+      for (int w = 1; w < SCAN_BS / 64; w++) { bm = max(bm, wmax[w]); }
+      overflow[1] = bm;
       reinterpret_cast<long long *>(overflow + 2)[0] = tot;
    }
 }
@@ -372,30 +398,51 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_apply(const int *in, int n, in
    }
 }
 
-long long exclusive_scan(const int *in, int *out, int n)
+// the three passes on the library stream; the total's 32-bit overflow flag,
+// the largest input and the 64-bit total land at flags + 8, 9 and 10..11
+static void scan_launch(const int *in, int *out, int n)
+{
+   int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+   long long *tiles = dalloc_n<long long>(ntiles);
+   int *tmax = dalloc_n<int>(ntiles);
+   int *flags = device_flags();
+   k_scan_tiles<<<ntiles, SCAN_BS, 0, stream()>>>(in, n, tiles, tmax);
+   HX_LAUNCH_CHECK();
+   k_scan_sums<<<1, SCAN_BS, 0, stream()>>>(tiles, tmax, ntiles, out + n, flags + 8);
+   HX_LAUNCH_CHECK();
+   k_scan_apply<<<ntiles, SCAN_BS, 0, stream()>>>(in, n, out, tiles);
+   HX_LAUNCH_CHECK();
+   // stream-ordered reuse: later work on this stream runs after the passes
+   dfree(tiles);
+   dfree(tmax);
+}
+
+long long exclusive_scan(const int *in, int *out, int n, int *max_in)
 {
    if (n <= 0) {
       int z = 0;
       h2d(out, &z, sizeof(int));
+      if (max_in) { *max_in = 0; }
       return 0;
    }
-   int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-   long long *tiles = dalloc_n<long long>(ntiles);
-   int *flags = device_flags();
-   k_scan_tiles<<<ntiles, SCAN_BS, 0, stream()>>>(in, n, tiles);
-   HX_LAUNCH_CHECK();
-   k_scan_sums<<<1, SCAN_BS, 0, stream()>>>(tiles, ntiles, out + n, flags + 8);
-   HX_LAUNCH_CHECK();
-   k_scan_apply<<<ntiles, SCAN_BS, 0, stream()>>>(in, n, out, tiles);
-   HX_LAUNCH_CHECK();
+   scan_launch(in, out, n);
    int *h = static_cast<int *>(pinned_scratch(64));
-   HX_HIP(hipMemcpyAsync(h, flags + 8, 4 * sizeof(int), hipMemcpyDeviceToHost, stream()));
+   HX_HIP(hipMemcpyAsync(h, device_flags() + 8, 4 * sizeof(int), hipMemcpyDeviceToHost, stream()));
    sync();
-   dfree(tiles);
    long long tot;
    std::memcpy(&tot, h + 2, sizeof(long long));
    if (h[0]) { fail(HYPRE_ERROR_MEMORY, "prefix sum exceeds 32-bit index range"); }
+   if (max_in) { *max_in = h[1]; }
    return tot;
+}
+
+void exclusive_scan_nowait(const int *in, int *out, int n)
+{
+   if (n <= 0) {
+      dzero(out, sizeof(int));
+      return;
+   }
+   scan_launch(in, out, n);
 }
 
 __global__ void k_reduce_int(const int *__restrict__ in, int n, unsigned long long *sum, int *mx)

@@ -3494,7 +3494,7 @@ void transpose(const Csr &P, Csr &R)
    const int nbe = (int) std::max<long long>(1, std::min<long long>(nch, 8192));
    if (P.nnz > 0) { k_col_count_h<<<nbe, 256, 0, stream()>>>(P.nnz, P.ja, R.ia); }
    HX_LAUNCH_CHECK();
-   exclusive_scan(R.ia, R.ia, nc);
+   exclusive_scan_nowait(R.ia, R.ia, nc);   // the total is P.nnz
    int *fillp = dalloc_n<int>((size_t) nc + 1);
    d2d(fillp, R.ia, sizeof(int) * ((size_t) nc + 1));
    int nb = std::max(1, std::min((n + 255) / 256, 8192));
@@ -4337,7 +4337,8 @@ __global__ void k_rap_compact(int nc, const int *__restrict__ cnt, const int *__
    }
 }
 
-void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac, int p_row_bound)
+void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac, int p_row_bound, int a_row_bound,
+         int *ac_max_row)
 {
    int nc = R.nrows;
    TimedLaunch tl("setup_rap", 0.0);
@@ -4385,7 +4386,7 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac, int p_row_bound)
       if (pmax <= 16) {
          pw = 0;
          while ((1 << pw) < pmax) { pw++; }
-         const int amax = csr_max_row(A);
+         const int amax = a_row_bound > 0 ? a_row_bound : csr_max_row(A);
          if (amax <= 32) {
             aw = 2;
             while ((1 << aw) < amax) { aw++; }
@@ -4446,7 +4447,7 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac, int p_row_bound)
       a.cnt = cnt;
       rap_pass<MODE_SYM>(a, list, nwork);
    }
-   Ac.nnz = exclusive_scan(cnt, Ac.ia, nc);
+   Ac.nnz = exclusive_scan(cnt, Ac.ia, nc, ac_max_row);
    Ac.ja = dalloc_n<int>((size_t) std::max<long long>(Ac.nnz, 1));
    Ac.a = dalloc_n<double>((size_t) std::max<long long>(Ac.nnz, 1));
    int nbc = (int) std::max<long long>(1, std::min<long long>(((long long) nc * 16 + 255) / 256, 16384));
