@@ -147,3 +147,54 @@ def test_vcycle_bits_equal_across_formats_all_column_widths(hx, oracle):
         assert np.isfinite(u0).all() and np.abs(u0).max() > 0
         bad = np.nonzero(u0.view(np.int64) != u1.view(np.int64))[0]
         assert bad.size == 0, f"{bad.size} entries differ (fp32={fp32}), first at {bad[:5]}"
+
+
+def banded_zones_matrix(n=20_000):
+    """Symmetric, diagonally dominant: row i couples to i + d for d = 1 in
+    [0, 6400), d = 1..40 in [6400, 12800) and d = 1..90 beyond, so that its
+    64-row slices are at most 32 wide, between 33 and 128, and wider than
+    128."""
+    import scipy.sparse as sp
+    rows, cols = [], []
+    for lo, hi, dmax in ((0, 6400, 1), (6400, 12800, 40), (12800, n, 90)):
+        for d in range(1, dmax + 1):
+            i = np.arange(lo, min(hi, n - d))
+            rows += [i, i + d]
+            cols += [i + d, i]
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    M = sp.csr_matrix((-np.ones(len(r)), (r, c)), shape=(n, n))
+    M.sum_duplicates()
+    d = -np.asarray(M.sum(axis=1)).ravel() + 1.0
+    M = (M + sp.diags(d)).tocsr()
+    M.sort_indices()
+    # diagonal first, then ascending columns
+    ia = M.indptr.astype(np.int32)
+    ja = M.indices.astype(np.int32).copy()
+    a = M.data.copy()
+    for k in range(n):
+        s, e = ia[k], ia[k + 1]
+        cs = ja[s:e]
+        p = int(np.nonzero(cs == k)[0][0])
+        order = np.r_[p, np.arange(0, p), np.arange(p + 1, e - s)]
+        ja[s:e] = cs[order]
+        a[s:e] = a[s:e][order]
+    return ia, ja, a
+
+
+def test_vcycle_bits_equal_across_formats_all_slice_widths(hx, oracle):
+    """Slices up to 32 wide, 33 to 128 and wider than 128 at the fine level:
+    the V-cycle is bit-equal to the CSR kernels'."""
+    say("slice widths")
+    ia, ja, a = banded_zones_matrix()
+    lens = np.diff(ia)
+    w = np.array([lens[s:s + 64].max() for s in range(0, len(lens), 64)])
+    assert (w <= 32).any() and ((w > 32) & (w <= 128)).any() and (w > 128).any()
+    A = hx.ParCSRMatrix.from_csr(ia, ja, a)
+    f = 2 * oracle.rand_stream(13, len(ia) - 1) - 1
+    for fp32 in (False, True):
+        u0 = cycle(hx, A, 0, f, fp32)
+        u1 = cycle(hx, A, 1, f, fp32)
+        assert np.isfinite(u0).all() and np.abs(u0).max() > 0
+        bad = np.nonzero(u0.view(np.int64) != u1.view(np.int64))[0]
+        assert bad.size == 0, f"{bad.size} entries differ (fp32={fp32}), first at {bad[:5]}"
