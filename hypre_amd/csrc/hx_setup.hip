@@ -2555,7 +2555,17 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
       const int incl = group_incl_scan<G>(klen, gl);
       const int L = __shfl(incl, G - 1, G);
       const int off = incl - klen;
-      const int nseg = max(ns, 1);
+      // candidate q's source, tabulated by its segment's lane in the group's
+      // term space (free until the terms are staged): ~column for a strong C
+      // neighbour, else the position of the candidate in SC_ja
+      int *ctab = reinterpret_cast<int *>(st_a);
+      if (live && L > 2 * OLC) {
+         defer = true;
+         live = false;
+      }
+      if (live) {
+         for (int t = 0; t < klen; t++) { ctab[off + t] = (kind == 1) ? ~kb : kb + t; }
+      }
       // the diagonal of each neighbour row and the mirror guess of column i
       double dg1 = 0.0;
       int gk = -1, gv = -1;
@@ -2574,21 +2584,11 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
          int key[XR_U];
 #pragma unroll
          for (int u = 0; u < XR_U; u++) {
-            const int c0 = cb + u * G;   // uniform within the group
-            const int q = c0 + gl;
+            const int q = cb + u * G + gl;
             key[u] = -1;
-            if (live && c0 < L) {
-               const int sa = group_seg_find<G>(off, nseg, q);
-               const int kind_s = __shfl(kind, sa, G);
-               const int b_s = __shfl(kb, sa, G);
-               const int o_s = __shfl(off, sa, G);
-               if (q < L) {
-                  if (kind_s == 1) {
-                     key[u] = b_s;
-                  } else if (kind_s == 2) {
-                     key[u] = p.SC_ja[b_s + (q - o_s)];
-                  }
-               }
+            if (live && q < L) {
+               const int e = ctab[q];
+               key[u] = (e < 0) ? ~e : p.SC_ja[e];
             }
          }
 #pragma unroll
@@ -2685,11 +2685,22 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
       const int oincl = group_incl_scan<G>(olen, gl);
       const int OL = __shfl(oincl, G - 1, G);
       const int ooff = oincl - olen;
-      const int nseg2 = max(na, 1);
       if (live && OL > OLC) {
          defer = true;
          live = false;
       }
+      // term q's entry (lane) and source, tabulated by the entry's lane:
+      // st_k holds the lane, the term space as ints the OC record's index
+      // (-1: the entry's own C-hat weight); both are read before the terms
+      // are staged over them
+      wave_sync();
+      if (live) {
+         for (int t = 0; t < olen; t++) {
+            st_k[ooff + t] = gl;
+            ctab[ooff + t] = (sk == 1) ? -1 : ob + t;
+         }
+      }
+      wave_sync();
       int tg[3], ts[3], tcol[3], tk[3];
       bool dr[3];
       double ta[3];
@@ -2703,26 +2714,26 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(5))) void 
          dr[u] = false;
          ta[u] = 0.0;
          if (!live || u * G >= OL) { continue; }   // uniform within the group
-         const int s = group_seg_find<G>(ooff, nseg2, q);
-         const int sk_s = __shfl(sk, s, G);
-         const int ob_s = __shfl(ob, s, G);
-         const int oo_s = __shfl(ooff, s, G);
+         const bool in = q < OL;
+         const int s = in ? st_k[q] : 0;
+         const int e = in ? ctab[q] : -1;
          const int v_s = __shfl(v, s, G);
          const double aij_s = __shfl(aij, s, G);
          ts[u] = s;
-         if (q < OL) {
-            if (sk_s == 1) {
+         if (in) {
+            if (e < 0) {
                dr[u] = true;
                tg[u] = v_s;
                ta[u] = aij_s;
             } else {
-               const OCEntry en = p.OC[ob_s + (q - oo_s)];
+               const OCEntry en = p.OC[e];
                tcol[u] = en.j;
                tk[u] = en.k;
                ta[u] = en.a;
             }
          }
       }
+      wave_sync();   // the tables are read before the terms are staged
 #pragma unroll
       for (int u = 0; u < 3; u++) {
          const int q = u * G + gl;
