@@ -2088,6 +2088,15 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
          const int L = readlane_i(incl, WAVE - 1);
          const int off = incl - klen;
          const int nseg = max(ns, 1);
+         // up to 2 XR_OL candidates: candidate q's source tabulated by its
+         // segment's lane in the term space (free until the terms are
+         // staged): ~column for a strong C neighbour, else its position in
+         // SC_ja; longer lists search the segment offsets
+         int *ctab = reinterpret_cast<int *>(st_a);
+         const bool ctab_on = L <= 2 * XR_OL;
+         if (ctab_on) {
+            for (int t = 0; t < klen; t++) { ctab[off + t] = (kind == 1) ? ~kb : kb + t; }
+         }
          // with the first batch: the diagonal of each neighbour row c1 (its
          // sign) and the entry at the mirror of c1's position in row i, where
          // a symmetric stencil stored in a symmetric order keeps column i
@@ -2111,7 +2120,12 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                const int c0 = cb + u * WAVE;   // uniform: the chunk's first position
                const int q = c0 + lane;
                key[u] = -1;
-               if (c0 < L) {
+               if (ctab_on) {
+                  if (q < L) {
+                     const int e = ctab[q];
+                     key[u] = (e < 0) ? ~e : p.SC_ja[e];
+                  }
+               } else if (c0 < L) {
                   const int sa = seg_find_lanes(off, nseg, q);
                   const int kind_s = __shfl(kind, sa, WAVE);
                   const int b_s = __shfl(kb, sa, WAVE);
@@ -2218,9 +2232,17 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
             const int oincl = wave_incl_scan(olen, lane);
             const int OL = readlane_i(oincl, WAVE - 1);
             const int ooff = oincl - olen;
-            const int nseg2 = max(na, 1);
             if (OL > XR_OL) { defer = true; }
             if (!defer) {
+               // term q's entry (lane) in st_k and its OC record's index in
+               // the term space as ints (-1: the entry's own C-hat weight),
+               // read before the terms are staged over them
+               wave_sync();
+               for (int t = 0; t < olen; t++) {
+                  st_k[ooff + t] = lane;
+                  ctab[ooff + t] = (sk == 1) ? -1 : ob + t;
+               }
+               wave_sync();
                int tg[3], ts[3], tcol[3], tk[3];
                bool dr[3];
                double ta[3];
@@ -2235,26 +2257,26 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
                   dr[u] = false;
                   ta[u] = 0.0;
                   if (u * WAVE >= OL) { continue; }   // uniform
-                  const int s = seg_find_lanes(ooff, nseg2, q);
-                  const int sk_s = __shfl(sk, s, WAVE);
-                  const int ob_s = __shfl(ob, s, WAVE);
-                  const int oo_s = __shfl(ooff, s, WAVE);
+                  const bool in = q < OL;
+                  const int s = in ? st_k[q] : 0;
+                  const int e = in ? ctab[q] : -1;
                   const int v_s = __shfl(v, s, WAVE);
                   const double aij_s = shfl_d(aij, s);
                   ts[u] = s;
-                  if (q < OL) {
-                     if (sk_s == 1) {
+                  if (in) {
+                     if (e < 0) {
                         dr[u] = true;
                         tg[u] = v_s;
                         ta[u] = aij_s;
                      } else {
-                        const OCEntry en = p.OC[ob_s + (q - oo_s)];
+                        const OCEntry en = p.OC[e];
                         tcol[u] = en.j;
                         tk[u] = en.k;
                         ta[u] = en.a;
                      }
                   }
                }
+               wave_sync();   // the tables are read before the terms are staged
 #pragma unroll
                for (int u = 0; u < 3; u++) {
                   const int q = u * WAVE + lane;
