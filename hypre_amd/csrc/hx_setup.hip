@@ -1727,6 +1727,15 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
          const int incl = wave_incl_scan(klen, lane);
          const int L = readlane_i(incl, WAVE - 1);
          const int off = incl - klen;
+         // up to 2 QC candidates: candidate q's source tabulated by its
+         // segment's lane in the term space (free until pass B): ~column for
+         // a strong C neighbour, else its position in SC_ja; longer lists
+         // search the segment offsets
+         int *ctab = reinterpret_cast<int *>(q_a);
+         const bool ctab_on = L <= 2 * QC;
+         if (ctab_on) {
+            for (int t = 0; t < klen; t++) { ctab[off + t] = (kind == 1) ? ~kb : kb + t; }
+         }
          wave_sync();
          // ---- C-hat in first-seen order.  vals[] of a C key holds the
          // smallest position seen (>= 64 after the first chunk) until the key
@@ -1738,16 +1747,23 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
 #pragma unroll
             for (int u = 0; u < XF_U; u++) {
                const int q = cb + u * WAVE + lane;
-               const int s = seg_find_lanes(off, nseg, q);
-               const int kind_s = __shfl(kind, s, WAVE);
-               const int b_s = __shfl(kb, s, WAVE);
-               const int off_s = __shfl(off, s, WAVE);
                key[u] = -1;
-               if (q < L) {
-                  if (kind_s == 1) {
-                     key[u] = b_s;
-                  } else if (kind_s == 2) {
-                     key[u] = p.SC_ja[b_s + (q - off_s)];
+               if (ctab_on) {
+                  if (q < L) {
+                     const int e = ctab[q];
+                     key[u] = (e < 0) ? ~e : p.SC_ja[e];
+                  }
+               } else {
+                  const int s = seg_find_lanes(off, nseg, q);
+                  const int kind_s = __shfl(kind, s, WAVE);
+                  const int b_s = __shfl(kb, s, WAVE);
+                  const int off_s = __shfl(off, s, WAVE);
+                  if (q < L) {
+                     if (kind_s == 1) {
+                        key[u] = b_s;
+                     } else if (kind_s == 2) {
+                        key[u] = p.SC_ja[b_s + (q - off_s)];
+                     }
                   }
                }
             }
