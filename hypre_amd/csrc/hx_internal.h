@@ -169,12 +169,11 @@ struct Csr
    void      alloc(int n, int nc, long long nz, bool values = true);
 };
 
-// Builds E from A when the padding stays within max_pad (stored entries <=
-// (1 + max_pad) nnz) and A has at least min_rows rows; returns whether it did.
-// fp32: also the FP32 values.
-bool ell_build(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32);
-// ell_build in two parts: ell_plan sizes and allocates the copy on the library
-// stream (false: no copy for this operator), ell_fill launches the copy on s
+// The sliced-ELL copy E of A, built when the padding stays within max_pad
+// (stored entries <= (1 + max_pad) nnz) and A has at least min_rows rows
+// (fp32: also the FP32 values), in two parts: ell_plan sizes and allocates
+// the copy on the library stream (false: no copy for this operator),
+// ell_fill launches the copy on s
 bool ell_plan(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32);
 void ell_fill(const Csr &A, Ell &E, bool fp32, hipStream_t s);
 

@@ -4063,7 +4063,7 @@ static size_t rap_seg_lds(int rc, int rcl, int ac, int acl)
 }
 
 template <int RAREG, bool MULTI>
-__global__ __launch_bounds__(WAVE) void k_rap_seg(RapFastArgs p, int wsh, int wsa)
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu((RAREG <= RAP_RAREG && !MULTI) ? 5 : 1))) void k_rap_seg(RapFastArgs p, int wsh, int wsa)
 {
    extern __shared__ double dyn_lds[];
    const int lane = threadIdx.x;
@@ -4360,17 +4360,28 @@ __global__ __launch_bounds__(WAVE) void k_rap_seg(RapFastArgs p, int wsh, int ws
    }
 }
 
-// copy the rows finished by the fast stages into Ac (16 lanes per row)
-__global__ void k_rap_compact(int nc, const int *__restrict__ cnt, const int *__restrict__ src,
-                              const int *__restrict__ out_ia, const int *__restrict__ tj0,
-                              const double *__restrict__ ta0, const int *__restrict__ tj1,
-                              const double *__restrict__ ta1, const int *__restrict__ tj2,
-                              const double *__restrict__ ta2, int *__restrict__ oj,
-                              double *__restrict__ oa)
+// copy the rows finished by the fast stages into Ac: K lanes per row, the
+// loads of RC_U entries per lane issued before their stores
+constexpr int RC_U = 4;
+
+template <int K>
+__global__ __launch_bounds__(256) void k_rap_compact(int nc, const int *__restrict__ cnt,
+                                                     const int *__restrict__ src,
+                                                     const int *__restrict__ out_ia,
+                                                     const int *__restrict__ tj0,
+                                                     const double *__restrict__ ta0,
+                                                     const int *__restrict__ tj1,
+                                                     const double *__restrict__ ta1,
+                                                     const int *__restrict__ tj2,
+                                                     const double *__restrict__ ta2,
+                                                     int *__restrict__ oj, double *__restrict__ oa)
 {
-   const int g = threadIdx.x & 15;
-   for (long long r = ((long long) blockIdx.x * blockDim.x + threadIdx.x) >> 4; r < nc;
-        r += ((long long) gridDim.x * blockDim.x) >> 4) {
+   const int g = threadIdx.x & (K - 1);
+   const int rows_per_block = 256 / K;
+   for (long long rb = (long long) blockIdx.x * rows_per_block; rb < nc;
+        rb += (long long) gridDim.x * rows_per_block) {
+      const long long r = rb + threadIdx.x / K;
+      if (r >= nc) { continue; }
       const int s = src[r];
       if (s == -1) { continue; }
       const int c = cnt[r];
@@ -4379,9 +4390,25 @@ __global__ void k_rap_compact(int nc, const int *__restrict__ cnt, const int *__
       const int *tj = later == 0 ? tj0 : (later == 1 ? tj1 : tj2);
       const double *ta = later == 0 ? ta0 : (later == 1 ? ta1 : ta2);
       const size_t b = s >= 0 ? (size_t) s : (size_t) ((-2 - s) >> 1);
-      for (int k = g; k < c; k += 16) {
-         oj[o + k] = tj[b + k];
-         oa[o + k] = ta[b + k];
+      for (int k0 = g; k0 < c; k0 += K * RC_U) {
+         int jv[RC_U];
+         double av[RC_U];
+#pragma unroll
+         for (int u = 0; u < RC_U; u++) {
+            const int k = k0 + u * K;
+            if (k < c) {
+               jv[u] = tj[b + k];
+               av[u] = ta[b + k];
+            }
+         }
+#pragma unroll
+         for (int u = 0; u < RC_U; u++) {
+            const int k = k0 + u * K;
+            if (k < c) {
+               oj[o + k] = jv[u];
+               oa[o + k] = av[u];
+            }
+         }
       }
    }
 }
@@ -4499,10 +4526,15 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac, int p_row_bound, int
    Ac.nnz = exclusive_scan(cnt, Ac.ia, nc, ac_max_row);
    Ac.ja = dalloc_n<int>((size_t) std::max<long long>(Ac.nnz, 1));
    Ac.a = dalloc_n<double>((size_t) std::max<long long>(Ac.nnz, 1));
-   int nbc = (int) std::max<long long>(1, std::min<long long>(((long long) nc * 16 + 255) / 256, 16384));
-   k_rap_compact<<<nbc, 256, 0, stream()>>>(nc, cnt, src, Ac.ia, tj[0], ta[0], tj[1], ta[1], tj[2],
-                                            ta[2], Ac.ja, Ac.a);
-   HX_LAUNCH_CHECK();
+   {
+      const int K = lanes_per_row(Ac.nnz, nc);
+      const int nbc = (int) std::max<long long>(1, std::min<long long>(((long long) nc * K + 255) / 256, 16384));
+      dispatch_k(K, [&](auto kc) {
+         k_rap_compact<decltype(kc)::value><<<nbc, 256, 0, stream()>>>(
+             nc, cnt, src, Ac.ia, tj[0], ta[0], tj[1], ta[1], tj[2], ta[2], Ac.ja, Ac.a);
+      });
+      HX_LAUNCH_CHECK();
+   }
    if (nwork > 0) {
       a.out_ia = Ac.ia;
       a.out_j = Ac.ja;
