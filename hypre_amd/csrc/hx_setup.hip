@@ -3154,7 +3154,10 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
       int nwork = n;
       // passes over the rows left by the previous one: two rows per wave (rows
       // of <= 32 entries; tried where the mean row is that short), one row
-      // per wave (C-hat <= 32 points), then C-hat <= 64 points
+      // per wave (C-hat <= 32 points; tried where the mean row has at most
+      // 64 entries: on longer rows most C-hats exceed 32 and the pass only
+      // defers them, e.g. 82% of the third level of the 27-point problem),
+      // then C-hat <= 64 points
       int kinds[3];
       int nk = 0;
       const double mean_row = (double) A.nnz / std::max(n, 1);
@@ -3163,7 +3166,7 @@ void extpi_interp(const Csr &A, const Csr &S, int *CF, const int *f2c, int nc,
       } else if (mean_row <= 28.0) {
          kinds[nk++] = 0;   // two rows per wave
       }
-      kinds[nk++] = 1;
+      if (mean_row <= 64.0) { kinds[nk++] = 1; }
       kinds[nk++] = 2;
       for (int pass = 0; pass < nk && nwork > 0; pass++) {
          const int kind = kinds[pass];
