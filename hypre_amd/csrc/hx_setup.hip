@@ -4472,7 +4472,11 @@ void rap(const Csr &R, const Csr &A, const Csr &P, Csr &Ac, int p_row_bound, int
          }
       }
    }
-   for (int stage = 0; stage < 3 && nwork > 0; stage++) {
+   // where A's mean row has more than 64 entries, most RA rows overflow stage
+   // 0's tables (two thirds at the third level of the 27-point problem) and
+   // the stage would only defer them
+   const int first_stage = ((double) A.nnz / std::max(A.nrows, 1) > 64.0) ? 1 : 0;
+   for (int stage = first_stage; stage < 3 && nwork > 0; stage++) {
       // offsets are kept as int (times 2 after stage 0)
       if ((long long) nwork * acls[stage] * 2 >= INT_MAX) { break; }
       tj[stage] = dalloc_n<int>((size_t) nwork * acls[stage]);
