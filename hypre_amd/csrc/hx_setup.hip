@@ -386,15 +386,32 @@ __global__ __launch_bounds__(256) void k_strength_fill(int n, const int *__restr
    }
    const long long r_begin = (long long) blockIdx.x * SF_ROWS;
    const long long r_end = min((long long) n, r_begin + SF_ROWS);
-   for (long long rb = r_begin; rb < r_end; rb += rows_per_iter) {
+   // a row's own operands are loaded one iteration ahead, while the rows
+   // before it run
+   int n_s = 0, n_e = 0, n_fl = 2, n_o = 0;
+   double n_t = 0.0;
+   auto load_row = [&](long long rb) {
       const long long ir = rb + threadIdx.x / K;
-      const bool act = ir < r_end;
-      const int i = act ? (int) ir : 0;
-      const int s = act ? ia[i] : 0, e = act ? ia[i + 1] : 0;
-      const double t = act ? thr[i] : 0.0;
-      const int fl = act ? flg[i] : 2;
+      n_s = n_e = n_o = 0;
+      n_fl = 2;
+      n_t = 0.0;
+      if (ir < r_end) {
+         const int i = (int) ir;
+         n_s = ia[i];
+         n_e = ia[i + 1];
+         n_t = thr[i];
+         n_fl = flg[i];
+         n_o = S_ia[i];
+      }
+   };
+   if (r_begin < r_end) { load_row(r_begin); }
+   for (long long rb = r_begin; rb < r_end; rb += rows_per_iter) {
+      const int s = n_s, e = n_e;
+      const double t = n_t;
+      const int fl = n_fl;
       const bool neg = fl & 1, allweak = fl & 2;
-      const int o = act ? S_ia[i] : 0;
+      const int o = n_o;
+      if (rb + rows_per_iter < r_end) { load_row(rb + rows_per_iter); }
       int c = 0;
       // SF_U strided steps of the row per batch: every load of a batch is
       // issued before any is used (positions past the row end read its last
@@ -600,15 +617,35 @@ __global__ __launch_bounds__(256) void k_c_lists_t(int n, const int *__restrict_
    const unsigned long long below = (1ULL << gl) - 1;
    const long long r_begin = (long long) blockIdx.x * SF_ROWS;
    const long long r_end = min((long long) n, r_begin + SF_ROWS);
+   // a row's own operands are loaded one iteration ahead, while the rows
+   // before it run
+   int n_s = 0, n_e = 0, n_fl = 2, n_so = 0;
+   double n_t = 0.0;
+   auto load_row = [&](long long rb) {
+      const long long ir = rb + threadIdx.x / K;
+      n_s = n_e = n_so = 0;
+      n_fl = 2;
+      n_t = 0.0;
+      if (ir < r_end) {
+         const int i = (int) ir;
+         n_s = ia[i];
+         n_e = ia[i + 1];
+         n_t = thr[i];
+         n_fl = flg[i];
+         n_so = S_ia[i];
+      }
+   };
+   if (r_begin < r_end) { load_row(r_begin); }
    for (long long rb = r_begin; rb < r_end; rb += rows_per_iter) {
       const long long ir = rb + threadIdx.x / K;
       const bool act = ir < r_end;
       const int i = act ? (int) ir : 0;
-      const int s = act ? ia[i] : 0, e = act ? ia[i + 1] : 0;
-      const double t = act ? thr[i] : 0.0;
-      const int fl = act ? flg[i] : 2;
+      const int s = n_s, e = n_e;
+      const double t = n_t;
+      const int fl = n_fl;
       const bool neg = fl & 1, allweak = fl & 2;
-      const int so = act ? S_ia[i] : 0;
+      const int so = n_so;
+      if (rb + rows_per_iter < r_end) { load_row(rb + rows_per_iter); }
       // positions past the row end read its last entry and are masked (a
       // batch is only entered when some row of the wave has an entry left)
       const int klast = max(e - 1, 0);
