@@ -579,6 +579,8 @@ __global__ __launch_bounds__(64) void k_ell_fill(int n, int ns, const int *__res
    }
 }
 
+constexpr int EP_U = 9;
+
 // per slice: the least and the largest col - row of each step over the rows
 // that have an entry there; a slice whose steps all span at most 255 (65535)
 // reads 8-bit (16-bit) offsets from its steps' least values, mode 2 (1)
@@ -599,20 +601,30 @@ __global__ __launch_bounds__(64) void k_ell_pack(int n, int ns, const int *__res
       unsigned short *e16s = e16 + (size_t) w0 * 64 + lane;
       unsigned char *e8s = e8 + (size_t) w0 * 64 + lane;
       long long span = 0;
-      for (int k = 0; k < w; k++) {
-         const bool in = k < len;
-         const int off = ejs[(size_t) k * 64] - r;   // both in [0, 2^31)
-         int lo = in ? off : INT_MAX, hi = in ? off : INT_MIN;
-         for (int d = 32; d > 0; d >>= 1) {
-            lo = min(lo, __shfl_xor(lo, d, 64));
-            hi = max(hi, __shfl_xor(hi, d, 64));
+      // the columns of EP_U steps are loaded before any is reduced (steps
+      // past the slice width re-read its last step and are not used)
+      for (int k0 = 0; k0 < w; k0 += EP_U) {
+         int cv[EP_U];
+#pragma unroll
+         for (int u = 0; u < EP_U; u++) { cv[u] = ejs[(size_t) min(k0 + u, w - 1) * 64]; }
+#pragma unroll
+         for (int u = 0; u < EP_U; u++) {
+            const int k = k0 + u;
+            if (k >= w) { break; }   // uniform
+            const bool in = k < len;
+            const int off = cv[u] - r;   // both in [0, 2^31)
+            int lo = in ? off : INT_MAX, hi = in ? off : INT_MIN;
+            for (int d = 32; d > 0; d >>= 1) {
+               lo = min(lo, __shfl_xor(lo, d, 64));
+               hi = max(hi, __shfl_xor(hi, d, 64));
+            }
+            // some row of the slice has step k (w is its longest row)
+            span = max(span, (long long) hi - (long long) lo);
+            if (lane == 0) { jb[w0 + k] = lo; }
+            const unsigned d = in ? (unsigned) (off - lo) : 0u;   // truncated where unused
+            e16s[(size_t) k * 64] = (unsigned short) d;
+            e8s[(size_t) k * 64] = (unsigned char) d;
          }
-         // some row of the slice has step k (w is its longest row)
-         span = max(span, (long long) hi - (long long) lo);
-         if (lane == 0) { jb[w0 + k] = lo; }
-         const unsigned d = in ? (unsigned) (off - lo) : 0u;   // truncated where unused
-         e16s[(size_t) k * 64] = (unsigned short) d;
-         e8s[(size_t) k * 64] = (unsigned char) d;
       }
       if (lane == 0) { mode[sl] = span <= 255 ? 2 : (span <= 65535 ? 1 : 0); }
    }
