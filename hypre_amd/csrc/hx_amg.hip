@@ -217,6 +217,8 @@ void AMGSolver::setup(const Csr &A0)
       Level &nx = L[level + 1];
       Level &cur = L[level];
       start_ell(level);
+      // the copy's plan found the level operator's longest row
+      if (a_max_row <= 0 && cur.ell.nslices > 0) { a_max_row = cur.ell.wmax; }
       // P's rows: one entry for a C point, at most P_max_elmts otherwise
       const int pmx = aggressive ? p.agg_P_max_elmts : p.P_max_elmts;
       {

@@ -151,6 +151,7 @@ struct Ell
    unsigned char  *j8 = nullptr;  // 8-bit column offsets (slices with mode 2)
    int      *jb = nullptr;        // per slice step: least col - row (sum of widths)
    int      *mode = nullptr;      // per slice: 2 = 8-bit offsets, 1 = 16-bit, 0 = 32-bit columns
+   int       wmax = 0;            // widest slice: the operator's longest row
    double   *a = nullptr;
    float    *a32 = nullptr;
    void      free_all();

@@ -581,7 +581,7 @@ void Ell::free_all()
 {
    dfree(sw); dfree(j); dfree(j16); dfree(j8); dfree(jb); dfree(mode); dfree(a); dfree(a32);
    sw = j = nullptr; j16 = nullptr; j8 = nullptr; jb = mode = nullptr; a = nullptr; a32 = nullptr;
-   nslices = 0; stored = 0;
+   nslices = 0; stored = 0; wmax = 0;
 }
 
 void Csr::alloc(int n, int nc, long long nz, bool values)

@@ -641,7 +641,8 @@ bool ell_plan(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32)
    int *w = dalloc_n<int>((size_t) ns + 1);
    k_ell_width<<<(ns + BS - 1) / BS, BS, 0, stream()>>>(A.nrows, ns, A.ia, w);
    HX_LAUNCH_CHECK();
-   const long long tot = exclusive_scan(w, w, ns);
+   int wmax = 0;
+   const long long tot = exclusive_scan(w, w, ns, &wmax);
    const long long stored = 64LL * tot;
    if ((double) stored > (1.0 + max_pad) * (double) A.nnz || stored > 0x7fffffffLL) {
       dfree(w);
@@ -649,6 +650,7 @@ bool ell_plan(const Csr &A, Ell &E, double max_pad, int min_rows, bool fp32)
    }
    E.nslices = ns;
    E.stored = stored;
+   E.wmax = wmax;
    E.sw = w;
    E.j = dalloc_n<int>((size_t) stored);
    E.a = dalloc_n<double>((size_t) stored);
