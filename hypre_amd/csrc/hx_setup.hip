@@ -44,6 +44,19 @@ __device__ __forceinline__ int readlane_i(int v, int src) { return __builtin_amd
 
 constexpr __host__ __device__ int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
 
+// Slot of column c in an LDS table of 2^bits slots for columns that come in
+// runs of consecutive indices: a run takes consecutive slots (its flushes add
+// to consecutive counters), and each 2^bits-aligned block of columns is
+// rotated by 3077 slots, so runs a multiple of the table size apart (grid
+// planes of 320^2 = 25 * 4096 points, say) do not pile onto one slot.  With
+// 3077 no two columns of a 256-row block of the 7- or 27-point operators of
+// the 200^3 and 320^3 grids share a slot (checked on the host for the
+// rotations tried; the low bits alone collide on 2/3 of the keys at 320^3).
+__device__ __forceinline__ unsigned run_slot(unsigned c, int bits)
+{
+   return (c + (c >> bits) * 3077u) & ((1u << bits) - 1u);
+}
+
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }   // blocks are one wave
 
 static int launch_blocks(long long work_items)
@@ -440,12 +453,12 @@ __global__ __launch_bounds__(256) void k_strength_fill(int n, const int *__restr
                S_ja[o + c + __popcll(m & ((1ULL << gl) - 1))] = j;
                if (stc) {
                   // RUNS (a fine-grid stencil operator in its natural order): a
-                  // block's columns are a few runs of consecutive indices; keyed
-                  // by the column's low bits they fill consecutive slots, and the
-                  // flush below adds to consecutive counters (one memory
-                  // transaction for many lanes).  Otherwise (coarse operators,
+                  // block's columns are a few runs of consecutive indices; by
+                  // run_slot they fill consecutive slots, and the flush below
+                  // adds to consecutive counters (one memory transaction for
+                  // many lanes).  Otherwise (coarse operators,
                   // scattered columns) a multiplicative hash spreads them.
-                  unsigned h = RUNS ? ((unsigned) j & (SF_SLOTS - 1))
+                  unsigned h = RUNS ? run_slot((unsigned) j, ilog2(SF_SLOTS))
                                     : (((unsigned) j * 2654435761u) >> (32 - ilog2(SF_SLOTS)));
                   bool done = false;
                   for (int probe = 0; probe < 32; probe++) {
@@ -3377,12 +3390,12 @@ constexpr int TF_SLOTS = 4096;
 constexpr int TF_E = 8;
 constexpr int TC_U = 8;
 
-// the column's low bits: the columns of a chunk are runs of consecutive
-// coarse indices, so they take consecutive slots and the flushes add to
-// consecutive counters (one memory transaction for many lanes)
+// the columns of a chunk are runs of consecutive coarse indices, so they take
+// consecutive slots and the flushes add to consecutive counters (one memory
+// transaction for many lanes)
 __device__ __forceinline__ unsigned tf_hash(int c)
 {
-   return (unsigned) c & (TF_SLOTS - 1);
+   return run_slot((unsigned) c, ilog2(TF_SLOTS));
 }
 
 __global__ __launch_bounds__(256) void k_col_count_h(long long nnz, const int *__restrict__ ja,
